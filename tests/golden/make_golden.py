@@ -1,0 +1,118 @@
+"""Writes the golden fixtures in tests/golden/ from the reference implementation.
+
+Runs oracle/_ref/ref_harness (built by ``make -C oracle ref`` from the
+unmodified reference sources, numTh == 1) on the inputs in tests/golden/data/
+and stores its 17-digit JSON output, gzip-compressed, one file per case.
+Run in the development container (the reference is not present elsewhere):
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+The config-1 stochastic block model input (sbm_1k_c25_s1.txt) is generated
+here with numpy and committed as data.
+"""
+import gzip
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DATA = os.path.join(HERE, "data")
+ROOT = os.path.dirname(os.path.dirname(HERE))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def harness(*args):
+    out = subprocess.check_output([HARNESS] + [str(a) for a in args])
+    return json.loads(out)
+
+
+def save(name, obj):
+    with gzip.open(os.path.join(HERE, name + ".json.gz"), "wt") as f:
+        json.dump(obj, f)
+
+
+def sbm_config1(path, n=1000, blocks=25, mean_deg=15, mu=0.1, seed=1):
+    """Directed SBM: n nodes in equal blocks, out-degree ~ Poisson(mean_deg),
+    each target in-block w.p. 1-mu else uniform outside, distinct targets,
+    no self-loops, unit weights; written 1-based as an edge list."""
+    rng = np.random.default_rng(seed)
+    size = n // blocks
+    block = np.arange(n) // size
+    lines = []
+    for a in range(n):
+        k = min(int(rng.poisson(mean_deg)), n - 1)
+        tg = set()
+        while len(tg) < k:
+            if rng.random() < 1.0 - mu:
+                b = int(block[a] * size + rng.integers(size))
+            else:
+                b = int(rng.integers(n - size))
+                if b >= block[a] * size:
+                    b += size
+            if b != a:
+                tg.add(b)
+        for b in sorted(tg):
+            lines.append(f"{a + 1}\t{b + 1}\n")
+    with open(path, "w") as f:
+        f.writelines(lines)
+    with open(path.replace(".txt", "_community.dat"), "w") as f:
+        for a in range(n):
+            f.write(f"{a + 1}\t{block[a] + 1}\n")
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference harness first: make -C oracle ref")
+    sbm = os.path.join(DATA, "sbm_1k_c25_s1.txt")
+    if not os.path.exists(sbm):
+        sbm_config1(sbm)
+    tmp = tempfile.mkdtemp()
+    try:
+        five_k = os.path.join(tmp, "directNet_5k_sm_mu_05.txt")
+        with gzip.open(os.path.join(DATA, "directNet_5k_sm_mu_05.txt.gz"), "rb") as fi, open(five_k, "wb") as fo:
+            shutil.copyfileobj(fi, fo)
+        inputs = {
+            "flow_net": os.path.join(DATA, "flow.net"),
+            "flow_txt": os.path.join(DATA, "flow.txt"),
+            "lfr1k_mu03": os.path.join(DATA, "directNet_1k_sm_mu_03.txt"),
+            "lfr5k_mu05": five_k,
+            "sbm1k": sbm,
+        }
+        for key, path in inputs.items():
+            save(f"{key}.flows", harness("flows", path))
+            variants = ["move", "prior", "pmove", "ppmove"] if key != "lfr5k_mu05" else ["move", "ppmove"]
+            for v in variants:
+                save(f"{key}.sweeps_{v}_s7", harness("sweeps", path, 7, 4, v))
+            if key == "lfr5k_mu05":
+                # contraction fixtures of the 5k graph would be ~1.5 MB;
+                # the smaller inputs cover contraction
+                for mode in ("prior", "normal"):
+                    save(f"{key}.run_{mode}_s1", harness("run", path, 1, 1e-3, 0, 10, mode))
+                continue
+            # fixed partitions: the state after the 'move' sweeps, and a
+            # random partition into n/20 labels
+            part_swept = harness("sweeps", path, 3, 3, "move")["sweeps"][-1]["partition"]
+            n = len(part_swept)
+            rng = np.random.default_rng(11)
+            part_rand = rng.integers(0, max(2, n // 20), n).tolist()
+            for pname, part in (("swept", part_swept), ("random", part_rand)):
+                pf = os.path.join(tmp, "part.txt")
+                np.savetxt(pf, np.array(part), fmt="%d")
+                obj = {"partition": part, "codelength": harness("codelength", path, pf),
+                       "contract": harness("contract", path, pf)}
+                for v in ("move", "ppmove"):
+                    obj[f"spsweeps_{v}"] = harness("spsweeps", path, pf, 11, 3, v)
+                save(f"{key}.part_{pname}", obj)
+            for mode in ("prior", "normal"):
+                save(f"{key}.run_{mode}_s1", harness("run", path, 1, 1e-3, 0, 10, mode))
+    finally:
+        shutil.rmtree(tmp)
+
+
+if __name__ == "__main__":
+    main()
