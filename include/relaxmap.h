@@ -1,0 +1,150 @@
+/* relaxmap_amd: C ABI of the MI355X (gfx950) RelaxMap hot path.
+ *
+ * One rmx_ctx holds one network resident on one GPU, driven by one HIP
+ * stream; a context is not thread-safe.  Every call returns 0 on success
+ * or a negative status (RMX_E*); rmx_last_error() describes the failure.
+ * No C++ exception crosses this boundary.  Arrays are host pointers owned
+ * by the caller; ids are 0-based.
+ *
+ * The reference exposes this path only as member functions of its C++
+ * `Network` class (uwescience/RelaxMap Module.h:176-199), called from
+ * OmpRelaxmap.cpp.  Each entry point below names the reference function it
+ * replaces.
+ */
+#ifndef RELAXMAP_H
+#define RELAXMAP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RMX_OK 0
+#define RMX_EINVAL (-22)   /* bad argument or malformed input */
+#define RMX_ESTATE (-1)    /* call out of order (e.g. sweep before initiate) */
+#define RMX_EDEVICE (-19)  /* no usable GPU, or a HIP runtime error */
+#define RMX_EIO (-5)       /* file could not be read or written */
+
+#define RMX_FP64 0         /* FP64 throughout */
+#define RMX_MIXED 1        /* FP32 arc weights / SpMV bulk, FP64 accumulation */
+
+typedef struct rmx_ctx rmx_ctx;
+
+/* Number of visible GPUs (0 if none or no HIP runtime). */
+int rmx_device_count(void);
+
+/* Creates a context on `device`.  No reference counterpart (the reference's
+ * state is the Network object itself, OmpRelaxmap.cpp:81). */
+int rmx_create(rmx_ctx **ctx, int device);
+void rmx_destroy(rmx_ctx *ctx);
+const char *rmx_last_error(const rmx_ctx *ctx);
+
+/* Precision of the steady state and of the sweeps; call before
+ * rmx_initiate.  Sub-sweeps per sweep (default 8). */
+int rmx_set_precision(rmx_ctx *ctx, int precision);
+int rmx_set_batches(rmx_ctx *ctx, int batches);
+
+/* Network construction: replaces the arc-list build of OmpRelaxmap.cpp:113-145
+ * (after load_pajek_format_network / load_linkList_format_network,
+ * FileIO.cpp:32-265).  out_ptr[n+1] / out_idx[m] / w[m] is the out-CSR with
+ * arcs sorted by (src, dst), no self-loops, no duplicates, w > 0; node_w[n]
+ * are node (teleport) weights, NULL for 1.0.  m < 2^32 - 1. */
+int rmx_graph_upload(rmx_ctx *ctx, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx,
+                     const double *w, const double *node_w);
+
+/* Reads a .net (Pajek) or edge-list .txt file with the reference's parsing
+ * rules (FileIO.cpp:32-265) and uploads it (rmx_graph_upload).  Node names
+ * are kept on the host for the output writers. */
+int rmx_load_network(rmx_ctx *ctx, const char *path, int32_t *n, int64_t *m, int32_t *n_self_links,
+                     int32_t *n_double_links);
+
+/* Network::initiate (Module.cpp:210-333): dangling nodes, row
+ * normalisation, steady state (Network::calculateSteadyState,
+ * Module.cpp:340-469: at least 50, at most 200 iterations, L1 change
+ * <= 1e-15), arc flows, singleton modules, calibrate (Module.cpp:474-501).
+ * Outputs may be NULL.  *codelength is in nats. */
+int rmx_initiate(rmx_ctx *ctx, int32_t *iterations, double *codelength, int64_t *n_dangling);
+/* Steady-state node flows p (Node::Size after initiate). */
+int rmx_get_flows(rmx_ctx *ctx, double *p);
+/* Singleton exit probabilities (Module::Module(int, Node*), Module.cpp:86-95). */
+int rmx_get_node_exit(rmx_ctx *ctx, double *q);
+
+/* Tracked codelength, sum of module exits and module count of the current
+ * partition (Network::CodeLength / SumAllExitPr / NModule). */
+int rmx_codelength(rmx_ctx *ctx, double *codelength, double *sum_exit, int32_t *n_module);
+/* Network::calculateCodeLength (Module.cpp:2966-3002): from scratch, FP64. */
+int rmx_calculate_codelength(rmx_ctx *ctx, double *codelength);
+
+/* Installs a node partition (module index per node, 0..n-1) with freshly
+ * summed module statistics (as Network::updateMembersInModule +
+ * updateCodeLength, Module.cpp:2881-2962, do for a given assignment). */
+int rmx_set_partition(rmx_ctx *ctx, const int32_t *part);
+/* Module index per node (Node::ModIdx). */
+int rmx_get_partition(rmx_ctx *ctx, int32_t *part);
+
+/* All units active again (OmpRelaxmap.cpp:282-288, 370-376). */
+int rmx_reset_active(rmx_ctx *ctx);
+/* One sweep over the current level's units: Network::prioritize_parallelMove
+ * / parallelMove (Module.cpp:962-1685) on nodes, and their super-node twins
+ * (Module.cpp:2154-2869) after rmx_contract.  vthresh is the CLI vThresh
+ * (moves need dL < -vthresh when prioritized, dL < 0 otherwise).  *moved
+ * counts original nodes moved.  Outputs may be NULL. */
+int rmx_move_sweep(rmx_ctx *ctx, int prioritized, double vthresh, uint64_t seed, int64_t *moved,
+                   double *codelength, int64_t *edge_visits);
+
+/* Network::updateMembersInModule (Module.cpp:2881-2902). */
+int rmx_update_members(rmx_ctx *ctx);
+/* Network::convertModulesToSuperNodes (Module.cpp:3008-3077): contracts the
+ * original arcs by the current node modules; later sweeps move super nodes. */
+int rmx_contract(rmx_ctx *ctx, int32_t *n_super, int64_t *m_super);
+/* Super-node statistics after rmx_contract (SuperNode(Module, int),
+ * Node.cpp:76-106): flow, exit, teleport, dangling flow, module index and
+ * member count.  Any pointer may be NULL. */
+int rmx_get_super(rmx_ctx *ctx, double *size, double *exit, double *tp, double *dangling, int32_t *mod,
+                  int32_t *n_members);
+/* Super-node out- and in-lists (SuperNode::outLinks / inLinks). */
+int rmx_get_super_arcs(rmx_ctx *ctx, int64_t *out_ptr, int32_t *out_idx, double *out_w, int64_t *in_ptr,
+                       int32_t *in_idx, double *in_w);
+/* Back to moving original nodes (node modules follow their super nodes). */
+int rmx_to_node_level(rmx_ctx *ctx);
+
+/* The CLI's optimisation: main()'s super-step loop (OmpRelaxmap.cpp:181-219)
+ * around stochastic_greedy_partition (OmpRelaxmap.cpp:263-429). */
+typedef struct {
+  double threshold;   /* stop when a super step gains less (bits) */
+  double vthresh;     /* CLI vThresh */
+  int32_t max_iter;   /* sweeps per phase */
+  int32_t prior;      /* 1: prioritized sweeps ("prior"), 0: "normal" */
+  uint64_t seed;
+  int32_t verbose;    /* 1: print the reference's progress lines to stdout */
+} rmx_partition_opts;
+
+typedef struct {
+  double codelength;          /* tracked, nats */
+  double computed_codelength; /* Network::calculateCodeLength, nats */
+  int32_t n_module;
+  int32_t super_steps;
+  int64_t sweeps;
+  int64_t edge_visits;
+  double time_partition_s;    /* sweeps + contraction (OmpRelaxmap.cpp:222) */
+  double time_sweeps_s;
+  double time_contract_s;
+} rmx_partition_result;
+
+int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_partition_result *res);
+
+/* Output writers: the .tree (OmpRelaxmap.cpp:790-812) and .clu
+ * (OmpRelaxmap.cpp:240-249) files of the current partition.  Node names
+ * come from rmx_load_network (decimal 1-based ids after rmx_graph_upload). */
+int rmx_write_tree(rmx_ctx *ctx, const char *path);
+int rmx_write_clu(rmx_ctx *ctx, const char *path);
+
+/* Kernel timings of the last rmx_initiate / rmx_partition (milliseconds). */
+int rmx_timing(rmx_ctx *ctx, double *flow_ms, double *sweep_ms, double *contract_ms, int64_t *edge_visits,
+               int64_t *sweeps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

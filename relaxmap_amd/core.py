@@ -1,0 +1,282 @@
+"""ctypes binding of librelaxmap.so (include/relaxmap.h).
+
+``Network`` mirrors the reference's C++ ``Network`` (uwescience/RelaxMap
+Module.h:99-200): ``initiate``, ``calculate_codelength``, ``move`` sweeps,
+``update_members_in_module``, ``convert_modules_to_super_nodes`` and the
+CLI's ``partition`` loop.  Every call goes to the GPU through the C ABI; a
+missing library or GPU raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_I32P = C.POINTER(C.c_int32)
+_I64P = C.POINTER(C.c_int64)
+_F64P = C.POINTER(C.c_double)
+
+RMX_FP64 = 0
+RMX_MIXED = 1
+
+
+def library_path():
+    return os.path.join(_HERE, "librelaxmap.so")
+
+
+class _Opts(C.Structure):
+    _fields_ = [("threshold", C.c_double), ("vthresh", C.c_double), ("max_iter", C.c_int32),
+                ("prior", C.c_int32), ("seed", C.c_uint64), ("verbose", C.c_int32)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("codelength", C.c_double), ("computed_codelength", C.c_double), ("n_module", C.c_int32),
+                ("super_steps", C.c_int32), ("sweeps", C.c_int64), ("edge_visits", C.c_int64),
+                ("time_partition_s", C.c_double), ("time_sweeps_s", C.c_double),
+                ("time_contract_s", C.c_double)]
+
+
+@dataclass
+class PartitionResult:
+    codelength: float
+    computed_codelength: float
+    n_module: int
+    super_steps: int
+    sweeps: int
+    edge_visits: int
+    time_partition_s: float
+    time_sweeps_s: float
+    time_contract_s: float
+
+
+def load_library():
+    """Loads librelaxmap.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run __graft_entry__.build() or make -C relaxmap_amd/csrc")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    sig = {
+        "rmx_device_count": (C.c_int, []),
+        "rmx_create": (C.c_int, [C.POINTER(vp), C.c_int]),
+        "rmx_destroy": (None, [vp]),
+        "rmx_last_error": (C.c_char_p, [vp]),
+        "rmx_set_precision": (C.c_int, [vp, C.c_int]),
+        "rmx_set_batches": (C.c_int, [vp, C.c_int]),
+        "rmx_graph_upload": (C.c_int, [vp, C.c_int32, C.c_int64, _I64P, _I32P, _F64P, _F64P]),
+        "rmx_load_network": (C.c_int, [vp, C.c_char_p, _I32P, _I64P, _I32P, _I32P]),
+        "rmx_initiate": (C.c_int, [vp, _I32P, _F64P, _I64P]),
+        "rmx_get_flows": (C.c_int, [vp, _F64P]),
+        "rmx_get_node_exit": (C.c_int, [vp, _F64P]),
+        "rmx_codelength": (C.c_int, [vp, _F64P, _F64P, _I32P]),
+        "rmx_calculate_codelength": (C.c_int, [vp, _F64P]),
+        "rmx_set_partition": (C.c_int, [vp, _I32P]),
+        "rmx_get_partition": (C.c_int, [vp, _I32P]),
+        "rmx_reset_active": (C.c_int, [vp]),
+        "rmx_move_sweep": (C.c_int, [vp, C.c_int, C.c_double, C.c_uint64, _I64P, _F64P, _I64P]),
+        "rmx_update_members": (C.c_int, [vp]),
+        "rmx_contract": (C.c_int, [vp, _I32P, _I64P]),
+        "rmx_get_super": (C.c_int, [vp, _F64P, _F64P, _F64P, _F64P, _I32P, _I32P]),
+        "rmx_get_super_arcs": (C.c_int, [vp, _I64P, _I32P, _F64P, _I64P, _I32P, _F64P]),
+        "rmx_to_node_level": (C.c_int, [vp]),
+        "rmx_partition": (C.c_int, [vp, C.POINTER(_Opts), C.POINTER(_Result)]),
+        "rmx_write_tree": (C.c_int, [vp, C.c_char_p]),
+        "rmx_write_clu": (C.c_int, [vp, C.c_char_p]),
+        "rmx_timing": (C.c_int, [vp, _F64P, _F64P, _F64P, _I64P, _I64P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def device_count():
+    return load_library().rmx_device_count()
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Network:
+    """One network resident on one GPU (the reference's ``Network``)."""
+
+    def __init__(self, device=0, precision="fp64", batches=8):
+        L = load_library()
+        self._L = L
+        h = C.c_void_p()
+        rc = L.rmx_create(C.byref(h), int(device))
+        if rc != 0:
+            raise RuntimeError(f"rmx_create failed with status {rc} (no usable GPU?)")
+        self._h = h
+        self.n = 0
+        self.m = 0
+        self._check(L.rmx_set_precision(h, RMX_MIXED if precision == "mixed" else RMX_FP64))
+        self._check(L.rmx_set_batches(h, int(batches)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.rmx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._L.rmx_last_error(self._h)
+            raise RuntimeError(f"relaxmap status {rc}: {msg.decode() if msg else ''}")
+
+    # ----------------------------------------------------------- input
+    def load(self, path):
+        """load_pajek_format_network / load_linkList_format_network (FileIO.cpp)."""
+        n, m, ns, nd = C.c_int32(), C.c_int64(), C.c_int32(), C.c_int32()
+        self._check(self._L.rmx_load_network(self._h, str(path).encode(), C.byref(n), C.byref(m), C.byref(ns),
+                                             C.byref(nd)))
+        self.n, self.m = n.value, m.value
+        self.n_self_links, self.n_double_links = ns.value, nd.value
+        return self
+
+    def upload(self, n, out_ptr, out_idx, w, node_w=None):
+        out_ptr = np.ascontiguousarray(out_ptr, dtype=np.int64)
+        out_idx = np.ascontiguousarray(out_idx, dtype=np.int32)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        nwp = None
+        if node_w is not None:
+            node_w = np.ascontiguousarray(node_w, dtype=np.float64)
+            nwp = _p(node_w, _F64P)
+        self._check(self._L.rmx_graph_upload(self._h, int(n), len(out_idx), _p(out_ptr, _I64P), _p(out_idx, _I32P),
+                                             _p(w, _F64P), nwp))
+        self.n, self.m = int(n), len(out_idx)
+        return self
+
+    # -------------------------------------------------- steady state
+    def initiate(self):
+        """Network::initiate (Module.cpp:210-333); returns (iterations, codelength)."""
+        it, L0, nd = C.c_int32(), C.c_double(), C.c_int64()
+        self._check(self._L.rmx_initiate(self._h, C.byref(it), C.byref(L0), C.byref(nd)))
+        self.iterations, self.initial_codelength, self.n_dangling = it.value, L0.value, nd.value
+        return self.iterations, self.initial_codelength
+
+    def flows(self):
+        a = np.empty(self.n, dtype=np.float64)
+        self._check(self._L.rmx_get_flows(self._h, _p(a, _F64P)))
+        return a
+
+    def node_exit(self):
+        a = np.empty(self.n, dtype=np.float64)
+        self._check(self._L.rmx_get_node_exit(self._h, _p(a, _F64P)))
+        return a
+
+    # ------------------------------------------------------ codelength
+    def tracked(self):
+        L, S, nm = C.c_double(), C.c_double(), C.c_int32()
+        self._check(self._L.rmx_codelength(self._h, C.byref(L), C.byref(S), C.byref(nm)))
+        return L.value, S.value, nm.value
+
+    @property
+    def codelength(self):
+        return self.tracked()[0]
+
+    @property
+    def sum_exit(self):
+        return self.tracked()[1]
+
+    @property
+    def n_module(self):
+        return self.tracked()[2]
+
+    def calculate_codelength(self):
+        """Network::calculateCodeLength (Module.cpp:2966-3002)."""
+        L = C.c_double()
+        self._check(self._L.rmx_calculate_codelength(self._h, C.byref(L)))
+        return L.value
+
+    # ------------------------------------------------------- partitions
+    def set_partition(self, part):
+        part = np.ascontiguousarray(part, dtype=np.int32)
+        self._check(self._L.rmx_set_partition(self._h, _p(part, _I32P)))
+
+    def partition(self):
+        a = np.empty(self.n, dtype=np.int32)
+        self._check(self._L.rmx_get_partition(self._h, _p(a, _I32P)))
+        return a
+
+    # ------------------------------------------------------------ sweeps
+    def reset_active(self):
+        self._check(self._L.rmx_reset_active(self._h))
+
+    def move(self, prioritized=True, vthresh=0.0, seed=1):
+        """One sweep (Network::prioritize_parallelMove / parallelMove and the
+        super-node twins); returns the number of original nodes moved."""
+        mv, L, ev = C.c_int64(), C.c_double(), C.c_int64()
+        self._check(self._L.rmx_move_sweep(self._h, 1 if prioritized else 0, float(vthresh), int(seed) & (2**64 - 1),
+                                           C.byref(mv), C.byref(L), C.byref(ev)))
+        self.last_edge_visits = ev.value
+        return mv.value
+
+    def update_members_in_module(self):
+        self._check(self._L.rmx_update_members(self._h))
+
+    def convert_modules_to_super_nodes(self):
+        ns, ms = C.c_int32(), C.c_int64()
+        self._check(self._L.rmx_contract(self._h, C.byref(ns), C.byref(ms)))
+        self.n_super, self.m_super = ns.value, ms.value
+        return self.n_super, self.m_super
+
+    def super_nodes(self):
+        s, m = self.n_super, self.m_super
+        size, ex, tp, dg = (np.empty(s) for _ in range(4))
+        mod = np.empty(s, dtype=np.int32)
+        nm = np.empty(s, dtype=np.int32)
+        self._check(self._L.rmx_get_super(self._h, _p(size, _F64P), _p(ex, _F64P), _p(tp, _F64P), _p(dg, _F64P),
+                                          _p(mod, _I32P), _p(nm, _I32P)))
+        optr = np.empty(s + 1, dtype=np.int64)
+        iptr = np.empty(s + 1, dtype=np.int64)
+        oidx = np.empty(max(m, 1), dtype=np.int32)
+        iidx = np.empty(max(m, 1), dtype=np.int32)
+        ow = np.empty(max(m, 1))
+        iw = np.empty(max(m, 1))
+        self._check(self._L.rmx_get_super_arcs(self._h, _p(optr, _I64P), _p(oidx, _I32P), _p(ow, _F64P),
+                                               _p(iptr, _I64P), _p(iidx, _I32P), _p(iw, _F64P)))
+        return {"size": size, "exit": ex, "tp": tp, "dangling": dg, "mod": mod, "n_members": nm,
+                "out_ptr": optr, "out_idx": oidx[:m], "out_w": ow[:m], "in_ptr": iptr, "in_idx": iidx[:m],
+                "in_w": iw[:m]}
+
+    def to_node_level(self):
+        self._check(self._L.rmx_to_node_level(self._h))
+
+    # ------------------------------------------------------------ driver
+    def run(self, threshold=1e-3, vthresh=0.0, max_iter=10, prior=True, seed=1, verbose=False):
+        """main()'s super-step loop (OmpRelaxmap.cpp:181-219)."""
+        o = _Opts(float(threshold), float(vthresh), int(max_iter), 1 if prior else 0, int(seed), 1 if verbose else 0)
+        r = _Result()
+        self._check(self._L.rmx_partition(self._h, C.byref(o), C.byref(r)))
+        return PartitionResult(r.codelength, r.computed_codelength, r.n_module, r.super_steps, r.sweeps,
+                               r.edge_visits, r.time_partition_s, r.time_sweeps_s, r.time_contract_s)
+
+    def write_tree(self, path):
+        self._check(self._L.rmx_write_tree(self._h, str(path).encode()))
+
+    def write_clu(self, path):
+        self._check(self._L.rmx_write_clu(self._h, str(path).encode()))
+
+    def timing(self):
+        f, s, c, e, k = C.c_double(), C.c_double(), C.c_double(), C.c_int64(), C.c_int64()
+        self._check(self._L.rmx_timing(self._h, C.byref(f), C.byref(s), C.byref(c), C.byref(e), C.byref(k)))
+        return {"flow_ms": f.value, "sweep_ms": s.value, "contract_ms": c.value, "edge_visits": e.value,
+                "sweeps": k.value}

@@ -1,0 +1,551 @@
+// C ABI of relaxmap_amd (include/relaxmap.h) and the host-side driver that
+// restates the reference's super-step loop around the device kernels.
+#include "../../include/relaxmap.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "context.h"
+#include "netio.h"
+
+using namespace rmx;
+
+struct rmx_ctx {
+  Ctx c;
+  std::vector<std::string> names;
+  std::string last_error;
+};
+
+namespace {
+
+template <typename F>
+int guarded(rmx_ctx *ctx, F &&f) {
+  if (!ctx) return RMX_EINVAL;
+  try {
+    ctx->last_error.clear();
+    return f();
+  } catch (const HipError &e) {
+    ctx->last_error = e.what();
+    return RMX_EDEVICE;
+  } catch (const std::bad_alloc &) {
+    ctx->last_error = "host allocation failed";
+    return RMX_EDEVICE;
+  } catch (const std::exception &e) {
+    ctx->last_error = e.what();
+    return RMX_EINVAL;
+  }
+}
+
+int fail(rmx_ctx *ctx, int code, const std::string &msg) {
+  ctx->last_error = msg;
+  return code;
+}
+
+void free_ctx_device(Ctx &c) {
+  c.L0.free_all();
+  c.L1.free_all();
+  c.node_w.free(); c.raw_w.free(); c.dangling.free(); c.in_of_out.free(); c.node_sp.free();
+  c.mP.free(); c.mT.free(); c.mD.free(); c.mX.free(); c.mNodes.free(); c.mUnits.free(); c.mStale.free();
+  c.mInst.free();
+  c.act.free(); c.act_next.free(); c.target.free(); c.newmod.free(); c.stamp.free(); c.movers.free();
+  c.empty_list.free(); c.want_empty.free(); c.rank_tmp.free(); c.agg_buf.free();
+  c.hub_keys.free(); c.hub_out.free(); c.hub_in.free();
+  c.counters.free(); c.scal.free(); c.partial.free(); c.x.free(); c.y.free(); c.xf.free(); c.node_q.free();
+  c.scratch.release();
+  c.scratch2.release();
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int rmx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int rmx_create(rmx_ctx **out, int device) {
+  if (!out) return RMX_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return RMX_EDEVICE;
+  if (device < 0 || device >= n) return RMX_EINVAL;
+  rmx_ctx *ctx = new rmx_ctx();
+  int rc = guarded(ctx, [&] {
+    ctx->c.device = device;
+    RMX_HIP(hipSetDevice(device));
+    RMX_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+    RMX_HIP(hipHostMalloc((void **)&ctx->c.h_scal, sizeof(double) * 16, hipHostMallocDefault));
+    RMX_HIP(hipHostMalloc((void **)&ctx->c.h_cnt, sizeof(uint32_t) * 16, hipHostMallocDefault));
+    return RMX_OK;
+  });
+  if (rc != RMX_OK) {
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return RMX_OK;
+}
+
+void rmx_destroy(rmx_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  free_ctx_device(ctx->c);
+  if (ctx->c.h_scal) (void)hipHostFree(ctx->c.h_scal);
+  if (ctx->c.h_cnt) (void)hipHostFree(ctx->c.h_cnt);
+  if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+}
+
+const char *rmx_last_error(const rmx_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int rmx_set_precision(rmx_ctx *ctx, int precision) {
+  return guarded(ctx, [&] {
+    if (precision != RMX_FP64 && precision != RMX_MIXED) return fail(ctx, RMX_EINVAL, "precision must be 0 or 1");
+    ctx->c.precision = precision;
+    return RMX_OK;
+  });
+}
+
+int rmx_set_batches(rmx_ctx *ctx, int batches) {
+  return guarded(ctx, [&] {
+    if (batches < 1 || batches > 1024) return fail(ctx, RMX_EINVAL, "batches must be in 1..1024");
+    ctx->c.batches = batches;
+    return RMX_OK;
+  });
+}
+
+int rmx_graph_upload(rmx_ctx *ctx, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx,
+                     const double *w, const double *node_w) {
+  return guarded(ctx, [&] {
+    if (n <= 0 || m < 0 || !out_ptr || (m > 0 && (!out_idx || !w)))
+      return fail(ctx, RMX_EINVAL, "empty network or null arrays");
+    if (out_ptr[0] != 0 || out_ptr[n] != m) return fail(ctx, RMX_EINVAL, "out_ptr does not span the arcs");
+    for (int32_t i = 0; i < n; ++i) {
+      if (out_ptr[i + 1] < out_ptr[i]) return fail(ctx, RMX_EINVAL, "out_ptr not monotone");
+      for (int64_t k = out_ptr[i]; k < out_ptr[i + 1]; ++k) {
+        int32_t d = out_idx[k];
+        if (d < 0 || d >= n) return fail(ctx, RMX_EINVAL, "arc head out of range");
+        if (d == i) return fail(ctx, RMX_EINVAL, "self-loop in arc list");
+        if (k > out_ptr[i] && out_idx[k - 1] >= d) return fail(ctx, RMX_EINVAL, "arcs not sorted or duplicated");
+        if (!(w[k] > 0.0) || !std::isfinite(w[k])) return fail(ctx, RMX_EINVAL, "arc weight not positive");
+      }
+    }
+    RMX_HIP(hipSetDevice(ctx->c.device));
+    upload_graph(ctx->c, n, m, out_ptr, out_idx, w, node_w);
+    if ((int32_t)ctx->names.size() != n) {
+      ctx->names.resize(n);
+      for (int32_t i = 0; i < n; ++i) ctx->names[i] = std::to_string(i + 1);
+    }
+    return RMX_OK;
+  });
+}
+
+int rmx_load_network(rmx_ctx *ctx, const char *path, int32_t *n, int64_t *m, int32_t *n_self, int32_t *n_double) {
+  return guarded(ctx, [&] {
+    if (!path) return fail(ctx, RMX_EINVAL, "null path");
+    NetFile net;
+    std::string err;
+    int rc = read_network(path, net, err);
+    if (rc != 0) return fail(ctx, rc == -2 ? RMX_EIO : RMX_EINVAL, err);
+    ctx->names = std::move(net.names);
+    int r = rmx_graph_upload(ctx, net.n, (int64_t)net.idx.size(), net.ptr.data(), net.idx.data(), net.w.data(),
+                             net.node_w.data());
+    if (r != RMX_OK) return r;
+    if (n) *n = net.n;
+    if (m) *m = (int64_t)net.idx.size();
+    if (n_self) *n_self = net.n_self;
+    if (n_double) *n_double = net.n_double;
+    return RMX_OK;
+  });
+}
+
+int rmx_initiate(rmx_ctx *ctx, int32_t *iterations, double *codelength, int64_t *n_dangling) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (c.N <= 0) return fail(ctx, RMX_ESTATE, "no network uploaded");
+    RMX_HIP(hipSetDevice(c.device));
+    c.level = 0;
+    steady_state(c, 200, 50, 1.0e-15);
+    init_modules_singletons(c);
+    module_totals(c, &c.sum_exit, &c.codelength);
+    ensure_sweep_buffers(c);
+    reset_active(c);
+    c.initiated = true;
+    if (iterations) *iterations = c.iterations;
+    if (codelength) *codelength = c.initial_codelength;
+    if (n_dangling) *n_dangling = c.n_dangling;
+    return RMX_OK;
+  });
+}
+
+int rmx_get_flows(rmx_ctx *ctx, double *p) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    RMX_HIP(hipMemcpyAsync(p, c.L0.p.p, sizeof(double) * c.N, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    return RMX_OK;
+  });
+}
+
+int rmx_get_node_exit(rmx_ctx *ctx, double *q) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    RMX_HIP(hipMemcpyAsync(q, c.node_q.p, sizeof(double) * c.N, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    return RMX_OK;
+  });
+}
+
+int rmx_codelength(rmx_ctx *ctx, double *codelength, double *sum_exit, int32_t *n_module) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    if (codelength) *codelength = c.codelength;
+    if (sum_exit) *sum_exit = c.sum_exit;
+    if (n_module) *n_module = c.n_module;
+    return RMX_OK;
+  });
+}
+
+int rmx_calculate_codelength(rmx_ctx *ctx, double *codelength) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    propagate_labels(c);
+    *codelength = calculate_codelength(c);
+    return RMX_OK;
+  });
+}
+
+int rmx_set_partition(rmx_ctx *ctx, const int32_t *part) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    for (int32_t i = 0; i < c.N; ++i)
+      if (part[i] < 0 || part[i] >= c.N) return fail(ctx, RMX_EINVAL, "module index out of range");
+    c.level = 0;
+    DevBuf<uint32_t> d;
+    d.alloc(c.N);
+    RMX_HIP(hipMemcpyAsync(d.p, part, sizeof(uint32_t) * c.N, hipMemcpyHostToDevice, c.stream));
+    set_partition(c, d.p);
+    reset_active(c);
+    c.sync();
+    return RMX_OK;
+  });
+}
+
+int rmx_get_partition(rmx_ctx *ctx, int32_t *part) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    propagate_labels(c);
+    RMX_HIP(hipMemcpyAsync(part, c.L0.mod.p, sizeof(uint32_t) * c.N, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    return RMX_OK;
+  });
+}
+
+int rmx_reset_active(rmx_ctx *ctx) {
+  return guarded(ctx, [&] {
+    if (!ctx->c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    reset_active(ctx->c);
+    return RMX_OK;
+  });
+}
+
+int rmx_move_sweep(rmx_ctx *ctx, int prioritized, double vthresh, uint64_t seed, int64_t *moved, double *codelength,
+                   int64_t *edge_visits) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    int64_t mv = move_sweep(c, prioritized != 0, -vthresh, seed, edge_visits);
+    if (moved) *moved = mv;
+    if (codelength) *codelength = c.codelength;
+    return RMX_OK;
+  });
+}
+
+int rmx_update_members(rmx_ctx *ctx) {
+  return guarded(ctx, [&] {
+    if (!ctx->c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    update_members(ctx->c);
+    return RMX_OK;
+  });
+}
+
+int rmx_contract(rmx_ctx *ctx, int32_t *n_super, int64_t *m_super) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    contract(c);
+    reset_active(c);
+    if (n_super) *n_super = c.L1.n;
+    if (m_super) *m_super = c.L1.m;
+    return RMX_OK;
+  });
+}
+
+int rmx_get_super(rmx_ctx *ctx, double *size, double *exit, double *tp, double *dangling, int32_t *mod,
+                  int32_t *n_members) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (c.level != 1) return fail(ctx, RMX_ESTATE, "no super nodes (rmx_contract first)");
+    const int32_t n = c.L1.n;
+    std::vector<long long> up(n), ut(n), ud(n), uw(n);
+    std::vector<int32_t> nm(n);
+    std::vector<uint32_t> md(n);
+    RMX_HIP(hipMemcpyAsync(up.data(), c.L1.up.p, sizeof(long long) * n, hipMemcpyDeviceToHost, c.stream));
+    RMX_HIP(hipMemcpyAsync(ut.data(), c.L1.ut.p, sizeof(long long) * n, hipMemcpyDeviceToHost, c.stream));
+    RMX_HIP(hipMemcpyAsync(ud.data(), c.L1.ud.p, sizeof(long long) * n, hipMemcpyDeviceToHost, c.stream));
+    RMX_HIP(hipMemcpyAsync(uw.data(), c.L1.uw.p, sizeof(long long) * n, hipMemcpyDeviceToHost, c.stream));
+    RMX_HIP(hipMemcpyAsync(nm.data(), c.L1.nmem.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c.stream));
+    RMX_HIP(hipMemcpyAsync(md.data(), c.L1.mod.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    for (int32_t i = 0; i < n; ++i) {
+      const double P = fx_to(up[i]), T = fx_to(ut[i]), D = fx_to(ud[i]), X = fx_to(uw[i]);
+      if (size) size[i] = P;
+      if (tp) tp[i] = T;
+      if (dangling) dangling[i] = D;
+      if (exit) exit[i] = (kAlpha * P + kBeta * D) * (1.0 - T) + kBeta * X;
+      if (mod) mod[i] = (int32_t)md[i];
+      if (n_members) n_members[i] = nm[i];
+    }
+    return RMX_OK;
+  });
+}
+
+int rmx_get_super_arcs(rmx_ctx *ctx, int64_t *out_ptr, int32_t *out_idx, double *out_w, int64_t *in_ptr,
+                       int32_t *in_idx, double *in_w) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (c.level != 1) return fail(ctx, RMX_ESTATE, "no super nodes (rmx_contract first)");
+    const int32_t n = c.L1.n;
+    const int64_t m = c.L1.m;
+    std::vector<uint32_t> op(n + 1), ip(n + 1);
+    RMX_HIP(hipMemcpyAsync(op.data(), c.L1.out_ptr.p, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToHost, c.stream));
+    RMX_HIP(hipMemcpyAsync(ip.data(), c.L1.in_ptr.p, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToHost, c.stream));
+    if (m > 0) {
+      if (out_idx) RMX_HIP(hipMemcpyAsync(out_idx, c.L1.out_idx.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, c.stream));
+      if (out_w) RMX_HIP(hipMemcpyAsync(out_w, c.L1.out_w.p, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+      if (in_idx) RMX_HIP(hipMemcpyAsync(in_idx, c.L1.in_idx.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, c.stream));
+      if (in_w) RMX_HIP(hipMemcpyAsync(in_w, c.L1.in_w.p, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+    }
+    c.sync();
+    for (int32_t i = 0; i <= n; ++i) {
+      if (out_ptr) out_ptr[i] = op[i];
+      if (in_ptr) in_ptr[i] = ip[i];
+    }
+    return RMX_OK;
+  });
+}
+
+int rmx_to_node_level(rmx_ctx *ctx) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    to_node_level(c);
+    reset_active(c);
+    return RMX_OK;
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ driver
+namespace {
+
+struct Driver {
+  Ctx &c;
+  const rmx_partition_opts &o;
+  rmx_partition_result &r;
+  double t0;
+  double t_sweeps = 0, t_contract = 0;
+  uint64_t sweep_no = 0;
+  const double ln2 = std::log(2.0);
+
+  int64_t sweep() {
+    double a = now_s();
+    int64_t vis = 0;
+    int64_t moved = move_sweep(c, o.prior != 0, -o.vthresh, o.seed ^ (0x5851f42d4c957f2dull * ++sweep_no), &vis);
+    t_sweeps += now_s() - a;
+    r.sweeps++;
+    r.edge_visits += vis;
+    return moved;
+  }
+
+  void log_iter(const char *tag, int outer, int it, double t_it, int64_t moved) {
+    if (!o.verbose) return;
+    if (outer > 0)
+      std::printf("%s %d-%d: code length =\t%g\t, elapsed time:\t%g\t(sec), accumulated time:\t%g\t(sec)\tsumExitPr = %g\tnumMoved:\t%lld\n",
+                  tag, outer, it, c.codelength / ln2, t_it, now_s() - t0, c.sum_exit, (long long)moved);
+    else
+      std::printf("%s %d: code length =\t%g\t, elapsed time:\t%g\t(sec), accumulated time:\t%g\t(sec)\tsumExitPr = %g\tnumMoved:\t%lld\n",
+                  tag, it, c.codelength / ln2, t_it, now_s() - t0, c.sum_exit, (long long)moved);
+  }
+
+  // stochastic_greedy_partition (OmpRelaxmap.cpp:263-429)
+  void greedy(bool fine) {
+    const double thr = o.threshold;
+    if (fine) to_node_level(c);
+    else {
+      // coarse tune: move the current modules as units
+      double a = now_s();
+      contract(c);
+      t_contract += now_s() - a;
+    }
+    reset_active(c);
+    if (o.verbose) std::printf("%d, ", c.cur().n);
+    int iter = 0;
+    bool stop = false;
+    while (!stop && iter < o.max_iter) {
+      double a = now_s();
+      double old = c.codelength;
+      int64_t moved = sweep();
+      ++iter;
+      if (old - c.codelength >= 0 && (old - c.codelength) / ln2 < thr) stop = true;
+      log_iter("Iteration", 0, iter, now_s() - a, moved);
+    }
+    update_members(c);
+    int outer = 1;
+    double old;
+    do {
+      old = c.codelength;
+      stop = false;
+      double a = now_s();
+      contract(c);
+      t_contract += now_s() - a;
+      reset_active(c);
+      if (o.verbose) std::printf("%d, ", c.L1.n);
+      int sp_iter = 0;
+      while (!stop && sp_iter < o.max_iter) {
+        double b = now_s();
+        double inner = c.codelength;
+        int64_t moved = sweep();
+        ++sp_iter;
+        if (inner - c.codelength >= 0.0 && (inner - c.codelength) / ln2 < thr) stop = true;
+        log_iter("SuperIteration", outer, sp_iter, now_s() - b, moved);
+      }
+      update_members(c);
+      ++outer;
+    } while ((old - c.codelength) / ln2 > thr);
+    propagate_labels(c);
+  }
+};
+
+}  // namespace
+
+void rmx::to_node_level(Ctx &c) {
+  if (c.level == 1) {
+    propagate_labels(c);
+    c.level = 0;
+  }
+  // at the node level every node is a unit
+  RMX_HIP(hipMemcpyAsync(c.mUnits.p, c.mNodes.p, sizeof(int32_t) * c.N, hipMemcpyDeviceToDevice, c.stream));
+}
+
+extern "C" int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_partition_result *res) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    if (!opts || !res) return fail(ctx, RMX_EINVAL, "null options or result");
+    if (opts->max_iter < 0) return fail(ctx, RMX_EINVAL, "max_iter < 0");
+    std::memset(res, 0, sizeof(*res));
+    RMX_HIP(hipSetDevice(c.device));
+    Driver d{c, *opts, *res, now_s()};
+    const double ln2 = std::log(2.0);
+    const double t_start = now_s();
+    // main(): OmpRelaxmap.cpp:181-219
+    bool fine = true;
+    int step = 1;
+    double old = c.codelength;
+    d.greedy(fine);
+    if (opts->verbose)
+      std::printf("SuperStep [%d] - codeLength = %g in %d modules.\n", step, c.codelength / ln2, c.n_module);
+    bool next = !((old - c.codelength) / ln2 < opts->threshold);
+    while (next) {
+      old = c.codelength;
+      d.greedy(fine);
+      ++step;
+      if (opts->verbose)
+        std::printf("SuperStep [%d] - codeLength = %g in %d modules.\n", step, c.codelength / ln2, c.n_module);
+      if ((old - c.codelength) / ln2 < opts->threshold) next = false;
+      fine = !fine;
+    }
+    c.sync();
+    res->time_partition_s = now_s() - t_start;
+    res->time_sweeps_s = d.t_sweeps;
+    res->time_contract_s = d.t_contract;
+    res->codelength = c.codelength;
+    res->n_module = c.n_module;
+    res->super_steps = step;
+    res->computed_codelength = calculate_codelength(c);
+    c.timing.sweep_ms = d.t_sweeps * 1e3;
+    c.timing.contract_ms = d.t_contract * 1e3;
+    if (opts->verbose) {
+      std::printf("Time for partitioning : %g (sec)\n", res->time_partition_s);
+      std::printf("DONE: Code Length = %g in %d modules, with %d nodes.\n\n", c.codelength / ln2, c.n_module, c.N);
+    }
+    return RMX_OK;
+  });
+}
+
+extern "C" int rmx_write_tree(rmx_ctx *ctx, const char *path) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    propagate_labels(c);
+    std::vector<uint32_t> mod(c.N);
+    std::vector<double> p(c.N);
+    RMX_HIP(hipMemcpyAsync(mod.data(), c.L0.mod.p, sizeof(uint32_t) * c.N, hipMemcpyDeviceToHost, c.stream));
+    RMX_HIP(hipMemcpyAsync(p.data(), c.L0.p.p, sizeof(double) * c.N, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    std::string err;
+    if (write_tree(path, c.codelength, mod, p, ctx->names, err) != 0) return fail(ctx, RMX_EIO, err);
+    return RMX_OK;
+  });
+}
+
+extern "C" int rmx_write_clu(rmx_ctx *ctx, const char *path) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    propagate_labels(c);
+    std::vector<uint32_t> mod(c.N);
+    RMX_HIP(hipMemcpyAsync(mod.data(), c.L0.mod.p, sizeof(uint32_t) * c.N, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    std::string err;
+    if (write_clu(path, mod, err) != 0) return fail(ctx, RMX_EIO, err);
+    return RMX_OK;
+  });
+}
+
+extern "C" int rmx_timing(rmx_ctx *ctx, double *flow_ms, double *sweep_ms, double *contract_ms, int64_t *edge_visits,
+                          int64_t *sweeps) {
+  return guarded(ctx, [&] {
+    const Timing &t = ctx->c.timing;
+    if (flow_ms) *flow_ms = t.flow_ms;
+    if (sweep_ms) *sweep_ms = t.sweep_ms;
+    if (contract_ms) *contract_ms = t.contract_ms;
+    if (edge_visits) *edge_visits = t.edge_visits;
+    if (sweeps) *sweeps = t.sweeps;
+    return RMX_OK;
+  });
+}

@@ -1,0 +1,286 @@
+// Map-equation codelength of the current partition.
+//
+// Tracked form (the sweeps): module statistics P, T, D (flow, teleport,
+// dangling flow) and X (flow on arcs leaving the module) are fixed-point
+// sums; q_M = (alpha P + beta D)(1 - T) + beta X is the exit probability
+// of Module.cpp:2978-2991, and
+//   L = phi(sum q) - 2 sum phi(q) + sum phi(q + P) - sum_a phi(p_a)
+// as in Network::calibrate (Module.cpp:474-501).
+//
+// From-scratch form (Network::calculateCodeLength, Module.cpp:2966-3002):
+// nodes are stably sorted by module, so every module's members are visited
+// in ascending node order and their out-arcs in stored order, and each
+// module's sums are formed in FP64 in exactly the reference's order.
+#include "devutil.h"
+
+namespace rmx {
+
+namespace {
+
+constexpr int kRedGrid = 1024;
+constexpr int kRedBlock = 256;
+enum { kS = 0, kR = 1, kD = 2, kL = 3, kNM = 7 };
+
+__device__ inline double reduce_partials(const double *__restrict__ part, double *lds) {
+  double a = 0.0;
+  for (int i = threadIdx.x; i < kRedGrid; i += kRedBlock) a += part[i];
+  a = block_sum<double, kRedBlock>(a, lds);
+  if (threadIdx.x == 0) lds[0] = a;
+  __syncthreads();
+  double s = lds[0];
+  __syncthreads();
+  return s;
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_module_totals(const long long *__restrict__ mP,
+                                                             const long long *__restrict__ mT,
+                                                             const long long *__restrict__ mD,
+                                                             const long long *__restrict__ mX,
+                                                             const int32_t *__restrict__ mNodes, int32_t n,
+                                                             double *__restrict__ part) {
+  __shared__ double lds[kRedBlock / kWave];
+  double s = 0, se = 0, ss = 0, cnt = 0;
+  for (int64_t m = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; m < n; m += (int64_t)kRedGrid * kRedBlock) {
+    if (mNodes[m] > 0) {
+      double q = module_exit(mP[m], mT[m], mD[m], mX[m]);
+      s += q;
+      se += plogp(q);
+      ss += plogp(q + fx_to(mP[m]));
+      cnt += 1.0;
+    }
+  }
+  s = block_sum<double, kRedBlock>(s, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  se = block_sum<double, kRedBlock>(se, lds);
+  if (threadIdx.x == 0) part[kRedGrid + blockIdx.x] = se;
+  ss = block_sum<double, kRedBlock>(ss, lds);
+  if (threadIdx.x == 0) part[2 * kRedGrid + blockIdx.x] = ss;
+  cnt = block_sum<double, kRedBlock>(cnt, lds);
+  if (threadIdx.x == 0) part[3 * kRedGrid + blockIdx.x] = cnt;
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_totals_final(const double *__restrict__ part, double all_log_all,
+                                                            double *__restrict__ scal) {
+  __shared__ double lds[kRedBlock / kWave];
+  double s = reduce_partials(part, lds);
+  double se = reduce_partials(part + kRedGrid, lds);
+  double ss = reduce_partials(part + 2 * kRedGrid, lds);
+  double cnt = reduce_partials(part + 3 * kRedGrid, lds);
+  if (threadIdx.x == 0) {
+    scal[kS] = s;
+    scal[kL] = plogp(s) - 2.0 * se + ss - all_log_all;
+    scal[kNM] = cnt;
+  }
+}
+
+__global__ void k_zero_modules(int32_t n, long long *mP, long long *mT, long long *mD, long long *mX,
+                               int32_t *mNodes, int32_t *mUnits) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  mP[i] = 0; mT[i] = 0; mD[i] = 0; mX[i] = 0;
+  mNodes[i] = 0; mUnits[i] = 0;
+}
+
+__global__ void k_accumulate_units(int32_t n, const uint32_t *__restrict__ mod, const long long *__restrict__ up,
+                                   const long long *__restrict__ ut, const long long *__restrict__ ud,
+                                   const int32_t *__restrict__ nmem, long long *mP, long long *mT, long long *mD,
+                                   int32_t *mNodes, int32_t *mUnits) {
+  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  uint32_t m = mod[u];
+  atomicAdd((unsigned long long *)&mP[m], (unsigned long long)up[u]);
+  atomicAdd((unsigned long long *)&mT[m], (unsigned long long)ut[u]);
+  atomicAdd((unsigned long long *)&mD[m], (unsigned long long)ud[u]);
+  atomicAdd(&mNodes[m], nmem[u]);
+  atomicAdd(&mUnits[m], 1);
+}
+
+// X_M = sum over arcs a->b with a in M, b not in M of the arc flow
+template <int G>
+__global__ void k_exit_flows(int32_t n, const uint32_t *__restrict__ optr, const uint32_t *__restrict__ oidx,
+                             ArcW aw, const uint32_t *__restrict__ mod, long long *mX) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t u = t / G;
+  const int lane = (int)(t % G);
+  if (u >= n) return;
+  const uint32_t m = mod[u];
+  long long acc = 0;
+  for (uint32_t k = optr[u] + lane; k < optr[u + 1]; k += G)
+    if (mod[oidx[k]] != m) acc += aw.fxw(k);
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+  if (lane == 0 && acc != 0) atomicAdd((unsigned long long *)&mX[m], (unsigned long long)acc);
+}
+
+template <typename T>
+__global__ void k_copy(const T *__restrict__ a, T *__restrict__ b, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+__global__ void k_seg_heads(const uint32_t *__restrict__ key, int32_t n, uint32_t *__restrict__ flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1u : 0u;
+}
+
+__global__ void k_seg_starts(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ pos, int32_t n,
+                             uint32_t *__restrict__ start) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flag[i]) start[pos[i]] = (uint32_t)i;
+}
+
+// one thread per module: sums in the reference's order (members ascending,
+// out-arcs in stored order), Module.cpp:2976-2995
+__global__ void k_p5_modules(const uint32_t *__restrict__ start, const uint32_t *__restrict__ nseg_p, int32_t n,
+                             const uint32_t *__restrict__ key, const uint32_t *__restrict__ node,
+                             const double *__restrict__ p, const double *__restrict__ tau,
+                             const uint8_t *__restrict__ dang, const uint32_t *__restrict__ optr,
+                             const uint32_t *__restrict__ oidx, const double *__restrict__ ow,
+                             const uint32_t *__restrict__ mod, double *__restrict__ segq,
+                             double *__restrict__ segqp) {
+  const uint32_t nseg = *nseg_p;
+  int64_t sidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sidx >= nseg) return;
+  const uint32_t b = start[sidx];
+  const uint32_t e = (sidx + 1 < nseg) ? start[sidx + 1] : (uint32_t)n;
+  const uint32_t m = key[b];
+  double P = 0.0, T = 0.0, D = 0.0, ext = 0.0;
+  for (uint32_t i = b; i < e; ++i) {
+    const uint32_t a = node[i];
+    P += p[a];
+    T += tau[a];
+    if (dang[a]) D += p[a];
+    for (uint32_t k = optr[a]; k < optr[a + 1]; ++k)
+      if (mod[oidx[k]] != m) ext += ow[k];
+  }
+  double q = kAlpha * (1.0 - T) * P;
+  q += kBeta * (ext + (1.0 - T) * D);
+  segq[sidx] = q;
+  segqp[sidx] = q + P;
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_p5_partials(const double *__restrict__ segq,
+                                                           const double *__restrict__ segqp,
+                                                           const uint32_t *__restrict__ nseg_p,
+                                                           double *__restrict__ part) {
+  __shared__ double lds[kRedBlock / kWave];
+  const uint32_t nseg = *nseg_p;
+  double s = 0, se = 0, ss = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < nseg; i += (int64_t)kRedGrid * kRedBlock) {
+    s += segq[i];
+    se += plogp(segq[i]);
+    ss += plogp(segqp[i]);
+  }
+  s = block_sum<double, kRedBlock>(s, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  se = block_sum<double, kRedBlock>(se, lds);
+  if (threadIdx.x == 0) part[kRedGrid + blockIdx.x] = se;
+  ss = block_sum<double, kRedBlock>(ss, lds);
+  if (threadIdx.x == 0) part[2 * kRedGrid + blockIdx.x] = ss;
+  if (threadIdx.x == 0) part[3 * kRedGrid + blockIdx.x] = 0.0;
+}
+
+}  // namespace
+
+ArcW arcw_out(const Ctx &c, const Level &L) {
+  ArcW a;
+  a.w = L.out_w.p;
+  a.wf = L.out_wf.p;
+  a.wx = L.out_wx.p;
+  a.mode = (&L == &c.L1) ? 2 : (c.precision == 1 ? 1 : 0);
+  return a;
+}
+
+ArcW arcw_in(const Ctx &c, const Level &L) {
+  ArcW a;
+  a.w = L.in_w.p;
+  a.wf = L.in_wf.p;
+  a.wx = L.in_wx.p;
+  a.mode = (&L == &c.L1) ? 2 : (c.precision == 1 ? 1 : 0);
+  return a;
+}
+
+void module_totals_async(Ctx &c) {
+  k_module_totals<<<kRedGrid, kRedBlock, 0, c.stream>>>(c.mP.p, c.mT.p, c.mD.p, c.mX.p, c.mNodes.p, c.N,
+                                                         c.partial.p);
+  RMX_LAUNCH_CHECK();
+  k_totals_final<<<1, kRedBlock, 0, c.stream>>>(c.partial.p, c.all_log_all, c.scal.p);
+  RMX_LAUNCH_CHECK();
+}
+
+void module_totals(Ctx &c, double *S, double *L) {
+  module_totals_async(c);
+  RMX_HIP(hipMemcpyAsync(c.h_scal, c.scal.p, sizeof(double) * 8, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  if (S) *S = c.h_scal[kS];
+  if (L) *L = c.h_scal[kL];
+  c.n_module = (int32_t)c.h_scal[kNM];
+}
+
+void rebuild_exit_flows(Ctx &c) {
+  Level &L = c.cur();
+  fill_i64(c.mX.p, 0, c.N, c.stream);
+  k_exit_flows<16><<<grid_for((int64_t)L.n * 16, 256), 256, 0, c.stream>>>(L.n, L.out_ptr.p, L.out_idx.p,
+                                                                            arcw_out(c, L), L.mod.p, c.mX.p);
+  RMX_LAUNCH_CHECK();
+}
+
+void set_partition(Ctx &c, const uint32_t *d_part) {
+  if (c.level != 0) throw HipError("set_partition needs the node level");
+  Level &L = c.L0;
+  hipStream_t s = c.stream;
+  const int32_t n = L.n;
+  RMX_HIP(hipMemcpyAsync(L.mod.p, d_part, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+  unsigned g = grid_for(n, 256);
+  k_zero_modules<<<g, 256, 0, s>>>(n, c.mP.p, c.mT.p, c.mD.p, c.mX.p, c.mNodes.p, c.mUnits.p);
+  RMX_LAUNCH_CHECK();
+  k_accumulate_units<<<g, 256, 0, s>>>(n, L.mod.p, L.up.p, L.ut.p, L.ud.p, L.nmem.p, c.mP.p, c.mT.p, c.mD.p,
+                                       c.mNodes.p, c.mUnits.p);
+  RMX_LAUNCH_CHECK();
+  rebuild_exit_flows(c);
+  k_copy<int32_t><<<g, 256, 0, s>>>(c.mNodes.p, c.mStale.p, n);
+  RMX_LAUNCH_CHECK();
+  module_totals(c, &c.sum_exit, &c.codelength);
+}
+
+double calculate_codelength(Ctx &c) {
+  Level &L = c.L0;
+  hipStream_t s = c.stream;
+  const int32_t n = L.n;
+  DevBuf<uint32_t> key, node, flag, pos, start, total;
+  DevBuf<double> segq, segqp;
+  key.alloc(n);
+  node.alloc(n);
+  flag.alloc(n);
+  pos.alloc(n);
+  start.alloc(n);
+  total.alloc(1);
+  segq.alloc(n);
+  segqp.alloc(n);
+  RMX_HIP(hipMemcpyAsync(key.p, L.mod.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+  iota_u32(node.p, n, s);
+  int bits = 1;
+  while (bits < 32 && ((int64_t)1 << bits) < (int64_t)n) ++bits;
+  radix_sort_pairs_u32(key.p, node.p, n, 0, bits, c.scratch, s);
+  unsigned g = grid_for(n, 256);
+  k_seg_heads<<<g, 256, 0, s>>>(key.p, n, flag.p);
+  RMX_LAUNCH_CHECK();
+  exclusive_scan_u32(flag.p, pos.p, n, total.p, c.scratch, s);
+  k_seg_starts<<<g, 256, 0, s>>>(flag.p, pos.p, n, start.p);
+  RMX_LAUNCH_CHECK();
+  k_p5_modules<<<g, 256, 0, s>>>(start.p, total.p, n, key.p, node.p, L.p.p, L.tau.p, c.dangling.p, L.out_ptr.p,
+                                 L.out_idx.p, L.out_w.p, L.mod.p, segq.p, segqp.p);
+  RMX_LAUNCH_CHECK();
+  k_p5_partials<<<kRedGrid, kRedBlock, 0, s>>>(segq.p, segqp.p, total.p, c.partial.p);
+  RMX_LAUNCH_CHECK();
+  k_totals_final<<<1, kRedBlock, 0, s>>>(c.partial.p, c.all_log_all, c.scal.p + 8);
+  RMX_LAUNCH_CHECK();
+  double h[8];
+  RMX_HIP(hipMemcpyAsync(h, c.scal.p + 8, sizeof(h), hipMemcpyDeviceToHost, s));
+  c.sync();
+  return h[kL];
+}
+
+}  // namespace rmx

@@ -1,0 +1,154 @@
+// Device-resident state of one network (one HIP stream, one GPU).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "prims.h"
+
+namespace rmx {
+
+// Degree buckets of the move sweep: units are evaluated by groups of G
+// lanes whose LDS hash table holds H keys; the last bucket (hubs) uses one
+// workgroup per unit with its table in global memory.
+constexpr int kNumBuckets = 6;
+// upper bound of (out + in) degree per bucket
+constexpr int kBucketMaxDeg[kNumBuckets] = {16, 32, 128, 512, 2048, 0x7fffffff};
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  int64_t n = 0;
+  void alloc(int64_t count) {
+    free();
+    if (count > 0) RMX_HIP(hipMalloc(&p, sizeof(T) * count));
+    n = count;
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  T *get() const { return p; }
+};
+
+// One level of the hierarchy: the original nodes (level 0) or the super
+// nodes of the current contraction (level 1).  Units are nodes or super
+// nodes; arcs carry flows f_ab (level 0 after the steady state) or summed
+// inter-module flows (level 1).
+struct Level {
+  int32_t n = 0;
+  int64_t m = 0;
+  DevBuf<uint32_t> out_ptr, out_idx, in_ptr, in_idx;
+  DevBuf<double> out_w, in_w;     // FP64 flows
+  DevBuf<float> out_wf, in_wf;    // FP32 mirror (mixed precision)
+  DevBuf<long long> out_wx, in_wx;  // level 1: exact fixed-point arc flow sums
+  DevBuf<double> p, tau;          // level 0: steady-state flow and teleport weight
+  // unit statistics in fixed point: flow, teleport, dangling flow, out flow
+  DevBuf<long long> up, ut, ud, uw;
+  DevBuf<int32_t> nmem;           // original nodes inside the unit
+  DevBuf<uint32_t> mod;           // module of each unit
+  DevBuf<uint32_t> bucket;        // units grouped by degree bucket
+  int32_t bucket_off[kNumBuckets + 1] = {0};
+  int32_t max_deg = 0;
+  void free_all();
+};
+
+struct Timing {
+  double flow_ms = 0, sweep_ms = 0, contract_ms = 0;
+  int64_t edge_visits = 0;
+  int sweeps = 0;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  int32_t N = 0;
+  int64_t E = 0;
+  int precision = 0;   // 0 FP64, 1 mixed (FP32 bulk, FP64 accumulation)
+  int batches = 8;     // sub-sweeps per sweep
+  bool initiated = false;
+
+  Level L0, L1;
+  int level = 0;
+  DevBuf<double> node_w, raw_w;   // inputs kept for re-initiation
+  DevBuf<uint8_t> dangling;       // level-0 dangling flag
+  DevBuf<uint32_t> in_of_out;     // level-0: in-CSR position of each out arc
+  DevBuf<uint32_t> node_sp;       // node -> super node (level 1)
+
+  // module SoA, N slots (module ids are node ids, as in the reference)
+  DevBuf<long long> mP, mT, mD, mX;     // fixed-point sums: flow, teleport, dangling, exit arcs
+  DevBuf<int32_t> mNodes, mUnits, mStale;  // original nodes, units at the current level, stale count
+  DevBuf<uint32_t> mInst;                 // instance (graph) of each module (batched mode)
+
+  // sweep work arrays (sized for level 0, reused by level 1)
+  DevBuf<uint8_t> act, act_next;
+  DevBuf<uint32_t> target, newmod, stamp, movers, empty_list, want_empty, rank_tmp;
+  DevBuf<long long> agg_buf;         // per mover: flows to/from its old and new module
+  DevBuf<uint32_t> hub_keys;         // global hash tables of hub units
+  DevBuf<long long> hub_out, hub_in;
+  DevBuf<uint32_t> counters;         // small device counters
+  DevBuf<double> scal;               // device scalars: S, L, ...
+  DevBuf<double> partial;            // block partials for reductions
+  uint32_t stamp_now = 0;
+
+  // steady-state buffers
+  DevBuf<double> x, y;
+  DevBuf<float> xf;
+  DevBuf<double> node_q;   // singleton exits of Module.cpp:86-95 (reporting)
+
+  // tracked scalars (host copies); the initial values are the literal FP64
+  // calibrate of Module.cpp:474-501 over the singleton node exits
+  double codelength = 0, sum_exit = 0, all_log_all = 0;
+  double initial_codelength = 0, initial_sum_exit = 0;
+  int iterations = 0;
+  double resid = 0;
+  int32_t n_module = 0;
+  int64_t n_dangling = 0;
+
+  Scratch scratch, scratch2;
+  Timing timing;
+  uint64_t sweep_counter = 0;
+
+  // pinned host staging for per-sweep scalars
+  double *h_scal = nullptr;
+  uint32_t *h_cnt = nullptr;
+
+  Level &cur() { return level == 0 ? L0 : L1; }
+  void sync() { RMX_HIP(hipStreamSynchronize(stream)); }
+};
+
+// ----- graph.hip
+void upload_graph(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx,
+                  const double *w, const double *node_w);
+void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out);
+void build_buckets(Ctx &c, Level &L);
+
+// ----- flow.hip
+void steady_state(Ctx &c, int max_iter, int min_iter, double tol);
+void init_modules_singletons(Ctx &c);
+
+// ----- codelength.hip
+// Recomputes S = sum_m q_m and L from module statistics (tracked values).
+void module_totals(Ctx &c, double *S, double *L);
+void module_totals_async(Ctx &c);   // leaves S in c.scal[0], L in c.scal[1]
+void set_partition(Ctx &c, const uint32_t *d_part);
+double calculate_codelength(Ctx &c);
+void rebuild_exit_flows(Ctx &c);
+
+// ----- move.hip
+void ensure_sweep_buffers(Ctx &c);
+int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int64_t *edge_visits);
+void reset_active(Ctx &c);
+int64_t n_active(Ctx &c);
+
+// ----- contract.hip
+void update_members(Ctx &c);
+void contract(Ctx &c);
+void propagate_labels(Ctx &c);   // level 1 labels back to nodes
+void to_node_level(Ctx &c);      // api.cpp
+
+}  // namespace rmx

@@ -1,0 +1,476 @@
+// Steady-state flow: PageRank with teleportation and dangling mass as a CSR
+// pull SpMV (Network::calculateSteadyState, Module.cpp:340-469), then arc
+// flows, singleton modules and the initial codelength (Network::initiate,
+// Module.cpp:210-333, and calibrate, :474-501).
+//
+// Per iteration: p'_i = (alpha + beta D) tau_i + beta sum_{j->i} w_ji p_j,
+// p = p' / sum p', r = |p - p_prev|_1, D = sum of p over dangling nodes.
+// Stop after iteration k when k >= 200, or k >= 50 and r <= 1e-15.
+//
+// The pull is bucketed by in-degree: a group of G in {4..64} lanes per row
+// streams the row's in-arcs (coalesced in_idx / in_w) and gathers x; rows
+// with more than 256 in-arcs get a 256-lane workgroup.  Mixed precision
+// streams FP32 weights and an FP32 copy of x, accumulating in FP64; the
+// residual, the normalisation and D stay FP64.
+#include "context.h"
+
+#include <vector>
+
+namespace rmx {
+
+namespace {
+
+constexpr int kRedGrid = 1024;
+constexpr int kRedBlock = 256;
+enum { kS = 0, kR = 1, kD = 2, kL = 3, kSE = 4, kSS = 5, kSP = 6 };
+
+constexpr int kNumRowBins = 6;
+constexpr int kRowBinMax[kNumRowBins] = {4, 8, 16, 32, 256, 0x7fffffff};
+
+template <int G>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+template <int G>
+__device__ inline long long group_sum_ll(long long v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+
+// --- row-normalised out weights (Module.cpp:237-247) --------------------
+template <int G>
+__global__ void k_normalize(const uint32_t *__restrict__ optr, int32_t n, const double *__restrict__ raw,
+                            const uint32_t *__restrict__ in_of_out, double *__restrict__ out_w,
+                            double *__restrict__ in_w, float *__restrict__ in_wf, uint8_t *__restrict__ dangling) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = t / G;
+  const int lane = (int)(t % G);
+  if (r >= n) return;
+  const uint32_t b = optr[r], e = optr[r + 1];
+  double s = 0.0;
+  for (uint32_t k = b + lane; k < e; k += G) s += raw[k];
+  s = group_sum<G>(s);
+  if (lane == 0) dangling[r] = (b == e) ? 1 : 0;
+  for (uint32_t k = b + lane; k < e; k += G) {
+    double v = raw[k] / s;
+    out_w[k] = v;
+    uint32_t ik = in_of_out[k];
+    in_w[ik] = v;
+    in_wf[ik] = (float)v;
+  }
+}
+
+__global__ void k_row_bin(const uint32_t *__restrict__ iptr, int32_t n, uint32_t *__restrict__ key,
+                          uint32_t *__restrict__ cnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t d = iptr[i + 1] - iptr[i];
+  uint32_t b = 0;
+  while (b + 1 < (uint32_t)kNumRowBins && d > (uint32_t)kRowBinMax[b]) ++b;
+  key[i] = b;
+  atomicAdd(&cnt[b], 1u);
+}
+
+// --- pull SpMV ----------------------------------------------------------
+template <int G, bool MIXED>
+__global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rows, int32_t nrows,
+                                              const uint32_t *__restrict__ iptr, const uint32_t *__restrict__ iidx,
+                                              const double *__restrict__ iw, const float *__restrict__ iwf,
+                                              const double *__restrict__ x, const float *__restrict__ xf,
+                                              const double *__restrict__ tau, const double *__restrict__ scal,
+                                              double *__restrict__ y) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t g = t / G;
+  const int lane = (int)(t % G);
+  if (g >= nrows) return;
+  const uint32_t r = rows[g];
+  const uint32_t b = iptr[r], e = iptr[r + 1];
+  double acc = 0.0;
+  if (MIXED) {
+    for (uint32_t k = b + lane; k < e; k += G) acc += (double)(iwf[k] * xf[iidx[k]]);
+  } else {
+    for (uint32_t k = b + lane; k < e; k += G) acc = fma(iw[k], x[iidx[k]], acc);
+  }
+  acc = group_sum<G>(acc);
+  if (lane == 0) y[r] = (kAlpha + kBeta * scal[kD]) * tau[r] + kBeta * acc;
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_spmv_block(const uint32_t *__restrict__ rows, int32_t nrows,
+                                                    const uint32_t *__restrict__ iptr,
+                                                    const uint32_t *__restrict__ iidx, const double *__restrict__ iw,
+                                                    const float *__restrict__ iwf, const double *__restrict__ x,
+                                                    const float *__restrict__ xf, const double *__restrict__ tau,
+                                                    const double *__restrict__ scal, double *__restrict__ y) {
+  __shared__ double lds[4];
+  for (int64_t g = blockIdx.x; g < nrows; g += gridDim.x) {
+    const uint32_t r = rows[g];
+    const uint32_t b = iptr[r], e = iptr[r + 1];
+    double acc = 0.0;
+    if (MIXED) {
+      for (uint32_t k = b + threadIdx.x; k < e; k += 256) acc += (double)(iwf[k] * xf[iidx[k]]);
+    } else {
+      for (uint32_t k = b + threadIdx.x; k < e; k += 256) acc = fma(iw[k], x[iidx[k]], acc);
+    }
+    acc = block_sum<double, 256>(acc, lds);
+    if (threadIdx.x == 0) y[r] = (kAlpha + kBeta * scal[kD]) * tau[r] + kBeta * acc;
+  }
+}
+
+// --- reductions in a fixed launch shape (deterministic) ---------------------
+__global__ __launch_bounds__(kRedBlock) void k_sum_partials(const double *__restrict__ v, int32_t n,
+                                                            double *__restrict__ part) {
+  __shared__ double lds[kRedBlock / kWave];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n; i += (int64_t)kRedGrid * kRedBlock)
+    acc += v[i];
+  acc = block_sum<double, kRedBlock>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__device__ inline double reduce_partials(const double *__restrict__ part, double *lds) {
+  double a = 0.0;
+  for (int i = threadIdx.x; i < kRedGrid; i += kRedBlock) a += part[i];
+  a = block_sum<double, kRedBlock>(a, lds);
+  if (threadIdx.x == 0) lds[0] = a;
+  __syncthreads();
+  double s = lds[0];
+  __syncthreads();
+  return s;
+}
+
+// x <- y / sum(y); partials of |x_new - x_old| and of dangling mass
+__global__ __launch_bounds__(kRedBlock) void k_norm_step(const double *__restrict__ y, double *__restrict__ x,
+                                                         float *__restrict__ xf, const uint8_t *__restrict__ dang,
+                                                         int32_t n, const double *__restrict__ part_sum,
+                                                         double *__restrict__ part_r, double *__restrict__ part_d) {
+  __shared__ double lds[kRedBlock / kWave];
+  const double s = reduce_partials(part_sum, lds);
+  double r = 0.0, dm = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n; i += (int64_t)kRedGrid * kRedBlock) {
+    double pn = y[i] / s;
+    r += fabs(pn - x[i]);
+    if (dang[i]) dm += pn;
+    x[i] = pn;
+    xf[i] = (float)pn;
+  }
+  r = block_sum<double, kRedBlock>(r, lds);
+  if (threadIdx.x == 0) part_r[blockIdx.x] = r;
+  dm = block_sum<double, kRedBlock>(dm, lds);
+  if (threadIdx.x == 0) part_d[blockIdx.x] = dm;
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_final_rd(const double *__restrict__ part_r,
+                                                        const double *__restrict__ part_d, double *__restrict__ scal) {
+  __shared__ double lds[kRedBlock / kWave];
+  double r = reduce_partials(part_r, lds);
+  double d = reduce_partials(part_d, lds);
+  if (threadIdx.x == 0) {
+    scal[kR] = r;
+    scal[kD] = d;
+  }
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_init_x(double *__restrict__ x, float *__restrict__ xf,
+                                                      const uint8_t *__restrict__ dang, int32_t n,
+                                                      double *__restrict__ part_d, double *__restrict__ part_r) {
+  __shared__ double lds[kRedBlock / kWave];
+  const double v = 1.0 / n;
+  double dm = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n; i += (int64_t)kRedGrid * kRedBlock) {
+    x[i] = v;
+    xf[i] = (float)v;
+    if (dang[i]) dm += v;
+  }
+  dm = block_sum<double, kRedBlock>(dm, lds);
+  if (threadIdx.x == 0) {
+    part_d[blockIdx.x] = dm;
+    part_r[blockIdx.x] = 0.0;
+  }
+}
+
+// --- flows on arcs and node statistics (Module.cpp:267-325) ---------------
+template <int G>
+__global__ void k_flows(const uint32_t *__restrict__ optr, int32_t n, double *__restrict__ out_w,
+                        const uint32_t *__restrict__ in_of_out, double *__restrict__ in_w,
+                        float *__restrict__ out_wf, float *__restrict__ in_wf, const double *__restrict__ x,
+                        const double *__restrict__ tau, const uint8_t *__restrict__ dang, int mixed,
+                        double *__restrict__ p_out, long long *__restrict__ up, long long *__restrict__ ut,
+                        long long *__restrict__ ud, long long *__restrict__ uw, double *__restrict__ qlit) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = t / G;
+  const int lane = (int)(t % G);
+  if (r >= n) return;
+  const uint32_t b = optr[r], e = optr[r + 1];
+  const double pr = x[r];
+  double sf = 0.0;
+  long long sx = 0;
+  for (uint32_t k = b + lane; k < e; k += G) {
+    double f = pr * out_w[k];
+    float ff = (float)f;
+    out_w[k] = f;
+    out_wf[k] = ff;
+    uint32_t ik = in_of_out[k];
+    in_w[ik] = f;
+    in_wf[ik] = ff;
+    sf += f;
+    sx += fx_of(mixed ? (double)ff : f);
+  }
+  sf = group_sum<G>(sf);
+  sx = group_sum_ll<G>(sx);
+  if (lane == 0) {
+    const double tp = tau[r];
+    const bool dg = dang[r] != 0;
+    p_out[r] = pr;
+    up[r] = fx_of(pr);
+    ut[r] = fx_of(tp);
+    ud[r] = dg ? fx_of(pr) : 0;
+    uw[r] = sx;
+    qlit[r] = dg ? (1.0 - tp) * pr : kAlpha * (1.0 - tp) * pr + kBeta * sf;
+  }
+}
+
+__global__ void k_tau(const double *__restrict__ nw, int32_t n, const double *__restrict__ tot,
+                      double *__restrict__ tau) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) tau[i] = nw[i] / tot[0];
+}
+
+// calibrate over singleton modules with the node exits of Module.cpp:86-95
+__global__ __launch_bounds__(kRedBlock) void k_calib(const double *__restrict__ qlit, const double *__restrict__ p,
+                                                     int32_t n, double *__restrict__ part) {
+  __shared__ double lds[kRedBlock / kWave];
+  double s = 0, se = 0, ss = 0, sp = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n; i += (int64_t)kRedGrid * kRedBlock) {
+    double q = qlit[i], pi = p[i];
+    s += q;
+    se += plogp(q);
+    ss += plogp(q + pi);
+    sp += plogp(pi);
+  }
+  s = block_sum<double, kRedBlock>(s, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  se = block_sum<double, kRedBlock>(se, lds);
+  if (threadIdx.x == 0) part[kRedGrid + blockIdx.x] = se;
+  ss = block_sum<double, kRedBlock>(ss, lds);
+  if (threadIdx.x == 0) part[2 * kRedGrid + blockIdx.x] = ss;
+  sp = block_sum<double, kRedBlock>(sp, lds);
+  if (threadIdx.x == 0) part[3 * kRedGrid + blockIdx.x] = sp;
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_calib_final(const double *__restrict__ part, double *__restrict__ scal) {
+  __shared__ double lds[kRedBlock / kWave];
+  double s = reduce_partials(part, lds);
+  double se = reduce_partials(part + kRedGrid, lds);
+  double ss = reduce_partials(part + 2 * kRedGrid, lds);
+  double sp = reduce_partials(part + 3 * kRedGrid, lds);
+  if (threadIdx.x == 0) {
+    scal[kS] = s;
+    scal[kSP] = sp;
+    scal[kL] = plogp(s) - 2.0 * se + ss - sp;
+  }
+}
+
+template <typename T>
+__global__ void k_copy(const T *__restrict__ a, T *__restrict__ b, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+__global__ void k_singletons(int32_t n, uint32_t *__restrict__ mod, int32_t *__restrict__ nmem,
+                             int32_t *__restrict__ mNodes, int32_t *__restrict__ mUnits,
+                             int32_t *__restrict__ mStale) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  mod[i] = (uint32_t)i;
+  nmem[i] = 1;
+  mNodes[i] = 1;
+  mUnits[i] = 1;
+  mStale[i] = 1;
+}
+
+template <int G, bool MIXED>
+void launch_spmv_bin(int bin, const uint32_t *rows, int32_t cnt, const Level &L, const Ctx &c, double *y) {
+  (void)bin;
+  if (cnt <= 0) return;
+  k_spmv<G, MIXED><<<grid_for((int64_t)cnt * G, 256), 256, 0, c.stream>>>(
+      rows, cnt, L.in_ptr.p, L.in_idx.p, L.in_w.p, L.in_wf.p, c.x.p, c.xf.p, L.tau.p, c.scal.p, y);
+  RMX_LAUNCH_CHECK();
+}
+
+template <bool MIXED>
+void spmv_all(const Ctx &c, const Level &L, const uint32_t *rows, const int32_t *off, double *y) {
+  launch_spmv_bin<4, MIXED>(0, rows + off[0], off[1] - off[0], L, c, y);
+  launch_spmv_bin<8, MIXED>(1, rows + off[1], off[2] - off[1], L, c, y);
+  launch_spmv_bin<16, MIXED>(2, rows + off[2], off[3] - off[2], L, c, y);
+  launch_spmv_bin<32, MIXED>(3, rows + off[3], off[4] - off[3], L, c, y);
+  launch_spmv_bin<64, MIXED>(4, rows + off[4], off[5] - off[4], L, c, y);
+  int32_t big = off[6] - off[5];
+  if (big > 0) {
+    unsigned g = big < 4096 ? (unsigned)big : 4096u;
+    k_spmv_block<MIXED><<<g, 256, 0, c.stream>>>(rows + off[5], big, L.in_ptr.p, L.in_idx.p, L.in_w.p, L.in_wf.p,
+                                                 c.x.p, c.xf.p, L.tau.p, c.scal.p, y);
+    RMX_LAUNCH_CHECK();
+  }
+}
+
+}  // namespace
+
+// Row bins of the pull SpMV (in-degree classes), stable in row order.
+struct RowBins {
+  DevBuf<uint32_t> rows;
+  int32_t off[kNumRowBins + 1] = {0};
+};
+
+static void make_row_bins(Ctx &c, const Level &L, RowBins &rb) {
+  hipStream_t s = c.stream;
+  DevBuf<uint32_t> key, cnt;
+  key.alloc(L.n);
+  cnt.alloc(16);
+  fill_u32(cnt.p, 0u, 16, s);
+  k_row_bin<<<grid_for(L.n, 256), 256, 0, s>>>(L.in_ptr.p, L.n, key.p, cnt.p);
+  RMX_LAUNCH_CHECK();
+  rb.rows.alloc(L.n);
+  iota_u32(rb.rows.p, L.n, s);
+  radix_sort_pairs_u32(key.p, rb.rows.p, L.n, 0, 8, c.scratch, s);
+  uint32_t h[16];
+  RMX_HIP(hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  c.sync();
+  rb.off[0] = 0;
+  for (int b = 0; b < kNumRowBins; ++b) rb.off[b + 1] = rb.off[b] + (int32_t)h[b];
+}
+
+void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
+  hipStream_t s = c.stream;
+  Level &L = c.L0;
+  const int32_t n = L.n;
+  const int64_t m = L.m;
+  const bool mixed = c.precision == 1;
+  L.out_w.alloc(m);
+  L.in_w.alloc(m);
+  L.out_wf.alloc(m);
+  L.in_wf.alloc(m);
+  L.p.alloc(n);
+  L.tau.alloc(n);
+  L.up.alloc(n);
+  L.ut.alloc(n);
+  L.ud.alloc(n);
+  L.uw.alloc(n);
+  L.nmem.alloc(n);
+  L.mod.alloc(n);
+  c.dangling.alloc(n);
+  c.x.alloc(n);
+  c.y.alloc(n);
+  c.xf.alloc(n);
+  c.node_q.alloc(n);
+  c.scal.alloc(16);
+  c.partial.alloc(8 * kRedGrid);
+  double *part = c.partial.p;
+
+  // teleport weights tau = w / sum w (OmpRelaxmap.cpp:113-115)
+  k_sum_partials<<<kRedGrid, kRedBlock, 0, s>>>(c.node_w.p, n, part);
+  RMX_LAUNCH_CHECK();
+  k_final_rd<<<1, kRedBlock, 0, s>>>(part, part, c.scal.p);   // scal[kR] = scal[kD] = sum w
+  RMX_LAUNCH_CHECK();
+  k_tau<<<grid_for(n, 256), 256, 0, s>>>(c.node_w.p, n, c.scal.p + kR, L.tau.p);
+  RMX_LAUNCH_CHECK();
+
+  k_normalize<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, c.raw_w.p, c.in_of_out.p, L.out_w.p,
+                                                                  L.in_w.p, L.in_wf.p, c.dangling.p);
+  RMX_LAUNCH_CHECK();
+  RowBins rb;
+  make_row_bins(c, L, rb);
+
+  hipEvent_t e0, e1;
+  RMX_HIP(hipEventCreate(&e0));
+  RMX_HIP(hipEventCreate(&e1));
+  RMX_HIP(hipEventRecord(e0, s));
+  k_init_x<<<kRedGrid, kRedBlock, 0, s>>>(c.x.p, c.xf.p, c.dangling.p, n, part + 2 * kRedGrid, part + kRedGrid);
+  RMX_LAUNCH_CHECK();
+  k_final_rd<<<1, kRedBlock, 0, s>>>(part + kRedGrid, part + 2 * kRedGrid, c.scal.p);
+  RMX_LAUNCH_CHECK();
+  int it = 0;
+  double r = 1.0;
+  while (it < max_iter) {
+    if (mixed) spmv_all<true>(c, L, rb.rows.p, rb.off, c.y.p);
+    else spmv_all<false>(c, L, rb.rows.p, rb.off, c.y.p);
+    k_sum_partials<<<kRedGrid, kRedBlock, 0, s>>>(c.y.p, n, part);
+    RMX_LAUNCH_CHECK();
+    k_norm_step<<<kRedGrid, kRedBlock, 0, s>>>(c.y.p, c.x.p, c.xf.p, c.dangling.p, n, part, part + kRedGrid,
+                                               part + 2 * kRedGrid);
+    RMX_LAUNCH_CHECK();
+    k_final_rd<<<1, kRedBlock, 0, s>>>(part + kRedGrid, part + 2 * kRedGrid, c.scal.p);
+    RMX_LAUNCH_CHECK();
+    ++it;
+    if (it >= min_iter) {
+      RMX_HIP(hipMemcpyAsync(c.h_scal, c.scal.p, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+      c.sync();
+      r = c.h_scal[kR];
+      if (!(r > tol)) break;
+    }
+  }
+  RMX_HIP(hipEventRecord(e1, s));
+  RMX_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  RMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  c.timing.flow_ms = ms;
+  RMX_HIP(hipEventDestroy(e0));
+  RMX_HIP(hipEventDestroy(e1));
+  c.iterations = it;
+  c.resid = r;
+
+  // flows, node statistics and the literal node exits
+  k_flows<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, L.out_w.p, c.in_of_out.p, L.in_w.p,
+                                                              L.out_wf.p, L.in_wf.p, c.x.p, L.tau.p, c.dangling.p,
+                                                              mixed ? 1 : 0, L.p.p, L.up.p, L.ut.p, L.ud.p, L.uw.p,
+                                                              c.node_q.p);
+  RMX_LAUNCH_CHECK();
+  k_calib<<<kRedGrid, kRedBlock, 0, s>>>(c.node_q.p, L.p.p, n, part);
+  RMX_LAUNCH_CHECK();
+  k_calib_final<<<1, kRedBlock, 0, s>>>(part, c.scal.p);
+  RMX_LAUNCH_CHECK();
+  RMX_HIP(hipMemcpyAsync(c.h_scal, c.scal.p, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+  c.sync();
+  c.initial_codelength = c.h_scal[kL];
+  c.initial_sum_exit = c.h_scal[kS];
+  c.all_log_all = c.h_scal[kSP];
+  {
+    // number of dangling nodes
+    std::vector<uint8_t> dg(n);
+    RMX_HIP(hipMemcpyAsync(dg.data(), c.dangling.p, n, hipMemcpyDeviceToHost, s));
+    c.sync();
+    int64_t nd = 0;
+    for (int32_t i = 0; i < n; ++i) nd += dg[i];
+    c.n_dangling = nd;
+  }
+  c.x.free();
+  c.xf.free();
+  c.y.free();
+}
+
+void init_modules_singletons(Ctx &c) {
+  hipStream_t s = c.stream;
+  Level &L = c.L0;
+  const int32_t n = L.n;
+  c.mP.alloc(n);
+  c.mT.alloc(n);
+  c.mD.alloc(n);
+  c.mX.alloc(n);
+  c.mNodes.alloc(n);
+  c.mUnits.alloc(n);
+  c.mStale.alloc(n);
+  unsigned g = grid_for(n, 256);
+  k_copy<long long><<<g, 256, 0, s>>>(L.up.p, c.mP.p, n);
+  k_copy<long long><<<g, 256, 0, s>>>(L.ut.p, c.mT.p, n);
+  k_copy<long long><<<g, 256, 0, s>>>(L.ud.p, c.mD.p, n);
+  k_copy<long long><<<g, 256, 0, s>>>(L.uw.p, c.mX.p, n);
+  k_singletons<<<g, 256, 0, s>>>(n, L.mod.p, L.nmem.p, c.mNodes.p, c.mUnits.p, c.mStale.p);
+  RMX_LAUNCH_CHECK();
+  c.level = 0;
+  c.n_module = n;
+}
+
+}  // namespace rmx

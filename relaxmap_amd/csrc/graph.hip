@@ -1,0 +1,166 @@
+// Graph upload, in-CSR construction and degree bucketing.
+//
+// The host hands over the out-CSR the reference builds in
+// OmpRelaxmap.cpp:117-135 (arcs sorted by (src, dst), self-loops and w <= 0
+// removed).  In-links are built on the device by a stable radix sort of the
+// arcs by destination, so every in-list is in ascending source order as the
+// reference's push_back loop produces it (OmpRelaxmap.cpp:158-162).
+#include "context.h"
+
+#include <vector>
+
+namespace rmx {
+
+void Level::free_all() {
+  out_ptr.free(); out_idx.free(); in_ptr.free(); in_idx.free();
+  out_w.free(); in_w.free(); out_wf.free(); in_wf.free(); out_wx.free(); in_wx.free();
+  p.free(); tau.free(); up.free(); ut.free(); ud.free(); uw.free();
+  nmem.free(); mod.free(); bucket.free();
+  n = 0; m = 0;
+}
+
+namespace {
+
+// src of every arc: one wave per row strides over the row's arcs
+__global__ void k_arc_src(const uint32_t *__restrict__ ptr, int32_t n, uint32_t *__restrict__ src) {
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (wave >= n) return;
+  for (uint32_t e = ptr[wave] + lane; e < ptr[wave + 1]; e += kWave) src[e] = (uint32_t)wave;
+}
+
+__global__ void k_degree_hist(const uint32_t *__restrict__ dst, int64_t m, uint32_t *__restrict__ cnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < m; i += stride) atomicAdd(&cnt[dst[i]], 1u);
+}
+
+// after the stable sort: in_idx[k] = src[e_k], in_of_out[e_k] = k
+__global__ void k_in_from_sorted(const uint32_t *__restrict__ sorted_e, const uint32_t *__restrict__ src,
+                                 int64_t m, uint32_t *__restrict__ in_idx, uint32_t *__restrict__ in_of_out) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; k < m; k += stride) {
+    uint32_t e = sorted_e[k];
+    in_idx[k] = src[e];
+    if (in_of_out) in_of_out[e] = (uint32_t)k;
+  }
+}
+
+__global__ void k_bucket_key(const uint32_t *__restrict__ optr, const uint32_t *__restrict__ iptr, int32_t n,
+                             uint32_t *__restrict__ key, uint32_t *__restrict__ maxdeg) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t deg = (optr[i + 1] - optr[i]) + (iptr[i + 1] - iptr[i]);
+  uint32_t b = 0;
+  while (b + 1 < (uint32_t)kNumBuckets && deg > (uint32_t)kBucketMaxDeg[b]) ++b;
+  key[i] = b;
+  atomicMax(maxdeg, deg);
+}
+
+__global__ void k_small_hist(const uint32_t *__restrict__ key, int64_t n, uint32_t *__restrict__ cnt) {
+  __shared__ uint32_t h[16];
+  if (threadIdx.x < 16) h[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) atomicAdd(&h[key[i] & 15u], 1u);
+  __syncthreads();
+  if (threadIdx.x < 16 && h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
+}
+
+unsigned gs_grid(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > 16384) g = 16384;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+}  // namespace
+
+void upload_graph(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx,
+                  const double *w, const double *node_w) {
+  if (n <= 0) throw HipError("network has no nodes");
+  if (m >= (int64_t)0xffffffffLL) throw HipError("more than 2^32-2 arcs");
+  hipStream_t s = c.stream;
+  Level &L = c.L0;
+  L.free_all();
+  L.n = n;
+  L.m = m;
+  c.N = n;
+  c.E = m;
+  std::vector<uint32_t> p32((size_t)n + 1);
+  for (int32_t i = 0; i <= n; ++i) p32[i] = (uint32_t)out_ptr[i];
+  L.out_ptr.alloc(n + 1);
+  L.out_idx.alloc(m);
+  c.raw_w.alloc(m);
+  c.node_w.alloc(n);
+  RMX_HIP(hipMemcpyAsync(L.out_ptr.p, p32.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, s));
+  if (m) {
+    RMX_HIP(hipMemcpyAsync(L.out_idx.p, out_idx, sizeof(uint32_t) * m, hipMemcpyHostToDevice, s));
+    RMX_HIP(hipMemcpyAsync(c.raw_w.p, w, sizeof(double) * m, hipMemcpyHostToDevice, s));
+  }
+  std::vector<double> nw((size_t)n, 1.0);
+  if (node_w)
+    for (int32_t i = 0; i < n; ++i) nw[i] = node_w[i] > 0.0 ? node_w[i] : 1.0;
+  RMX_HIP(hipMemcpyAsync(c.node_w.p, nw.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  c.sync();   // host staging vectors go out of scope
+  build_in_csr(c, L, &c.in_of_out);
+  build_buckets(c, L);
+  c.initiated = false;
+}
+
+void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) {
+  hipStream_t s = c.stream;
+  const int64_t m = L.m;
+  const int32_t n = L.n;
+  L.in_ptr.alloc(n + 1);
+  L.in_idx.alloc(m);
+  if (in_of_out) in_of_out->alloc(m);
+  DevBuf<uint32_t> cnt;
+  cnt.alloc(n + 1);
+  fill_u32(cnt.p, 0u, n + 1, s);
+  if (m > 0) {
+    DevBuf<uint32_t> src, keys, vals;
+    src.alloc(m);
+    keys.alloc(m);
+    vals.alloc(m);
+    k_arc_src<<<grid_for((int64_t)n * kWave, 256), 256, 0, s>>>(L.out_ptr.p, n, src.p);
+    RMX_LAUNCH_CHECK();
+    k_degree_hist<<<gs_grid(m), 256, 0, s>>>(L.out_idx.p, m, cnt.p);
+    RMX_LAUNCH_CHECK();
+    RMX_HIP(hipMemcpyAsync(keys.p, L.out_idx.p, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, s));
+    iota_u32(vals.p, m, s);
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) < (int64_t)n) ++bits;
+    radix_sort_pairs_u32(keys.p, vals.p, m, 0, bits, c.scratch, s);
+    k_in_from_sorted<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, m, L.in_idx.p, in_of_out ? in_of_out->p : nullptr);
+    RMX_LAUNCH_CHECK();
+    c.sync();
+  }
+  exclusive_scan_u32(cnt.p, L.in_ptr.p, n + 1, nullptr, c.scratch, s);
+  c.sync();
+}
+
+void build_buckets(Ctx &c, Level &L) {
+  hipStream_t s = c.stream;
+  const int32_t n = L.n;
+  DevBuf<uint32_t> keys, cnt;
+  keys.alloc(n);
+  cnt.alloc(16 + 1);
+  fill_u32(cnt.p, 0u, 17, s);
+  L.bucket.alloc(n);
+  k_bucket_key<<<grid_for(n, 256), 256, 0, s>>>(L.out_ptr.p, L.in_ptr.p, n, keys.p, cnt.p + 16);
+  RMX_LAUNCH_CHECK();
+  k_small_hist<<<gs_grid(n), 256, 0, s>>>(keys.p, n, cnt.p);
+  RMX_LAUNCH_CHECK();
+  iota_u32(L.bucket.p, n, s);
+  radix_sort_pairs_u32(keys.p, L.bucket.p, n, 0, 8, c.scratch, s);
+  uint32_t h[17];
+  RMX_HIP(hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  c.sync();
+  L.bucket_off[0] = 0;
+  for (int b = 0; b < kNumBuckets; ++b) L.bucket_off[b + 1] = L.bucket_off[b] + (int32_t)h[b];
+  L.max_deg = (int32_t)h[16];
+}
+
+}  // namespace rmx

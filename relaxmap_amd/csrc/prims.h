@@ -1,0 +1,35 @@
+// Device primitives: scans, stable LSD radix sort, fills.  All deterministic.
+#pragma once
+
+#include "common.h"
+
+namespace rmx {
+
+// Scratch arena owned by the caller; grows on demand (hipMallocAsync-free:
+// the context allocates once and reuses).
+struct Scratch {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  void *get(size_t need);
+  void release();
+};
+
+// out[i] = sum_{j<i} in[i]; *d_total (device, may be null) = sum of all.
+// In-place (out == in) is allowed.  n < 2^32.
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *d_total,
+                        Scratch &scratch, hipStream_t s);
+
+// Stable LSD radix sort of (key, val) pairs on bits [begin_bit, end_bit).
+// keys/vals are sorted in place; tmp buffers of n elements are taken from
+// scratch.  Stability: equal keys keep their input order.
+void radix_sort_pairs_u32(uint32_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit,
+                          Scratch &scratch, hipStream_t s);
+void radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit,
+                          Scratch &scratch, hipStream_t s);
+
+void fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
+void fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s);
+void fill_i64(long long *p, long long v, int64_t n, hipStream_t s);
+void iota_u32(uint32_t *p, int64_t n, hipStream_t s);
+
+}  // namespace rmx

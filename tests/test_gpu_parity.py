@@ -1,0 +1,156 @@
+"""GPU path (librelaxmap.so through the C ABI) against the oracle and the
+reference's golden fixtures.
+
+Tolerances (written per check): steady-state flows 1e-12 in L1; codelength
+of a fixed partition 1e-10 relative; contraction bit-exact in its integer
+structure, arc weights 1e-13 relative (the weights are sums of flows, which
+carry the flows' last-bit differences); sweeps are relaxed (batched moves),
+so partitions are judged by codelength consistency and determinism here and
+by final quality in test_gpu_quality.py.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import INPUTS, data_path, golden
+
+pytestmark = pytest.mark.gpu
+
+ALL = list(INPUTS)
+PART_KEYS = [k for k in ALL if k != "lfr5k_mu05"]
+
+
+def _gpu(key, **kw):
+    from relaxmap_amd import Network
+    net = Network(**kw)
+    net.load(data_path(INPUTS[key]))
+    net.initiate()
+    return net
+
+
+@pytest.mark.parametrize("key", ALL)
+def test_flows(key):
+    g = golden(f"{key}.flows")
+    with _gpu(key) as net:
+        print(f"[gpu] flows {key} n={net.n} m={net.m}", flush=True)
+        assert net.n == g["n"]
+        p = net.flows()
+        assert np.abs(p - np.array(g["p"])).sum() <= 1e-12            # L1, FP64
+        assert abs(net.iterations - g["iterations"]) <= 1
+        q = net.node_exit()
+        assert np.allclose(q, np.array(g["q"]), rtol=1e-12, atol=1e-17)
+        assert net.n_dangling == g["n_dangling"]
+        assert abs(net.initial_codelength - g["codelength"]) <= 1e-10 * abs(g["codelength"])
+        # tracked (fixed-point) statistics agree with the literal calibrate
+        L, S, nm = net.tracked()
+        assert nm == g["n"]
+        assert abs(L - g["codelength"]) <= 1e-10 * abs(g["codelength"])
+        assert abs(S - g["sum_exit"]) <= 1e-12
+
+
+@pytest.mark.parametrize("key", ["lfr1k_mu03", "sbm1k"])
+def test_flows_mixed_precision(key):
+    g = golden(f"{key}.flows")
+    with _gpu(key, precision="mixed") as net:
+        p = net.flows()
+        err = np.abs(p - np.array(g["p"])).sum()
+        print(f"[gpu] mixed flows {key} L1 error {err:.3e}", flush=True)
+        assert err <= 1e-6                                            # L1 vs FP64
+
+
+@pytest.mark.parametrize("key", PART_KEYS)
+@pytest.mark.parametrize("pname", ["swept", "random"])
+def test_partition_codelength(key, pname):
+    g = golden(f"{key}.part_{pname}")
+    with _gpu(key) as net:
+        net.set_partition(np.array(g["partition"], dtype=np.int32))
+        calc = net.calculate_codelength()
+        ref = g["codelength"]["calculated"]
+        assert abs(calc - ref) <= 1e-10 * abs(ref)
+        L, S, nm = net.tracked()
+        assert abs(L - g["codelength"]["codelength"]) <= 1e-10 * abs(ref)
+        assert abs(S - g["codelength"]["sum_exit"]) <= 1e-12
+        assert nm == g["codelength"]["n_module"]
+        assert np.array_equal(net.partition(), np.array(g["partition"]))
+
+
+@pytest.mark.parametrize("key", PART_KEYS)
+@pytest.mark.parametrize("pname", ["swept", "random"])
+def test_contraction(key, pname):
+    g = golden(f"{key}.part_{pname}")
+    c = g["contract"]
+    with _gpu(key) as net:
+        net.set_partition(np.array(g["partition"], dtype=np.int32))
+        ns, ms = net.convert_modules_to_super_nodes()
+        assert ns == c["n_super"]
+        s = net.super_nodes()
+        ref_out_idx = [e[0] for sp in c["super"] for e in sp["out"]]
+        ref_out_w = [e[1] for sp in c["super"] for e in sp["out"]]
+        ref_in_idx = [e[0] for sp in c["super"] for e in sp["in"]]
+        ref_in_w = [e[1] for sp in c["super"] for e in sp["in"]]
+        ref_out_ptr = np.cumsum([0] + [len(sp["out"]) for sp in c["super"]])
+        ref_in_ptr = np.cumsum([0] + [len(sp["in"]) for sp in c["super"]])
+        # integer structure: bit-exact
+        assert ms == len(ref_out_idx)
+        assert np.array_equal(s["out_ptr"], ref_out_ptr)
+        assert np.array_equal(s["in_ptr"], ref_in_ptr)
+        assert s["out_idx"].tolist() == ref_out_idx
+        assert s["in_idx"].tolist() == ref_in_idx
+        assert s["mod"].tolist() == [sp["mod"] for sp in c["super"]]
+        assert s["n_members"].tolist() == [sp["n_members"] for sp in c["super"]]
+        # weights: sums of flows, 1e-13 relative
+        assert np.allclose(s["out_w"], ref_out_w, rtol=1e-13, atol=0)
+        assert np.allclose(s["in_w"], ref_in_w, rtol=1e-13, atol=0)
+        assert np.allclose(s["size"], [sp["size"] for sp in c["super"]], rtol=1e-12, atol=1e-18)
+        assert np.allclose(s["tp"], [sp["tp"] for sp in c["super"]], rtol=1e-12, atol=1e-18)
+        assert np.allclose(s["dangling"], [sp["dangling"] for sp in c["super"]], rtol=1e-12, atol=1e-18)
+        assert np.allclose(s["exit"], [sp["exit"] for sp in c["super"]], rtol=1e-10, atol=1e-17)
+
+
+@pytest.mark.parametrize("key", ["lfr1k_mu03", "sbm1k", "lfr5k_mu05"])
+@pytest.mark.parametrize("prior", [True, False])
+def test_sweeps_consistent(key, prior):
+    """The tracked codelength after every sweep equals the from-scratch
+    codelength of the partition it describes."""
+    with _gpu(key) as net:
+        L_prev = net.codelength
+        for it in range(4):
+            moved = net.move(prioritized=prior, seed=3)
+            L = net.codelength
+            calc = net.calculate_codelength()
+            print(f"[gpu] sweep {key} prior={prior} it={it} moved={moved} L={L / math.log(2):.6f}", flush=True)
+            assert abs(L - calc) <= 1e-9 * abs(calc)
+            assert L <= L_prev + 1e-9
+            L_prev = L
+        part = net.partition()
+        assert len(np.unique(part)) == net.n_module
+
+
+@pytest.mark.parametrize("key", ["lfr1k_mu03", "sbm1k"])
+def test_sweeps_deterministic(key):
+    parts = []
+    for _ in range(2):
+        with _gpu(key) as net:
+            for _ in range(3):
+                net.move(prioritized=True, seed=11)
+            parts.append((net.partition(), net.codelength))
+    assert np.array_equal(parts[0][0], parts[1][0])
+    assert parts[0][1] == parts[1][1]
+
+
+@pytest.mark.parametrize("key", ["lfr1k_mu03", "sbm1k"])
+def test_supernode_sweeps_consistent(key):
+    with _gpu(key) as net:
+        for _ in range(2):
+            net.move(prioritized=True, seed=5)
+        net.update_members_in_module()
+        net.convert_modules_to_super_nodes()
+        for _ in range(3):
+            net.move(prioritized=True, seed=6)
+            L = net.codelength
+            calc = net.calculate_codelength()
+            assert abs(L - calc) <= 1e-9 * abs(calc)
+        net.to_node_level()
+        net.move(prioritized=True, seed=7)
+        assert abs(net.codelength - net.calculate_codelength()) <= 1e-9 * abs(net.codelength)
