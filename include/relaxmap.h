@@ -140,6 +140,14 @@ int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_partition_re
 int rmx_write_tree(rmx_ctx *ctx, const char *path);
 int rmx_write_clu(rmx_ctx *ctx, const char *path);
 
+/* Profiling mode (off by default): every hot kernel launch is bracketed by
+ * HIP events on the context's stream and accounted with its algorithmic HBM
+ * bytes.  rmx_kernel_stats(ctx, i, ...) returns entry i (name, launches,
+ * total milliseconds, total bytes); with i < 0 it returns the entry count.
+ * No reference counterpart (the reference prints wall-clock phases only). */
+int rmx_set_profiling(rmx_ctx *ctx, int on);
+int rmx_kernel_stats(rmx_ctx *ctx, int i, char *name, int name_len, int64_t *launches, double *ms, double *bytes);
+
 /* Kernel timings of the last rmx_initiate / rmx_partition (milliseconds). */
 int rmx_timing(rmx_ctx *ctx, double *flow_ms, double *sweep_ms, double *contract_ms, int64_t *edge_visits,
                int64_t *sweeps);

@@ -89,6 +89,8 @@ def load_library():
         "rmx_write_tree": (C.c_int, [vp, C.c_char_p]),
         "rmx_write_clu": (C.c_int, [vp, C.c_char_p]),
         "rmx_timing": (C.c_int, [vp, _F64P, _F64P, _F64P, _I64P, _I64P]),
+        "rmx_set_profiling": (C.c_int, [vp, C.c_int]),
+        "rmx_kernel_stats": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int, _I64P, _F64P, _F64P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -274,6 +276,20 @@ class Network:
 
     def write_clu(self, path):
         self._check(self._L.rmx_write_clu(self._h, str(path).encode()))
+
+    def set_profiling(self, on=True):
+        self._check(self._L.rmx_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_stats(self):
+        """{name: (launches, total ms, total algorithmic bytes)} of the profiling mode."""
+        k = self._L.rmx_kernel_stats(self._h, -1, None, 0, None, None, None)
+        out = {}
+        for i in range(k):
+            name = C.create_string_buffer(64)
+            n, ms, b = C.c_int64(), C.c_double(), C.c_double()
+            self._check(self._L.rmx_kernel_stats(self._h, i, name, 64, C.byref(n), C.byref(ms), C.byref(b)))
+            out[name.value.decode()] = (n.value, ms.value, b.value)
+        return out
 
     def timing(self):
         f, s, c, e, k = C.c_double(), C.c_double(), C.c_double(), C.c_int64(), C.c_int64()

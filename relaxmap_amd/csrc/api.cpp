@@ -54,14 +54,16 @@ void free_ctx_device(Ctx &c) {
   c.L0.free_all();
   c.L1.free_all();
   c.node_w.free(); c.raw_w.free(); c.dangling.free(); c.in_of_out.free(); c.node_sp.free();
-  c.mP.free(); c.mT.free(); c.mD.free(); c.mX.free(); c.mNodes.free(); c.mUnits.free(); c.mStale.free();
+  c.ms.free(); c.mNodes.free(); c.mUnits.free(); c.mStale.free();
   c.mInst.free();
   c.act.free(); c.act_next.free(); c.target.free(); c.newmod.free(); c.stamp.free(); c.movers.free();
   c.empty_list.free(); c.want_empty.free(); c.rank_tmp.free(); c.agg_buf.free();
-  c.hub_keys.free(); c.hub_out.free(); c.hub_in.free();
+  c.batch_keys.free(); c.batch_list.free();
+  c.hub_keys.free(); c.hub_out.free(); c.hub_in.free(); c.hub_plan.free(); c.hub_bm.free(); c.hub_bd.free();
   c.counters.free(); c.scal.free(); c.partial.free(); c.x.free(); c.y.free(); c.xf.free(); c.node_q.free();
   c.scratch.release();
   c.scratch2.release();
+  c.cache.release();
 }
 
 double now_s() {
@@ -124,7 +126,7 @@ int rmx_set_precision(rmx_ctx *ctx, int precision) {
 
 int rmx_set_batches(rmx_ctx *ctx, int batches) {
   return guarded(ctx, [&] {
-    if (batches < 1 || batches > 1024) return fail(ctx, RMX_EINVAL, "batches must be in 1..1024");
+    if (batches < 1 || batches > 32) return fail(ctx, RMX_EINVAL, "batches must be in 1..32");
     ctx->c.batches = batches;
     return RMX_OK;
   });
@@ -242,7 +244,7 @@ int rmx_set_partition(rmx_ctx *ctx, const int32_t *part) {
     for (int32_t i = 0; i < c.N; ++i)
       if (part[i] < 0 || part[i] >= c.N) return fail(ctx, RMX_EINVAL, "module index out of range");
     c.level = 0;
-    DevBuf<uint32_t> d;
+    DevBuf<uint32_t> d(c.cache);
     d.alloc(c.N);
     RMX_HIP(hipMemcpyAsync(d.p, part, sizeof(uint32_t) * c.N, hipMemcpyHostToDevice, c.stream));
     set_partition(c, d.p);
@@ -309,18 +311,15 @@ int rmx_get_super(rmx_ctx *ctx, double *size, double *exit, double *tp, double *
     Ctx &c = ctx->c;
     if (c.level != 1) return fail(ctx, RMX_ESTATE, "no super nodes (rmx_contract first)");
     const int32_t n = c.L1.n;
-    std::vector<long long> up(n), ut(n), ud(n), uw(n);
+    std::vector<Stat4> us(n);
     std::vector<int32_t> nm(n);
     std::vector<uint32_t> md(n);
-    RMX_HIP(hipMemcpyAsync(up.data(), c.L1.up.p, sizeof(long long) * n, hipMemcpyDeviceToHost, c.stream));
-    RMX_HIP(hipMemcpyAsync(ut.data(), c.L1.ut.p, sizeof(long long) * n, hipMemcpyDeviceToHost, c.stream));
-    RMX_HIP(hipMemcpyAsync(ud.data(), c.L1.ud.p, sizeof(long long) * n, hipMemcpyDeviceToHost, c.stream));
-    RMX_HIP(hipMemcpyAsync(uw.data(), c.L1.uw.p, sizeof(long long) * n, hipMemcpyDeviceToHost, c.stream));
+    RMX_HIP(hipMemcpyAsync(us.data(), c.L1.us.p, sizeof(Stat4) * n, hipMemcpyDeviceToHost, c.stream));
     RMX_HIP(hipMemcpyAsync(nm.data(), c.L1.nmem.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c.stream));
     RMX_HIP(hipMemcpyAsync(md.data(), c.L1.mod.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c.stream));
     c.sync();
     for (int32_t i = 0; i < n; ++i) {
-      const double P = fx_to(up[i]), T = fx_to(ut[i]), D = fx_to(ud[i]), X = fx_to(uw[i]);
+      const double P = fx_to(us[i].p), T = fx_to(us[i].t), D = fx_to(us[i].d), X = fx_to(us[i].x);
       if (size) size[i] = P;
       if (tp) tp[i] = T;
       if (dangling) dangling[i] = D;
@@ -533,6 +532,28 @@ extern "C" int rmx_write_clu(rmx_ctx *ctx, const char *path) {
     c.sync();
     std::string err;
     if (write_clu(path, mod, err) != 0) return fail(ctx, RMX_EIO, err);
+    return RMX_OK;
+  });
+}
+
+extern "C" int rmx_set_profiling(rmx_ctx *ctx, int on) {
+  return guarded(ctx, [&] {
+    ctx->c.prof.on = on != 0;
+    if (on) ctx->c.prof.stats.clear();
+    return RMX_OK;
+  });
+}
+
+extern "C" int rmx_kernel_stats(rmx_ctx *ctx, int i, char *name, int name_len, int64_t *launches, double *ms,
+                                double *bytes) {
+  return guarded(ctx, [&] {
+    const auto &st = ctx->c.prof.stats;
+    if (i < 0) return (int)st.size();
+    if (i >= (int)st.size()) return fail(ctx, RMX_EINVAL, "no such kernel entry");
+    if (name && name_len > 0) std::snprintf(name, (size_t)name_len, "%s", st[i].name.c_str());
+    if (launches) *launches = st[i].launches;
+    if (ms) *ms = st[i].ms;
+    if (bytes) *bytes = st[i].bytes;
     return RMX_OK;
   });
 }

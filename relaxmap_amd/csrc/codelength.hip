@@ -8,9 +8,10 @@
 // as in Network::calibrate (Module.cpp:474-501).
 //
 // From-scratch form (Network::calculateCodeLength, Module.cpp:2966-3002):
-// nodes are stably sorted by module, so every module's members are visited
-// in ascending node order and their out-arcs in stored order, and each
-// module's sums are formed in FP64 in exactly the reference's order.
+// every node's exit-arc flow, then nodes stably sorted by module and one
+// wavefront per module summing its members in FP64 (a fixed tree order, so
+// the result is reproducible; it differs from the reference's sequential
+// order by rounding only).
 #include "devutil.h"
 
 namespace rmx {
@@ -32,20 +33,18 @@ __device__ inline double reduce_partials(const double *__restrict__ part, double
   return s;
 }
 
-__global__ __launch_bounds__(kRedBlock) void k_module_totals(const long long *__restrict__ mP,
-                                                             const long long *__restrict__ mT,
-                                                             const long long *__restrict__ mD,
-                                                             const long long *__restrict__ mX,
+__global__ __launch_bounds__(kRedBlock) void k_module_totals(const Stat4 *__restrict__ ms,
                                                              const int32_t *__restrict__ mNodes, int32_t n,
                                                              double *__restrict__ part) {
   __shared__ double lds[kRedBlock / kWave];
   double s = 0, se = 0, ss = 0, cnt = 0;
   for (int64_t m = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; m < n; m += (int64_t)kRedGrid * kRedBlock) {
     if (mNodes[m] > 0) {
-      double q = module_exit(mP[m], mT[m], mD[m], mX[m]);
+      const Stat4 st = ms[m];
+      double q = module_exit(st.p, st.t, st.d, st.x);
       s += q;
       se += plogp(q);
-      ss += plogp(q + fx_to(mP[m]));
+      ss += plogp(q + fx_to(st.p));
       cnt += 1.0;
     }
   }
@@ -73,24 +72,22 @@ __global__ __launch_bounds__(kRedBlock) void k_totals_final(const double *__rest
   }
 }
 
-__global__ void k_zero_modules(int32_t n, long long *mP, long long *mT, long long *mD, long long *mX,
-                               int32_t *mNodes, int32_t *mUnits) {
+__global__ void k_zero_modules(int32_t n, Stat4 *ms, int32_t *mNodes, int32_t *mUnits) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  mP[i] = 0; mT[i] = 0; mD[i] = 0; mX[i] = 0;
+  ms[i] = Stat4{0, 0, 0, 0};
   mNodes[i] = 0; mUnits[i] = 0;
 }
 
-__global__ void k_accumulate_units(int32_t n, const uint32_t *__restrict__ mod, const long long *__restrict__ up,
-                                   const long long *__restrict__ ut, const long long *__restrict__ ud,
-                                   const int32_t *__restrict__ nmem, long long *mP, long long *mT, long long *mD,
-                                   int32_t *mNodes, int32_t *mUnits) {
+__global__ void k_accumulate_units(int32_t n, const uint32_t *__restrict__ mod, const Stat4 *__restrict__ us,
+                                   const int32_t *__restrict__ nmem, Stat4 *ms, int32_t *mNodes, int32_t *mUnits) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= n) return;
   uint32_t m = mod[u];
-  atomicAdd((unsigned long long *)&mP[m], (unsigned long long)up[u]);
-  atomicAdd((unsigned long long *)&mT[m], (unsigned long long)ut[u]);
-  atomicAdd((unsigned long long *)&mD[m], (unsigned long long)ud[u]);
+  const Stat4 st = us[u];
+  atomicAdd((unsigned long long *)&ms[m].p, (unsigned long long)st.p);
+  atomicAdd((unsigned long long *)&ms[m].t, (unsigned long long)st.t);
+  atomicAdd((unsigned long long *)&ms[m].d, (unsigned long long)st.d);
   atomicAdd(&mNodes[m], nmem[u]);
   atomicAdd(&mUnits[m], 1);
 }
@@ -98,7 +95,7 @@ __global__ void k_accumulate_units(int32_t n, const uint32_t *__restrict__ mod, 
 // X_M = sum over arcs a->b with a in M, b not in M of the arc flow
 template <int G>
 __global__ void k_exit_flows(int32_t n, const uint32_t *__restrict__ optr, const uint32_t *__restrict__ oidx,
-                             ArcW aw, const uint32_t *__restrict__ mod, long long *mX) {
+                             ArcW aw, const uint32_t *__restrict__ mod, Stat4 *ms) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t u = t / G;
   const int lane = (int)(t % G);
@@ -109,7 +106,12 @@ __global__ void k_exit_flows(int32_t n, const uint32_t *__restrict__ optr, const
     if (mod[oidx[k]] != m) acc += aw.fxw(k);
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
-  if (lane == 0 && acc != 0) atomicAdd((unsigned long long *)&mX[m], (unsigned long long)acc);
+  if (lane == 0 && acc != 0) atomicAdd((unsigned long long *)&ms[m].x, (unsigned long long)acc);
+}
+
+__global__ void k_zero_exit(int32_t n, Stat4 *ms) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ms[i].x = 0;
 }
 
 template <typename T>
@@ -131,34 +133,54 @@ __global__ void k_seg_starts(const uint32_t *__restrict__ flag, const uint32_t *
   if (flag[i]) start[pos[i]] = (uint32_t)i;
 }
 
-// one thread per module: sums in the reference's order (members ascending,
-// out-arcs in stored order), Module.cpp:2976-2995
+// flow on each node's out-arcs that leave its module (FP64)
+template <int G>
+__global__ void k_p5_ext(int32_t n, const uint32_t *__restrict__ optr, const uint32_t *__restrict__ oidx,
+                         const double *__restrict__ ow, const uint32_t *__restrict__ mod, double *__restrict__ ext) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t a = t / G;
+  const int lane = (int)(t % G);
+  if (a >= n) return;
+  const uint32_t m = mod[a];
+  double acc = 0.0;
+  for (uint32_t k = optr[a] + lane; k < optr[a + 1]; k += G)
+    if (mod[oidx[k]] != m) acc += ow[k];
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+  if (lane == 0) ext[a] = acc;
+}
+
+// one wavefront per module (its members are a segment of the module-sorted
+// node list): FP64 sums of flow, teleport, dangling flow and exit-arc flow,
+// then the module exit of Module.cpp:2978-2991 in the reference's form
 __global__ void k_p5_modules(const uint32_t *__restrict__ start, const uint32_t *__restrict__ nseg_p, int32_t n,
-                             const uint32_t *__restrict__ key, const uint32_t *__restrict__ node,
-                             const double *__restrict__ p, const double *__restrict__ tau,
-                             const uint8_t *__restrict__ dang, const uint32_t *__restrict__ optr,
-                             const uint32_t *__restrict__ oidx, const double *__restrict__ ow,
-                             const uint32_t *__restrict__ mod, double *__restrict__ segq,
-                             double *__restrict__ segqp) {
+                             const uint32_t *__restrict__ node, const double *__restrict__ p,
+                             const double *__restrict__ tau, const uint8_t *__restrict__ dang,
+                             const double *__restrict__ ext, double *__restrict__ segq, double *__restrict__ segqp) {
   const uint32_t nseg = *nseg_p;
-  int64_t sidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t sidx = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
   if (sidx >= nseg) return;
   const uint32_t b = start[sidx];
   const uint32_t e = (sidx + 1 < nseg) ? start[sidx + 1] : (uint32_t)n;
-  const uint32_t m = key[b];
-  double P = 0.0, T = 0.0, D = 0.0, ext = 0.0;
-  for (uint32_t i = b; i < e; ++i) {
+  double P = 0.0, T = 0.0, D = 0.0, X = 0.0;
+  for (uint32_t i = b + lane; i < e; i += kWave) {
     const uint32_t a = node[i];
     P += p[a];
     T += tau[a];
     if (dang[a]) D += p[a];
-    for (uint32_t k = optr[a]; k < optr[a + 1]; ++k)
-      if (mod[oidx[k]] != m) ext += ow[k];
+    X += ext[a];
   }
-  double q = kAlpha * (1.0 - T) * P;
-  q += kBeta * (ext + (1.0 - T) * D);
-  segq[sidx] = q;
-  segqp[sidx] = q + P;
+  P = wave_sum(P);
+  T = wave_sum(T);
+  D = wave_sum(D);
+  X = wave_sum(X);
+  if (lane == 0) {
+    double q = kAlpha * (1.0 - T) * P;
+    q += kBeta * (X + (1.0 - T) * D);
+    segq[sidx] = q;
+    segqp[sidx] = q + P;
+  }
 }
 
 __global__ __launch_bounds__(kRedBlock) void k_p5_partials(const double *__restrict__ segq,
@@ -203,7 +225,7 @@ ArcW arcw_in(const Ctx &c, const Level &L) {
 }
 
 void module_totals_async(Ctx &c) {
-  k_module_totals<<<kRedGrid, kRedBlock, 0, c.stream>>>(c.mP.p, c.mT.p, c.mD.p, c.mX.p, c.mNodes.p, c.N,
+  k_module_totals<<<kRedGrid, kRedBlock, 0, c.stream>>>(c.ms.p, c.mNodes.p, c.N,
                                                          c.partial.p);
   RMX_LAUNCH_CHECK();
   k_totals_final<<<1, kRedBlock, 0, c.stream>>>(c.partial.p, c.all_log_all, c.scal.p);
@@ -221,9 +243,10 @@ void module_totals(Ctx &c, double *S, double *L) {
 
 void rebuild_exit_flows(Ctx &c) {
   Level &L = c.cur();
-  fill_i64(c.mX.p, 0, c.N, c.stream);
+  k_zero_exit<<<grid_for(c.N, 256), 256, 0, c.stream>>>(c.N, c.ms.p);
+  RMX_LAUNCH_CHECK();
   k_exit_flows<16><<<grid_for((int64_t)L.n * 16, 256), 256, 0, c.stream>>>(L.n, L.out_ptr.p, L.out_idx.p,
-                                                                            arcw_out(c, L), L.mod.p, c.mX.p);
+                                                                            arcw_out(c, L), L.mod.p, c.ms.p);
   RMX_LAUNCH_CHECK();
 }
 
@@ -234,10 +257,9 @@ void set_partition(Ctx &c, const uint32_t *d_part) {
   const int32_t n = L.n;
   RMX_HIP(hipMemcpyAsync(L.mod.p, d_part, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
   unsigned g = grid_for(n, 256);
-  k_zero_modules<<<g, 256, 0, s>>>(n, c.mP.p, c.mT.p, c.mD.p, c.mX.p, c.mNodes.p, c.mUnits.p);
+  k_zero_modules<<<g, 256, 0, s>>>(n, c.ms.p, c.mNodes.p, c.mUnits.p);
   RMX_LAUNCH_CHECK();
-  k_accumulate_units<<<g, 256, 0, s>>>(n, L.mod.p, L.up.p, L.ut.p, L.ud.p, L.nmem.p, c.mP.p, c.mT.p, c.mD.p,
-                                       c.mNodes.p, c.mUnits.p);
+  k_accumulate_units<<<g, 256, 0, s>>>(n, L.mod.p, L.us.p, L.nmem.p, c.ms.p, c.mNodes.p, c.mUnits.p);
   RMX_LAUNCH_CHECK();
   rebuild_exit_flows(c);
   k_copy<int32_t><<<g, 256, 0, s>>>(c.mNodes.p, c.mStale.p, n);
@@ -249,9 +271,10 @@ double calculate_codelength(Ctx &c) {
   Level &L = c.L0;
   hipStream_t s = c.stream;
   const int32_t n = L.n;
-  DevBuf<uint32_t> key, node, flag, pos, start, total;
-  DevBuf<double> segq, segqp;
+  DevBuf<uint32_t> key(c.cache), node(c.cache), flag(c.cache), pos(c.cache), start(c.cache), total(c.cache);
+  DevBuf<double> segq(c.cache), segqp(c.cache), ext(c.cache);
   key.alloc(n);
+  ext.alloc(n);
   node.alloc(n);
   flag.alloc(n);
   pos.alloc(n);
@@ -270,8 +293,11 @@ double calculate_codelength(Ctx &c) {
   exclusive_scan_u32(flag.p, pos.p, n, total.p, c.scratch, s);
   k_seg_starts<<<g, 256, 0, s>>>(flag.p, pos.p, n, start.p);
   RMX_LAUNCH_CHECK();
-  k_p5_modules<<<g, 256, 0, s>>>(start.p, total.p, n, key.p, node.p, L.p.p, L.tau.p, c.dangling.p, L.out_ptr.p,
-                                 L.out_idx.p, L.out_w.p, L.mod.p, segq.p, segqp.p);
+  k_p5_ext<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(n, L.out_ptr.p, L.out_idx.p, L.out_w.p, L.mod.p,
+                                                               ext.p);
+  RMX_LAUNCH_CHECK();
+  k_p5_modules<<<grid_for((int64_t)n * kWave, 256), 256, 0, s>>>(start.p, total.p, n, node.p, L.p.p, L.tau.p,
+                                                                 c.dangling.p, ext.p, segq.p, segqp.p);
   RMX_LAUNCH_CHECK();
   k_p5_partials<<<kRedGrid, kRedBlock, 0, s>>>(segq.p, segqp.p, total.p, c.partial.p);
   RMX_LAUNCH_CHECK();

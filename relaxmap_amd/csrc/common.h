@@ -25,6 +25,13 @@ constexpr double kFxInv = 1.0 / 2305843009213693952.0;
 __host__ __device__ inline long long fx_of(double x) { return (long long)__builtin_rint(x * kFxScale); }
 __host__ __device__ inline double fx_to(long long v) { return (double)v * kFxInv; }
 
+// Fixed-point statistics of a module (P flow, T teleport weight, D dangling
+// flow, X flow on arcs leaving the module) or of a unit (p, tau, d, w out-flow):
+// one 32-byte record, so a gather touches a single cache line.
+struct alignas(32) Stat4 {
+  long long p, t, d, x;
+};
+
 constexpr int kWave = 64;
 constexpr uint32_t kEmptyKey = 0xffffffffu;
 

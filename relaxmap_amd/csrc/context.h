@@ -16,19 +16,83 @@ constexpr int kNumBuckets = 6;
 // upper bound of (out + in) degree per bucket
 constexpr int kBucketMaxDeg[kNumBuckets] = {16, 32, 128, 512, 2048, 0x7fffffff};
 
+// Device blocks kept for reuse within one context.  All work of a context
+// runs on one stream, so a block handed back while kernels that use it are
+// still queued can be handed out again at once: later launches are ordered
+// after them.  Blocks are released when the context is destroyed.
+struct BlockCache {
+  std::vector<std::pair<size_t, void *>> free_blocks;
+  std::vector<void *> all;
+  void *get(size_t bytes, size_t *got) {
+    size_t best = (size_t)-1;
+    int bi = -1;
+    for (size_t i = 0; i < free_blocks.size(); ++i) {
+      size_t b = free_blocks[i].first;
+      if (b >= bytes && b <= 4 * bytes + (1u << 20) && b < best) {
+        best = b;
+        bi = (int)i;
+      }
+    }
+    if (bi >= 0) {
+      void *p = free_blocks[bi].second;
+      *got = free_blocks[bi].first;
+      free_blocks.erase(free_blocks.begin() + bi);
+      return p;
+    }
+    size_t b = (bytes + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
+    void *p = nullptr;
+    RMX_HIP(hipMalloc(&p, b));
+    all.push_back(p);
+    *got = b;
+    return p;
+  }
+  void put(void *p, size_t bytes) { free_blocks.push_back({bytes, p}); }
+  void release() {
+    for (void *p : all) (void)hipFree(p);
+    all.clear();
+    free_blocks.clear();
+  }
+};
+
+// Owning device allocation.  alloc() keeps the block when it is big enough.
+// With a BlockCache the block comes from and returns to the cache; without
+// one it is hipMalloc'ed and hipFree'd (hipFree waits for the device).
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
   int64_t n = 0;
+  size_t cap_bytes = 0;
+  BlockCache *cache = nullptr;
+  DevBuf() = default;
+  explicit DevBuf(BlockCache &bc) : cache(&bc) {}
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { free(); }
   void alloc(int64_t count) {
+    const size_t need = sizeof(T) * (size_t)(count > 0 ? count : 0);
+    if (p && need <= cap_bytes) {
+      n = count;
+      return;
+    }
     free();
-    if (count > 0) RMX_HIP(hipMalloc(&p, sizeof(T) * count));
+    if (count > 0) {
+      if (cache) {
+        p = (T *)cache->get(need, &cap_bytes);
+      } else {
+        RMX_HIP(hipMalloc(&p, need));
+        cap_bytes = need;
+      }
+    }
     n = count;
   }
   void free() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (cache) cache->put(p, cap_bytes);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    cap_bytes = 0;
   }
   T *get() const { return p; }
 };
@@ -46,11 +110,15 @@ struct Level {
   DevBuf<long long> out_wx, in_wx;  // level 1: exact fixed-point arc flow sums
   DevBuf<double> p, tau;          // level 0: steady-state flow and teleport weight
   // unit statistics in fixed point: flow, teleport, dangling flow, out flow
-  DevBuf<long long> up, ut, ud, uw;
+  DevBuf<Stat4> us;
   DevBuf<int32_t> nmem;           // original nodes inside the unit
   DevBuf<uint32_t> mod;           // module of each unit
   DevBuf<uint32_t> bucket;        // units grouped by degree bucket
   int32_t bucket_off[kNumBuckets + 1] = {0};
+  // hub units (last bucket): each owns a global hash table region of
+  // nextpow2(2 deg + 2) keys; hub_off[i] is the region of the i-th hub
+  DevBuf<long long> hub_off;
+  int64_t hub_pool = 0;
   int32_t max_deg = 0;
   void free_all();
 };
@@ -59,6 +127,33 @@ struct Timing {
   double flow_ms = 0, sweep_ms = 0, contract_ms = 0;
   int64_t edge_visits = 0;
   int sweeps = 0;
+};
+
+// Per-kernel launch statistics of the profiling mode: HIP events on the
+// kernel's own stream around each launch, and the launch's algorithmic HBM
+// bytes (the bytes the algorithm must move, not the cache traffic).
+struct KernelStat {
+  std::string name;
+  int64_t launches = 0;
+  double ms = 0, bytes = 0;
+};
+
+struct Profiler {
+  bool on = false;
+  float last_ms = 0;
+  std::vector<KernelStat> stats;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  KernelStat &get(const std::string &name) {
+    for (auto &k : stats)
+      if (k.name == name) return k;
+    stats.push_back(KernelStat{name});
+    return stats.back();
+  }
+  void begin(hipStream_t s);
+  // records the end event and waits; returns the launch's milliseconds
+  float stop(hipStream_t s);
+  // stop() and account the launch under `name`
+  void end(hipStream_t s, const std::string &name, double bytes);
 };
 
 struct Ctx {
@@ -80,7 +175,7 @@ struct Ctx {
   DevBuf<uint32_t> node_sp;       // node -> super node (level 1)
 
   // module SoA, N slots (module ids are node ids, as in the reference)
-  DevBuf<long long> mP, mT, mD, mX;     // fixed-point sums: flow, teleport, dangling, exit arcs
+  DevBuf<Stat4> ms;                     // fixed-point sums: flow, teleport, dangling, exit arcs
   DevBuf<int32_t> mNodes, mUnits, mStale;  // original nodes, units at the current level, stale count
   DevBuf<uint32_t> mInst;                 // instance (graph) of each module (batched mode)
 
@@ -88,8 +183,11 @@ struct Ctx {
   DevBuf<uint8_t> act, act_next;
   DevBuf<uint32_t> target, newmod, stamp, movers, empty_list, want_empty, rank_tmp;
   DevBuf<long long> agg_buf;         // per mover: flows to/from its old and new module
+  DevBuf<uint32_t> batch_keys, batch_list;   // this sweep's units by (bucket, batch)
   DevBuf<uint32_t> hub_keys;         // global hash tables of hub units
   DevBuf<long long> hub_out, hub_in;
+  DevBuf<uint32_t> hub_plan, hub_bm;  // chunk prefixes of the hubs in a batch; best module per chunk
+  DevBuf<double> hub_bd;              // best dL per chunk
   DevBuf<uint32_t> counters;         // small device counters
   DevBuf<double> scal;               // device scalars: S, L, ...
   DevBuf<double> partial;            // block partials for reductions
@@ -110,7 +208,9 @@ struct Ctx {
   int64_t n_dangling = 0;
 
   Scratch scratch, scratch2;
+  BlockCache cache;   // temporaries of contraction, sorting and codelength
   Timing timing;
+  Profiler prof;
   uint64_t sweep_counter = 0;
 
   // pinned host staging for per-sweep scalars

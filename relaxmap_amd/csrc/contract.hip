@@ -26,11 +26,8 @@ __global__ void k_nonempty(const int32_t *__restrict__ mNodes, int32_t n, uint32
 }
 
 __global__ void k_super_stats(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ mod2sp, int32_t n,
-                              const long long *__restrict__ mP, const long long *__restrict__ mT,
-                              const long long *__restrict__ mD, const long long *__restrict__ mX,
-                              const int32_t *__restrict__ mNodes, long long *__restrict__ up,
-                              long long *__restrict__ ut, long long *__restrict__ ud, long long *__restrict__ uw,
-                              int32_t *__restrict__ nmem, uint32_t *__restrict__ smod, int32_t *__restrict__ mUnits) {
+                              const Stat4 *__restrict__ ms, const int32_t *__restrict__ mNodes,
+                              Stat4 *__restrict__ us, int32_t *__restrict__ nmem, uint32_t *__restrict__ smod, int32_t *__restrict__ mUnits) {
   int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
   if (!flag[m]) {
@@ -38,10 +35,7 @@ __global__ void k_super_stats(const uint32_t *__restrict__ flag, const uint32_t 
     return;
   }
   const uint32_t s = mod2sp[m];
-  up[s] = mP[m];
-  ut[s] = mT[m];
-  ud[s] = mD[m];
-  uw[s] = mX[m];
+  us[s] = ms[m];
   nmem[s] = mNodes[m];
   smod[s] = (uint32_t)m;
   mUnits[m] = 1;
@@ -175,7 +169,7 @@ void contract(Ctx &c) {
   const int64_t m = L0.m;
 
   // relabel non-empty modules
-  DevBuf<uint32_t> flag, mod2sp, total;
+  DevBuf<uint32_t> flag(c.cache), mod2sp(c.cache), total(c.cache);
   flag.alloc(n);
   mod2sp.alloc(n);
   total.alloc(1);
@@ -185,24 +179,21 @@ void contract(Ctx &c) {
   uint32_t ns = 0;
   RMX_HIP(hipMemcpyAsync(&ns, total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   c.sync();
-  L1.free_all();
-  L1.n = (int32_t)ns;
-  L1.up.alloc(ns);
-  L1.ut.alloc(ns);
-  L1.ud.alloc(ns);
-  L1.uw.alloc(ns);
+  L1.n = (int32_t)ns;   // level arrays keep their blocks across contractions
+  L1.us.alloc(ns);
   L1.nmem.alloc(ns);
   L1.mod.alloc(ns);
-  k_super_stats<<<grid_for(n, 256), 256, 0, s>>>(flag.p, mod2sp.p, n, c.mP.p, c.mT.p, c.mD.p, c.mX.p, c.mNodes.p,
-                                                 L1.up.p, L1.ut.p, L1.ud.p, L1.uw.p, L1.nmem.p, L1.mod.p, c.mUnits.p);
+  k_super_stats<<<grid_for(n, 256), 256, 0, s>>>(flag.p, mod2sp.p, n, c.ms.p, c.mNodes.p, L1.us.p, L1.nmem.p,
+                                                 L1.mod.p, c.mUnits.p);
   RMX_LAUNCH_CHECK();
   c.node_sp.alloc(n);
   k_node_sp<<<grid_for(n, 256), 256, 0, s>>>(L0.mod.p, mod2sp.p, n, c.node_sp.p);
   RMX_LAUNCH_CHECK();
+  c.sync();
   mod2sp.free();
 
   // inter-super-node arcs in original arc order
-  DevBuf<uint32_t> aflag, apos, atot;
+  DevBuf<uint32_t> aflag(c.cache), apos(c.cache), atot(c.cache);
   aflag.alloc(m);
   apos.alloc(m);
   atot.alloc(1);
@@ -216,8 +207,8 @@ void contract(Ctx &c) {
     c.sync();
   }
   const int bits = bits_for(ns);
-  DevBuf<uint64_t> keys;
-  DevBuf<uint32_t> arcs;
+  DevBuf<uint64_t> keys(c.cache);
+  DevBuf<uint32_t> arcs(c.cache);
   keys.alloc(m_inter);
   arcs.alloc(m_inter);
   if (m_inter > 0) {
@@ -226,12 +217,13 @@ void contract(Ctx &c) {
     RMX_LAUNCH_CHECK();
     radix_sort_pairs_u64(keys.p, arcs.p, m_inter, 0, 2 * bits, c.scratch, s);
   }
+  c.sync();
   aflag.free();
   apos.free();
 
   // one super arc per distinct key
   uint32_t m_super = 0;
-  DevBuf<uint32_t> kflag, kpos, kstart, src_cnt;
+  DevBuf<uint32_t> kflag(c.cache), kpos(c.cache), kstart(c.cache), src_cnt(c.cache);
   src_cnt.alloc((int64_t)ns + 1);
   fill_u32(src_cnt.p, 0u, (int64_t)ns + 1, s);
   if (m_inter > 0) {
@@ -245,6 +237,7 @@ void contract(Ctx &c) {
     kstart.alloc(m_super);
     k_key_starts<<<grid_for(m_inter, 256), 256, 0, s>>>(kflag.p, kpos.p, m_inter, kstart.p);
     RMX_LAUNCH_CHECK();
+    c.sync();
     kflag.free();
     kpos.free();
   }
@@ -261,6 +254,7 @@ void contract(Ctx &c) {
     k_to_float<<<grid_for(m_super, 256), 256, 0, s>>>(L1.out_w.p, L1.out_wf.p, m_super);
     RMX_LAUNCH_CHECK();
   }
+  c.sync();
   keys.free();
   arcs.free();
   kstart.free();
@@ -269,7 +263,7 @@ void contract(Ctx &c) {
   c.sync();
 
   // transpose
-  DevBuf<uint32_t> in_of_out;
+  DevBuf<uint32_t> in_of_out(c.cache);
   build_in_csr(c, L1, &in_of_out);
   L1.in_w.alloc(m_super);
   L1.in_wx.alloc(m_super);

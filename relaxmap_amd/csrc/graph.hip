@@ -14,8 +14,9 @@ namespace rmx {
 void Level::free_all() {
   out_ptr.free(); out_idx.free(); in_ptr.free(); in_idx.free();
   out_w.free(); in_w.free(); out_wf.free(); in_wf.free(); out_wx.free(); in_wx.free();
-  p.free(); tau.free(); up.free(); ut.free(); ud.free(); uw.free();
-  nmem.free(); mod.free(); bucket.free();
+  p.free(); tau.free(); us.free();
+  nmem.free(); mod.free(); bucket.free(); hub_off.free();
+  hub_pool = 0;
   n = 0; m = 0;
 }
 
@@ -69,6 +70,14 @@ __global__ void k_small_hist(const uint32_t *__restrict__ key, int64_t n, uint32
   if (threadIdx.x < 16 && h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
 }
 
+__global__ void k_list_degree(const uint32_t *__restrict__ list, int32_t k, const uint32_t *__restrict__ optr,
+                              const uint32_t *__restrict__ iptr, uint32_t *__restrict__ deg) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const uint32_t u = list[i];
+  deg[i] = (optr[u + 1] - optr[u]) + (iptr[u + 1] - iptr[u]);
+}
+
 unsigned gs_grid(int64_t n) {
   int64_t g = (n + 255) / 256;
   if (g > 16384) g = 16384;
@@ -116,11 +125,11 @@ void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) {
   L.in_ptr.alloc(n + 1);
   L.in_idx.alloc(m);
   if (in_of_out) in_of_out->alloc(m);
-  DevBuf<uint32_t> cnt;
+  DevBuf<uint32_t> cnt(c.cache);
   cnt.alloc(n + 1);
   fill_u32(cnt.p, 0u, n + 1, s);
   if (m > 0) {
-    DevBuf<uint32_t> src, keys, vals;
+    DevBuf<uint32_t> src(c.cache), keys(c.cache), vals(c.cache);
     src.alloc(m);
     keys.alloc(m);
     vals.alloc(m);
@@ -144,7 +153,7 @@ void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) {
 void build_buckets(Ctx &c, Level &L) {
   hipStream_t s = c.stream;
   const int32_t n = L.n;
-  DevBuf<uint32_t> keys, cnt;
+  DevBuf<uint32_t> keys(c.cache), cnt(c.cache);
   keys.alloc(n);
   cnt.alloc(16 + 1);
   fill_u32(cnt.p, 0u, 17, s);
@@ -161,6 +170,29 @@ void build_buckets(Ctx &c, Level &L) {
   L.bucket_off[0] = 0;
   for (int b = 0; b < kNumBuckets; ++b) L.bucket_off[b + 1] = L.bucket_off[b] + (int32_t)h[b];
   L.max_deg = (int32_t)h[16];
+  // hash-table regions of the hubs
+  const int32_t nh = L.bucket_off[kNumBuckets] - L.bucket_off[kNumBuckets - 1];
+  L.hub_off.alloc(nh + 1);
+  L.hub_pool = 0;
+  if (nh > 0) {
+    DevBuf<uint32_t> deg(c.cache);
+    deg.alloc(nh);
+    k_list_degree<<<grid_for(nh, 256), 256, 0, s>>>(L.bucket.p + L.bucket_off[kNumBuckets - 1], nh, L.out_ptr.p,
+                                                    L.in_ptr.p, deg.p);
+    RMX_LAUNCH_CHECK();
+    std::vector<uint32_t> hd(nh);
+    RMX_HIP(hipMemcpyAsync(hd.data(), deg.p, sizeof(uint32_t) * nh, hipMemcpyDeviceToHost, s));
+    c.sync();
+    std::vector<long long> off(nh + 1, 0);
+    for (int32_t i = 0; i < nh; ++i) {
+      long long cap = 1;
+      while (cap < 2LL * hd[i] + 2) cap <<= 1;
+      off[i + 1] = off[i] + cap;
+    }
+    L.hub_pool = off[nh];
+    RMX_HIP(hipMemcpyAsync(L.hub_off.p, off.data(), sizeof(long long) * (nh + 1), hipMemcpyHostToDevice, s));
+    c.sync();
+  }
 }
 
 }  // namespace rmx

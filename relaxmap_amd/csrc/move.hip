@@ -30,6 +30,8 @@
 // with its table in global memory.
 #include "devutil.h"
 
+#include <algorithm>
+#include <cstring>
 #include <vector>
 
 namespace rmx {
@@ -37,18 +39,20 @@ namespace rmx {
 namespace {
 
 enum { kS = 0, kL = 3, kNM = 7 };
-enum { cMovers = 0, cWant = 1, cMoved = 2, cVisitsLo = 4 };
+enum { cMovers = 0, cWant = 1, cMoved = 2, cVisitsLo = 4, cCandsLo = 6, cUnits = 10 };
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kInvalid = 0xfffffffeu;
 
 struct EvalArgs {
-  const uint32_t *list;
+  const uint32_t *list;     // positions into `bucket` of this launch's units
   int32_t list_n;
+  const uint32_t *bucket;   // units in degree-bucket order
+  int32_t hub_base;         // position of the first hub in `bucket`
   const uint32_t *optr, *oidx, *iptr, *iidx;
   ArcW aout, ain;
   const uint32_t *mod;
-  const long long *up, *ut, *ud, *uw;
-  const long long *mP, *mT, *mD, *mX;
+  const Stat4 *us;
+  const Stat4 *ms;
   const int32_t *mUnits, *mStale;
   const uint8_t *act;
   int all_active;
@@ -65,16 +69,17 @@ struct EvalArgs {
   uint32_t *counters;
   uint32_t *want;
   unsigned long long *visits;
-  // hub tables
+  unsigned long long *cands;
+  uint32_t *units;
+  // hub tables: region [hub_off[g], hub_off[g+1]) of the pool for hub g
   uint32_t *hkeys;
   long long *hout, *hin;
-  int hcap;
+  const long long *hub_off;
 };
 
-__device__ inline bool in_batch(const EvalArgs &a, uint32_t u) {
-  if (!a.all_active && !a.act[u]) return false;
-  if (a.batches <= 1) return true;
-  return (uint32_t)(mix64(a.seed ^ ((uint64_t)u * 0x9e3779b97f4a7c15ull)) % (uint64_t)a.batches) == (uint32_t)a.batch;
+__device__ inline uint32_t batch_of(uint64_t seed, int batches, uint32_t u) {
+  if (batches <= 1) return 0;
+  return (uint32_t)(mix64(seed ^ ((uint64_t)u * 0x9e3779b97f4a7c15ull)) % (uint64_t)batches);
 }
 
 struct Best {
@@ -84,68 +89,57 @@ struct Best {
 
 __device__ inline bool better(double d, uint32_t m, const Best &b) { return d < b.d || (d == b.d && m < b.m); }
 
-// Per-unit constants of the evaluation.
+// Per-unit constants of the evaluation.  With C = S + q1 - q_o and
+// K = -phi(S) - 2 phi(q1) + 2 phi(q_o) + phi(q1 + P_o - p) - phi(q_o + P_o),
+// a candidate m costs
+//   dL = phi(C + q2 - q_m) - 2 phi(q2) + 2 phi(q_m) + phi(q2 + P_m + p) - phi(q_m + P_m) + K.
 struct UnitCtx {
   uint32_t o;
-  long long p, t, d;
-  double pd, td, dd, qu, S;
-  long long Po, To, Do, Xo;
-  double qo, q1, phi_qo, phi_qoP, phi_q1, phi_q1P, phi_S;
-  bool offer_empty;
+  long long p;
+  double tele_out;    // alpha p + beta d
+  double td, qu, C, K;
+  bool single, offer_empty;
 };
 
-__device__ inline double delta_for(const UnitCtx &U, long long Pm, long long Tm, long long Dm, long long Xm,
-                                   long long fo, long long fi) {
-  const double qm = module_exit(Pm, Tm, Dm, Xm);
-  const double outn = kBeta * fx_to(fo) + (kAlpha * U.pd + kBeta * U.dd) * fx_to(Tm);
-  const double inn = kBeta * fx_to(fi) + (kAlpha * fx_to(Pm) + kBeta * fx_to(Dm)) * U.td;
-  const double q2 = qm + U.qu - outn - inn;
-  const double Sn = U.S + U.q1 + q2 - U.qo - qm;
-  const double d_all = plogp(Sn) - U.phi_S;
-  const double d_exit = U.phi_q1 + plogp(q2) - U.phi_qo - plogp(qm);
-  const double d_stay = U.phi_q1P + plogp(q2 + fx_to(Pm + U.p)) - U.phi_qoP - plogp(qm + fx_to(Pm));
-  return d_all - 2.0 * d_exit + d_stay;
+__device__ inline double delta_for(const UnitCtx &U, const Stat4 &M, long long fo, long long fi) {
+  const double qm = module_exit(M.p, M.t, M.d, M.x);
+  const double flows = kBeta * (fx_to(fo) + fx_to(fi));
+  const double tele = U.tele_out * fx_to(M.t) + (kAlpha * fx_to(M.p) + kBeta * fx_to(M.d)) * U.td;
+  const double q2 = qm + U.qu - flows - tele;
+  return plogp(U.C + q2 - qm) - 2.0 * plogp(q2) + 2.0 * plogp(qm) + plogp(q2 + fx_to(M.p + U.p)) -
+         plogp(qm + fx_to(M.p)) + U.K;
 }
 
 __device__ inline double delta_empty(const UnitCtx &U) {
   // zero statistics and no flows to the new module (Module.cpp:1220-1223)
   const double q2 = U.qu;
-  const double Sn = U.S + U.q1 + q2 - U.qo;
-  const double d_all = plogp(Sn) - U.phi_S;
-  const double d_exit = U.phi_q1 + plogp(q2) - U.phi_qo;
-  const double d_stay = U.phi_q1P + plogp(q2 + U.pd) - U.phi_qoP;
-  return d_all - 2.0 * d_exit + d_stay;
+  return plogp(U.C + q2) - 2.0 * plogp(q2) + plogp(q2 + fx_to(U.p)) + U.K;
 }
 
 __device__ inline void unit_ctx(const EvalArgs &a, uint32_t u, long long fo_o, long long fi_o, UnitCtx &U) {
   U.o = a.mod[u];
-  U.p = a.up[u];
-  U.t = a.ut[u];
-  U.d = a.ud[u];
-  U.pd = fx_to(U.p);
-  U.td = fx_to(U.t);
-  U.dd = fx_to(U.d);
-  U.qu = module_exit(U.p, U.t, U.d, a.uw[u]);
-  U.Po = a.mP[U.o];
-  U.To = a.mT[U.o];
-  U.Do = a.mD[U.o];
-  U.Xo = a.mX[U.o];
-  U.qo = module_exit(U.Po, U.To, U.Do, U.Xo);
-  const double outo = kBeta * fx_to(fo_o) + (kAlpha * U.pd + kBeta * U.dd) * fx_to(U.To - U.t);
-  const double ino = kBeta * fx_to(fi_o) + (kAlpha * fx_to(U.Po - U.p) + kBeta * fx_to(U.Do - U.d)) * U.td;
-  U.q1 = U.qo - U.qu + outo + ino;
-  U.S = a.scal[kS];
-  U.phi_S = plogp(U.S);
-  U.phi_qo = plogp(U.qo);
-  U.phi_qoP = plogp(U.qo + fx_to(U.Po));
-  U.phi_q1 = plogp(U.q1);
-  U.phi_q1P = plogp(U.q1 + fx_to(U.Po - U.p));
-  U.offer_empty = a.empty_avail && (a.node_level ? a.mStale[U.o] > 1 : a.mUnits[U.o] > 1);
+  const Stat4 su = a.us[u];
+  U.p = su.p;
+  const double pd = fx_to(su.p), dd = fx_to(su.d);
+  U.td = fx_to(su.t);
+  U.tele_out = kAlpha * pd + kBeta * dd;
+  U.qu = module_exit(su.p, su.t, su.d, su.x);
+  const Stat4 so = a.ms[U.o];
+  const double qo = module_exit(so.p, so.t, so.d, so.x);
+  const double outo = kBeta * fx_to(fo_o) + U.tele_out * fx_to(so.t - su.t);
+  const double ino = kBeta * fx_to(fi_o) + (kAlpha * fx_to(so.p - su.p) + kBeta * fx_to(so.d - su.d)) * U.td;
+  const double q1 = qo - U.qu + outo + ino;
+  const double S = a.scal[kS];
+  U.C = S + q1 - qo;
+  U.K = -plogp(S) - 2.0 * plogp(q1) + 2.0 * plogp(qo) + plogp(q1 + fx_to(so.p - su.p)) - plogp(qo + fx_to(so.p));
+  const int32_t units_o = a.mUnits[U.o];
+  U.single = units_o == 1;
+  U.offer_empty = a.empty_avail && (a.node_level ? a.mStale[U.o] > 1 : units_o > 1);
 }
 
 __device__ inline bool allowed(const EvalArgs &a, const UnitCtx &U, uint32_t m) {
   // a singleton may join another singleton only if that module's id is smaller
-  return !(m > U.o && a.mUnits[U.o] == 1 && a.mUnits[m] == 1);
+  return !(m > U.o && U.single && a.mUnits[m] == 1);
 }
 
 __device__ inline void emit(const EvalArgs &a, uint32_t u, const Best &b, long long fo_o, long long fi_o,
@@ -225,18 +219,19 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
   __shared__ long long s_out[GPB][H];
   __shared__ long long s_in[GPB][H];
   __shared__ unsigned long long s_vis[256 / kWave];
+  __shared__ unsigned long long s_cand[256 / kWave];
   const int gl = threadIdx.x / G;
   const int lane = threadIdx.x % G;
   const int64_t g = (int64_t)blockIdx.x * GPB + gl;
   uint32_t *keys = s_keys[gl];
   long long *fo = s_out[gl];
   long long *fi = s_in[gl];
-  unsigned long long visits = 0;
+  unsigned long long visits = 0, ncand = 0;
   bool work = false;
   uint32_t u = 0;
   if (g < a.list_n) {
-    u = a.list[g];
-    work = in_batch(a, u);
+    u = a.bucket[a.list[g]];
+    work = true;
   }
   if (work) {
     for (int s = lane; s < H; s += G) {
@@ -269,9 +264,11 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
     Best b{0.0, kNone};
     for (int s = lane; s < H; s += G) {
       const uint32_t m = keys[s];
-      if (m == kEmptyKey || m == o) continue;
+      if (m == kEmptyKey) continue;
+      ++ncand;
+      if (m == o) continue;
       if (!allowed(a, U, m)) continue;
-      const double dl = delta_for(U, a.mP[m], a.mT[m], a.mD[m], a.mX[m], fo[s], fi[s]);
+      const double dl = delta_for(U, a.ms[m], fo[s], fi[s]);
       if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; }
     }
     if (lane == 0 && U.offer_empty) {
@@ -289,19 +286,30 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
       emit(a, u, b, fo_o, fi_o, fo_n, fi_n);
     }
   }
-  // edge-visit count, one atomic per block
+  // edge visits, candidates and units: one atomic each per block
   visits = wave_sum(visits);
-  if ((threadIdx.x & (kWave - 1)) == 0) s_vis[threadIdx.x / kWave] = visits;
+  ncand = wave_sum(ncand);
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s_vis[threadIdx.x / kWave] = visits;
+    s_cand[threadIdx.x / kWave] = ncand;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int i = 0; i < 256 / kWave; ++i) t += s_vis[i];
+    unsigned long long t = 0, c = 0;
+    for (int i = 0; i < 256 / kWave; ++i) {
+      t += s_vis[i];
+      c += s_cand[i];
+    }
     if (t) atomicAdd(a.visits, t);
+    if (c) atomicAdd(a.cands, c);
+    // a unit with no arcs is still one evaluation
   }
+  if (work && lane == 0) atomicAdd(a.units, 1u);
 }
 
-// One 256-lane workgroup per unit; the hash table lives in LDS (H keys) or,
-// for hubs (GLOBAL), in this block's slice of a global scratch table.
+// One 256-lane workgroup per unit; the hash table lives in LDS (up to H
+// keys, sized to the unit's degree) or, for hubs (GLOBAL), in the unit's own
+// region of a global pool.
 template <int H, bool GLOBAL>
 __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
   constexpr int LH = GLOBAL ? 1 : H;
@@ -310,15 +318,36 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
   __shared__ long long s_in[LH];
   __shared__ double s_bd[256 / kWave];
   __shared__ uint32_t s_bm[256 / kWave];
-  __shared__ unsigned long long s_vis;
-  const int cap = GLOBAL ? a.hcap : H;
-  uint32_t *keys = GLOBAL ? a.hkeys + (int64_t)blockIdx.x * cap : s_keys;
-  long long *fo = GLOBAL ? a.hout + (int64_t)blockIdx.x * cap : s_out;
-  long long *fi = GLOBAL ? a.hin + (int64_t)blockIdx.x * cap : s_in;
-  if (threadIdx.x == 0) s_vis = 0;
+  __shared__ unsigned long long s_vis, s_cand;
+  if (threadIdx.x == 0) {
+    s_vis = 0;
+    s_cand = 0;
+  }
+  __syncthreads();
+  unsigned long long ncand = 0;
   for (int64_t g = blockIdx.x; g < a.list_n; g += gridDim.x) {
-    const uint32_t u = a.list[g];
-    if (!in_batch(a, u)) continue;   // uniform across the block
+    const uint32_t pos = a.list[g];
+    const uint32_t u = a.bucket[pos];
+    const uint32_t ob = a.optr[u], oe = a.optr[u + 1];
+    const uint32_t ib = a.iptr[u], ie = a.iptr[u + 1];
+    int cap;
+    uint32_t *keys;
+    long long *fo, *fi;
+    if (GLOBAL) {
+      const uint32_t h = pos - (uint32_t)a.hub_base;
+      const long long base = a.hub_off[h];
+      cap = (int)(a.hub_off[h + 1] - base);
+      keys = a.hkeys + base;
+      fo = a.hout + base;
+      fi = a.hin + base;
+    } else {
+      cap = 16;
+      while (cap < 2 * (int)((oe - ob) + (ie - ib)) + 2) cap <<= 1;
+      if (cap > H) cap = H;
+      keys = s_keys;
+      fo = s_out;
+      fi = s_in;
+    }
     for (int s = threadIdx.x; s < cap; s += 256) {
       keys[s] = kEmptyKey;
       fo[s] = 0;
@@ -326,8 +355,6 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
     }
     if (GLOBAL) __threadfence();
     __syncthreads();
-    const uint32_t ob = a.optr[u], oe = a.optr[u + 1];
-    const uint32_t ib = a.iptr[u], ie = a.iptr[u + 1];
     for (uint32_t k = ob + threadIdx.x; k < oe; k += 256) {
       uint32_t m = a.mod[a.oidx[k]];
       uint32_t s = lds_insert<GLOBAL>(keys, cap, m);
@@ -340,7 +367,10 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
     }
     if (GLOBAL) __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) s_vis += (oe - ob) + (ie - ib);
+    if (threadIdx.x == 0) {
+      s_vis += (oe - ob) + (ie - ib);
+      atomicAdd(a.units, 1u);
+    }
     UnitCtx U;
     const uint32_t o = a.mod[u];
     long long fo_o = 0, fi_o = 0;
@@ -355,11 +385,13 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
     Best b{0.0, kNone};
     for (int s = threadIdx.x; s < cap; s += 256) {
       const uint32_t m = GLOBAL ? __hip_atomic_load(&keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : keys[s];
-      if (m == kEmptyKey || m == o) continue;
+      if (m == kEmptyKey) continue;
+      ++ncand;
+      if (m == o) continue;
       if (!allowed(a, U, m)) continue;
       const long long vo = GLOBAL ? __hip_atomic_load(&fo[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : fo[s];
       const long long vi = GLOBAL ? __hip_atomic_load(&fi[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : fi[s];
-      const double dl = delta_for(U, a.mP[m], a.mT[m], a.mD[m], a.mX[m], vo, vi);
+      const double dl = delta_for(U, a.ms[m], vo, vi);
       if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; }
     }
     if (threadIdx.x == 0 && U.offer_empty) {
@@ -386,7 +418,243 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0 && s_vis) atomicAdd(a.visits, s_vis);
+  ncand = wave_sum(ncand);
+  if ((threadIdx.x & (kWave - 1)) == 0 && ncand) atomicAdd(&s_cand, ncand);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_vis) atomicAdd(a.visits, s_vis);
+    if (s_cand) atomicAdd(a.cands, s_cand);
+  }
+}
+
+// ---------------------------------------------------------------- hubs
+// Hub units (degree > 2048) are split over many workgroups: their arcs and
+// their hash-table slots are cut into chunks of kHubChunk, and every phase
+// (clear, insert, evaluate) runs one workgroup per chunk, so a single huge
+// super node no longer serialises a launch; a final pass reduces each hub's
+// per-chunk best candidates in a fixed order.
+constexpr int kHubChunk = 4096;
+
+__device__ inline int chunk_owner(const uint32_t *__restrict__ pre, int cnt, uint32_t c) {
+  int lo = 0, hi = cnt;   // pre[lo] <= c < pre[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= c) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline uint32_t block_incl_scan(uint32_t v, uint32_t *lds, uint32_t *total) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint32_t y = __shfl_up(x, off, kWave);
+    if (lane >= off) x += y;
+  }
+  if (lane == kWave - 1) lds[w] = x;
+  __syncthreads();
+  uint32_t pre = 0, all = 0;
+  for (int i = 0; i < 256 / kWave; ++i) {
+    if (i < w) pre += lds[i];
+    all += lds[i];
+  }
+  __syncthreads();
+  *total = all;
+  return pre + x;
+}
+
+struct HubPlan {
+  uint32_t *A;      // prefix of arc chunks per hub (cnt + 1)
+  uint32_t *S;      // prefix of slot chunks per hub (cnt + 1)
+  uint32_t *tot;    // [0] arc chunks, [1] slot chunks
+  double *bd;       // best dL per slot chunk
+  uint32_t *bm;     // best module per slot chunk
+};
+
+__device__ inline void hub_of(const EvalArgs &a, int i, uint32_t &u, long long &base, int &cap) {
+  const uint32_t pos = a.list[i];
+  u = a.bucket[pos];
+  const uint32_t h = pos - (uint32_t)a.hub_base;
+  base = a.hub_off[h];
+  cap = (int)(a.hub_off[h + 1] - base);
+}
+
+__global__ __launch_bounds__(256) void k_hub_plan(EvalArgs a, HubPlan P) {
+  __shared__ uint32_t lds[256 / kWave];
+  uint32_t carryA = 0, carryS = 0;
+  if (threadIdx.x == 0) {
+    P.A[0] = 0;
+    P.S[0] = 0;
+  }
+  for (int base = 0; base < a.list_n; base += 256) {
+    const int i = base + threadIdx.x;
+    uint32_t ac = 0, sc = 0;
+    if (i < a.list_n) {
+      uint32_t u;
+      long long off;
+      int cap;
+      hub_of(a, i, u, off, cap);
+      const uint32_t deg = (a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u]);
+      ac = deg == 0 ? 1u : (deg + kHubChunk - 1) / kHubChunk;
+      sc = ((uint32_t)cap + kHubChunk - 1) / kHubChunk;
+    }
+    uint32_t ta, ts;
+    const uint32_t ia = block_incl_scan(ac, lds, &ta);
+    const uint32_t is = block_incl_scan(sc, lds, &ts);
+    if (i < a.list_n) {
+      P.A[i + 1] = carryA + ia;
+      P.S[i + 1] = carryS + is;
+    }
+    carryA += ta;
+    carryS += ts;
+  }
+  if (threadIdx.x == 0) {
+    P.tot[0] = carryA;
+    P.tot[1] = carryS;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_hub_clear(EvalArgs a, HubPlan P) {
+  const uint32_t total = P.tot[1];
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const int i = chunk_owner(P.S, a.list_n, c);
+    uint32_t u;
+    long long base;
+    int cap;
+    hub_of(a, i, u, base, cap);
+    const int s0 = (int)(c - P.S[i]) * kHubChunk;
+    const int s1 = min(cap, s0 + kHubChunk);
+    for (int s = s0 + threadIdx.x; s < s1; s += 256) {
+      a.hkeys[base + s] = kEmptyKey;
+      a.hout[base + s] = 0;
+      a.hin[base + s] = 0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_hub_insert(EvalArgs a, HubPlan P) {
+  const uint32_t total = P.tot[0];
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const int i = chunk_owner(P.A, a.list_n, c);
+    uint32_t u;
+    long long base;
+    int cap;
+    hub_of(a, i, u, base, cap);
+    const uint32_t ob = a.optr[u], od = a.optr[u + 1] - ob;
+    const uint32_t ib = a.iptr[u], deg = od + (a.iptr[u + 1] - ib);
+    const uint32_t j0 = (c - P.A[i]) * kHubChunk;
+    const uint32_t j1 = min(deg, j0 + kHubChunk);
+    uint32_t *keys = a.hkeys + base;
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256) {
+      if (j < od) {
+        const uint32_t k = ob + j;
+        const uint32_t sl = lds_insert<true>(keys, cap, a.mod[a.oidx[k]]);
+        atomicAdd((unsigned long long *)&a.hout[base + sl], (unsigned long long)a.aout.fxw(k));
+      } else {
+        const uint32_t k = ib + (j - od);
+        const uint32_t sl = lds_insert<true>(keys, cap, a.mod[a.iidx[k]]);
+        atomicAdd((unsigned long long *)&a.hin[base + sl], (unsigned long long)a.ain.fxw(k));
+      }
+    }
+  }
+}
+
+__device__ inline long long ld_ll(const long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void hub_unit_ctx(const EvalArgs &a, uint32_t u, long long base, int cap, UnitCtx &U,
+                                    long long &fo_o, long long &fi_o) {
+  const uint32_t o = a.mod[u];
+  fo_o = 0;
+  fi_o = 0;
+  const int so = lds_find<true>(a.hkeys + base, cap, o);
+  if (so >= 0) {
+    fo_o = ld_ll(&a.hout[base + so]);
+    fi_o = ld_ll(&a.hin[base + so]);
+  }
+  unit_ctx(a, u, fo_o, fi_o, U);
+}
+
+__global__ __launch_bounds__(256) void k_hub_eval(EvalArgs a, HubPlan P) {
+  __shared__ double s_bd[256 / kWave];
+  __shared__ uint32_t s_bm[256 / kWave];
+  const uint32_t total = P.tot[1];
+  unsigned long long ncand = 0;
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const int i = chunk_owner(P.S, a.list_n, c);
+    uint32_t u;
+    long long base;
+    int cap;
+    hub_of(a, i, u, base, cap);
+    UnitCtx U;
+    long long fo_o, fi_o;
+    hub_unit_ctx(a, u, base, cap, U, fo_o, fi_o);
+    const int s0 = (int)(c - P.S[i]) * kHubChunk;
+    const int s1 = min(cap, s0 + kHubChunk);
+    Best b{0.0, kNone};
+    for (int sl = s0 + threadIdx.x; sl < s1; sl += 256) {
+      const uint32_t m = ld_key<true>(a.hkeys + base, (uint32_t)sl);
+      if (m == kEmptyKey) continue;
+      ++ncand;
+      if (m == U.o || !allowed(a, U, m)) continue;
+      const double dl = delta_for(U, a.ms[m], ld_ll(&a.hout[base + sl]), ld_ll(&a.hin[base + sl]));
+      if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; }
+    }
+    b = group_best<kWave>(b);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      s_bd[threadIdx.x / kWave] = b.d;
+      s_bm[threadIdx.x / kWave] = b.m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      Best bb{s_bd[0], s_bm[0]};
+      for (int q = 1; q < 256 / kWave; ++q)
+        if (better(s_bd[q], s_bm[q], bb)) { bb.d = s_bd[q]; bb.m = s_bm[q]; }
+      P.bd[c] = bb.d;
+      P.bm[c] = bb.m;
+    }
+    __syncthreads();
+  }
+  ncand = wave_sum(ncand);
+  if ((threadIdx.x & (kWave - 1)) == 0 && ncand) atomicAdd(a.cands, ncand);
+}
+
+// one wavefront per hub: best over its chunks (fixed order), empty option, emit
+__global__ __launch_bounds__(256) void k_hub_finish(EvalArgs a, HubPlan P) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t wstride = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t i = w0; i < a.list_n; i += wstride) {
+    uint32_t u;
+    long long base;
+    int cap;
+    hub_of(a, (int)i, u, base, cap);
+    Best b{0.0, kNone};
+    for (uint32_t c = P.S[i] + lane; c < P.S[i + 1]; c += kWave)
+      if (better(P.bd[c], P.bm[c], b)) { b.d = P.bd[c]; b.m = P.bm[c]; }
+    b = group_best<kWave>(b);
+    if (lane == 0) {
+      UnitCtx U;
+      long long fo_o, fi_o;
+      hub_unit_ctx(a, u, base, cap, U, fo_o, fi_o);
+      if (U.offer_empty) {
+        const double dl = delta_empty(U);
+        if (better(dl, a.empty_key, b) && dl < 0.0) { b.d = dl; b.m = a.empty_key; }
+      }
+      long long fo_n = 0, fi_n = 0;
+      if (b.m != kNone && b.m != a.empty_key) {
+        const int sn = lds_find<true>(a.hkeys + base, cap, b.m);
+        fo_n = ld_ll(&a.hout[base + sn]);
+        fi_n = ld_ll(&a.hin[base + sn]);
+      }
+      emit(a, u, b, fo_o, fi_o, fo_n, fi_n);
+      atomicAdd(a.visits, (unsigned long long)((a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u])));
+      atomicAdd(a.units, 1u);
+    }
+  }
 }
 
 // ------------------------------------------------------------------ commit
@@ -424,11 +692,8 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
                                                 const uint32_t *__restrict__ iidx, ArcW aout,
                                                 const uint32_t *__restrict__ mod, const uint32_t *__restrict__ newmod,
                                                 const uint32_t *__restrict__ stamp, uint32_t cur,
-                                                const long long *__restrict__ agg, const long long *__restrict__ up,
-                                                const long long *__restrict__ ut, const long long *__restrict__ ud,
-                                                const long long *__restrict__ uw, const int32_t *__restrict__ nmem,
-                                                long long *mP, long long *mT, long long *mD, long long *mX,
-                                                int32_t *mNodes, int32_t *mUnits, uint8_t *act_next, int mark,
+                                                const long long *__restrict__ agg, const Stat4 *__restrict__ us,
+                                                const int32_t *__restrict__ nmem, Stat4 *ms, int32_t *mNodes, int32_t *mUnits, uint8_t *act_next, int mark,
                                                 uint32_t *moved) {
   const uint32_t nm = counters[cMovers];
   const int lane = threadIdx.x % G;
@@ -456,20 +721,20 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
     dxn = group_sum_ll<G>(dxn);
     if (lane == 0) {
       const long long *g = agg + 4 * (int64_t)u;
-      const long long W = uw[u];
+      const Stat4 su = us[u];
+      const long long W = su.x;
       dxo += -(W - g[0]) + g[1];
       dxn += (W - g[2]) - g[3];
-      const long long p = up[u], t = ut[u], d = ud[u];
-      atomicAdd((unsigned long long *)&mP[o], (unsigned long long)(-p));
-      atomicAdd((unsigned long long *)&mP[n], (unsigned long long)p);
-      atomicAdd((unsigned long long *)&mT[o], (unsigned long long)(-t));
-      atomicAdd((unsigned long long *)&mT[n], (unsigned long long)t);
-      if (d) {
-        atomicAdd((unsigned long long *)&mD[o], (unsigned long long)(-d));
-        atomicAdd((unsigned long long *)&mD[n], (unsigned long long)d);
+      atomicAdd((unsigned long long *)&ms[o].p, (unsigned long long)(-su.p));
+      atomicAdd((unsigned long long *)&ms[n].p, (unsigned long long)su.p);
+      atomicAdd((unsigned long long *)&ms[o].t, (unsigned long long)(-su.t));
+      atomicAdd((unsigned long long *)&ms[n].t, (unsigned long long)su.t);
+      if (su.d) {
+        atomicAdd((unsigned long long *)&ms[o].d, (unsigned long long)(-su.d));
+        atomicAdd((unsigned long long *)&ms[n].d, (unsigned long long)su.d);
       }
-      if (dxo) atomicAdd((unsigned long long *)&mX[o], (unsigned long long)dxo);
-      if (dxn) atomicAdd((unsigned long long *)&mX[n], (unsigned long long)dxn);
+      if (dxo) atomicAdd((unsigned long long *)&ms[o].x, (unsigned long long)dxo);
+      if (dxn) atomicAdd((unsigned long long *)&ms[n].x, (unsigned long long)dxn);
       const int32_t k = nmem[u];
       atomicSub(&mNodes[o], k);
       atomicAdd(&mNodes[n], k);
@@ -488,6 +753,31 @@ __global__ void k_apply(const uint32_t *__restrict__ movers, const uint32_t *__r
     const uint32_t u = movers[i];
     if (stamp[u] == cur) mod[u] = newmod[u];
   }
+}
+
+// sort key of every bucket position: (bucket, batch) or (bucket, inactive)
+__global__ void k_batch_keys(const uint32_t *__restrict__ bucket, int32_t n, int4 off_lo, int4 off_hi,
+                             const uint8_t *__restrict__ act, int all_active, uint64_t seed, int batches,
+                             uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int offs[8] = {off_lo.x, off_lo.y, off_lo.z, off_lo.w, off_hi.x, off_hi.y, off_hi.z, off_hi.w};
+  int b = 0;
+  while (b + 1 < kNumBuckets && i >= offs[b + 1]) ++b;
+  const uint32_t u = bucket[i];
+  const uint32_t k = (all_active || act[u]) ? batch_of(seed, batches, u) : (uint32_t)batches;
+  keys[i] = (uint32_t)b * (uint32_t)(batches + 1) + k;
+  vals[i] = (uint32_t)i;
+}
+
+__global__ void k_hist256(const uint32_t *__restrict__ key, int64_t n, uint32_t *__restrict__ cnt) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&h[key[i] & 255u], 1u);
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ void k_reset_counters(uint32_t *counters) {
@@ -513,6 +803,28 @@ unsigned stride_grid(int64_t items, int per_block) {
 
 }  // namespace
 
+// Algorithmic bytes of one evaluation launch: per arc visit the neighbour id,
+// its weight and its module (4 + w + 4 bytes); per unit its statistics, module
+// and CSR bounds (48 bytes); per candidate module its four statistics (32).
+static void profile_eval(Ctx &c, int bucket, hipStream_t s) {
+  static const char *names[kNumBuckets] = {"eval_g8_h32", "eval_g16_h64", "eval_g32_h256", "eval_g64_h1024",
+                                           "eval_block_h4096", "eval_block_hub"};
+  const float ms = c.prof.stop(s);   // end event right after the launch
+  uint32_t before[16];
+  std::memcpy(before, c.h_cnt, sizeof(before));
+  RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
+  c.sync();
+  auto u64 = [](const uint32_t *h, int i) { return (double)((uint64_t)h[i] | ((uint64_t)h[i + 1] << 32)); };
+  const double visits = u64(c.h_cnt, cVisitsLo) - u64(before, cVisitsLo);
+  const double cands = u64(c.h_cnt, cCandsLo) - u64(before, cCandsLo);
+  const double units = (double)c.h_cnt[cUnits] - (double)before[cUnits];
+  const double wbytes = (c.level == 0 && c.precision == 1) ? 4.0 : 8.0;
+  KernelStat &k = c.prof.get(std::string(names[bucket]) + (c.level == 0 ? "_nodes" : "_super"));
+  k.launches += 1;
+  k.ms += ms;
+  k.bytes += visits * (8.0 + wbytes) + units * 48.0 + cands * 32.0;
+}
+
 void ensure_sweep_buffers(Ctx &c) {
   const int32_t n = c.N;
   if (c.target.n >= n) return;
@@ -525,13 +837,15 @@ void ensure_sweep_buffers(Ctx &c) {
   c.empty_list.alloc(n);
   c.want_empty.alloc(n);
   c.rank_tmp.alloc(n);
-  c.counters.alloc(16);
+  c.counters.alloc(16 + 256);
   c.agg_buf.alloc(4 * (int64_t)n);
+  c.batch_keys.alloc(n);
+  c.batch_list.alloc(n);
   fill_u8(c.act.p, 1, n, c.stream);
   fill_u8(c.act_next.p, 0, n, c.stream);
   fill_u32(c.stamp.p, 0u, n, c.stream);
   fill_u32(c.want_empty.p, 0u, n, c.stream);
-  fill_u32(c.counters.p, 0u, 16, c.stream);
+  fill_u32(c.counters.p, 0u, 16 + 256, c.stream);
 }
 
 void reset_active(Ctx &c) {
@@ -556,28 +870,58 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     k_compact<<<grid_for(c.N, 256), 256, 0, s>>>(c.rank_tmp.p, c.target.p, c.N, c.empty_list.p);
     RMX_LAUNCH_CHECK();
   }
-  fill_u32(c.counters.p + 2, 0u, 6, s);   // moved, visits
+  fill_u32(c.counters.p + 2, 0u, 6, s);   // moved, visits, candidates
+  fill_u32(c.counters.p + cUnits, 0u, 1, s);
   RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
   c.sync();
   n_empty = c.h_cnt[8];
   uint32_t empty_cursor = 0;
+  if (c.prof.on) {   // h_cnt mirrors the counters from here on (profiling deltas)
+    RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
+    c.sync();
+  }
 
-  // hub scratch tables
-  int hcap = 0;
+  // hub hash-table pool
   const int32_t n_hubs = L.bucket_off[kNumBuckets] - L.bucket_off[kNumBuckets - 1];
-  const int hub_blocks = n_hubs > 0 ? (n_hubs < 64 ? n_hubs : 64) : 0;
+  if (n_hubs > 0 && c.hub_keys.n < L.hub_pool) {
+    c.hub_keys.alloc(L.hub_pool);
+    c.hub_out.alloc(L.hub_pool);
+    c.hub_in.alloc(L.hub_pool);
+  }
   if (n_hubs > 0) {
-    hcap = 1;
-    while (hcap < 2 * L.max_deg + 2) hcap <<= 1;
-    const int64_t need = (int64_t)hub_blocks * hcap;
-    if (c.hub_keys.n < need) {
-      c.hub_keys.alloc(need);
-      c.hub_out.alloc(need);
-      c.hub_in.alloc(need);
+    const int64_t chunks = L.hub_pool / kHubChunk + n_hubs + 1;
+    if (c.hub_bd.n < chunks) {
+      c.hub_bd.alloc(chunks);
+      c.hub_bm.alloc(chunks);
     }
+    if (c.hub_plan.n < 3 * ((int64_t)n_hubs + 1)) c.hub_plan.alloc(3 * ((int64_t)n_hubs + 1));
+  }
+
+  // this sweep's units per (bucket, batch), inactive units last: one stable
+  // 8-bit radix pass over the bucket positions
+  const int B = c.batches;
+  const int NK = kNumBuckets * (B + 1);
+  std::vector<uint32_t> seg(NK + 1, 0);
+  {
+    const int *o = L.bucket_off;
+    k_batch_keys<<<grid_for(n, 256), 256, 0, s>>>(L.bucket.p, n, make_int4(o[0], o[1], o[2], o[3]),
+                                                  make_int4(o[4], o[5], o[6], o[6]), c.act.p, prioritized ? 0 : 1,
+                                                  sweep_seed, B, c.batch_keys.p, c.batch_list.p);
+    RMX_LAUNCH_CHECK();
+    radix_sort_pairs_u32(c.batch_keys.p, c.batch_list.p, n, 0, 8, c.scratch2, s);
+    fill_u32(c.counters.p + 16, 0u, 256, s);
+    k_hist256<<<(unsigned)std::min<int64_t>(1024, (n + 255) / 256), 256, 0, s>>>(c.batch_keys.p, n,
+                                                                               c.counters.p + 16);
+    RMX_LAUNCH_CHECK();
+    std::vector<uint32_t> h(256);
+    RMX_HIP(hipMemcpyAsync(h.data(), c.counters.p + 16, sizeof(uint32_t) * 256, hipMemcpyDeviceToHost, s));
+    c.sync();
+    for (int k = 0; k < NK; ++k) seg[k + 1] = seg[k] + h[k];
   }
 
   EvalArgs a;
+  a.bucket = L.bucket.p;
+  a.hub_base = L.bucket_off[kNumBuckets - 1];
   a.optr = L.out_ptr.p;
   a.oidx = L.out_idx.p;
   a.iptr = L.in_ptr.p;
@@ -585,14 +929,8 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   a.aout = arcw_out(c, L);
   a.ain = arcw_in(c, L);
   a.mod = L.mod.p;
-  a.up = L.up.p;
-  a.ut = L.ut.p;
-  a.ud = L.ud.p;
-  a.uw = L.uw.p;
-  a.mP = c.mP.p;
-  a.mT = c.mT.p;
-  a.mD = c.mD.p;
-  a.mX = c.mX.p;
+  a.us = L.us.p;
+  a.ms = c.ms.p;
   a.mUnits = c.mUnits.p;
   a.mStale = c.mStale.p;
   a.act = c.act.p;
@@ -609,10 +947,12 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   a.counters = c.counters.p;
   a.want = c.want_empty.p;
   a.visits = (unsigned long long *)(c.counters.p + cVisitsLo);
+  a.cands = (unsigned long long *)(c.counters.p + cCandsLo);
+  a.units = c.counters.p + cUnits;
   a.hkeys = c.hub_keys.p;
   a.hout = c.hub_out.p;
   a.hin = c.hub_in.p;
-  a.hcap = hcap;
+  a.hub_off = L.hub_off.p;
 
   ArcW aout = arcw_out(c, L);
   for (int b = 0; b < c.batches; ++b) {
@@ -621,19 +961,30 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     k_reset_counters<<<1, 64, 0, s>>>(c.counters.p);
     RMX_LAUNCH_CHECK();
     for (int bk = 0; bk < kNumBuckets; ++bk) {
-      const int32_t cnt = L.bucket_off[bk + 1] - L.bucket_off[bk];
+      const int key = bk * (B + 1) + b;
+      const int32_t cnt = (int32_t)(seg[key + 1] - seg[key]);
       if (cnt <= 0) continue;
-      a.list = L.bucket.p + L.bucket_off[bk];
+      a.list = c.batch_list.p + seg[key];
       a.list_n = cnt;
+      if (c.prof.on) c.prof.begin(s);
       switch (bk) {
         case 0: k_eval_group<8, 32><<<(unsigned)((cnt + 31) / 32), 256, 0, s>>>(a); break;
         case 1: k_eval_group<16, 64><<<(unsigned)((cnt + 15) / 16), 256, 0, s>>>(a); break;
         case 2: k_eval_group<32, 256><<<(unsigned)((cnt + 7) / 8), 256, 0, s>>>(a); break;
         case 3: k_eval_group<64, 1024><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a); break;
         case 4: k_eval_block<4096, false><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a); break;
-        default: k_eval_block<1, true><<<(unsigned)hub_blocks, 256, 0, s>>>(a); break;
+        default: {
+          HubPlan P{c.hub_plan.p, c.hub_plan.p + (cnt + 1), c.hub_plan.p + 2 * (cnt + 1), c.hub_bd.p, c.hub_bm.p};
+          k_hub_plan<<<1, 256, 0, s>>>(a, P);
+          k_hub_clear<<<4096, 256, 0, s>>>(a, P);
+          k_hub_insert<<<4096, 256, 0, s>>>(a, P);
+          k_hub_eval<<<4096, 256, 0, s>>>(a, P);
+          k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, P);
+          break;
+        }
       }
       RMX_LAUNCH_CHECK();
+      if (c.prof.on) profile_eval(c, bk, s);
     }
     RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 2, hipMemcpyDeviceToHost, s));
     c.sync();
@@ -651,8 +1002,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     }
     k_commit<16><<<stride_grid((int64_t)n_movers * 16, 256), 256, 0, s>>>(
         c.movers.p, c.counters.p, L.out_ptr.p, L.out_idx.p, L.in_ptr.p, L.in_idx.p, aout, L.mod.p, c.newmod.p,
-        c.stamp.p, c.stamp_now, c.agg_buf.p, L.up.p, L.ut.p, L.ud.p, L.uw.p, L.nmem.p, c.mP.p, c.mT.p, c.mD.p,
-        c.mX.p, c.mNodes.p, c.mUnits.p, c.act_next.p, prioritized ? 1 : 0, c.counters.p + cMoved);
+        c.stamp.p, c.stamp_now, c.agg_buf.p, L.us.p, L.nmem.p, c.ms.p, c.mNodes.p, c.mUnits.p, c.act_next.p, prioritized ? 1 : 0, c.counters.p + cMoved);
     RMX_LAUNCH_CHECK();
     k_apply<<<stride_grid(n_movers, 256), 256, 0, s>>>(c.movers.p, c.counters.p, c.newmod.p, c.stamp.p,
                                                         c.stamp_now, L.mod.p);

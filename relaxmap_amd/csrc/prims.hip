@@ -8,6 +8,7 @@
 // (equal-digit peers within a wave) and a per-digit running count across the
 // tile's 16 striped steps.
 #include "prims.h"
+#include "context.h"
 
 namespace rmx {
 
@@ -19,6 +20,31 @@ void *Scratch::get(size_t need) {
     bytes = b;
   }
   return ptr;
+}
+
+void Profiler::begin(hipStream_t s) {
+  if (!e0) {
+    RMX_HIP(hipEventCreate(&e0));
+    RMX_HIP(hipEventCreate(&e1));
+  }
+  RMX_HIP(hipEventRecord(e0, s));
+}
+
+float Profiler::stop(hipStream_t s) {
+  RMX_HIP(hipEventRecord(e1, s));
+  RMX_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  RMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  last_ms = ms;
+  return ms;
+}
+
+void Profiler::end(hipStream_t s, const std::string &name, double bytes) {
+  float ms = stop(s);
+  KernelStat &k = get(name);
+  k.launches += 1;
+  k.ms += ms;
+  k.bytes += bytes;
 }
 
 void Scratch::release() {
