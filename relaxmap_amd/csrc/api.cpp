@@ -53,6 +53,7 @@ int fail(rmx_ctx *ctx, int code, const std::string &msg) {
 void free_ctx_device(Ctx &c) {
   c.L0.free_all();
   c.L1.free_all();
+  c.L2.free_all();
   c.node_w.free(); c.raw_w.free(); c.dangling.free(); c.in_of_out.free(); c.node_sp.free();
   c.ms.free(); c.mNodes.free(); c.mUnits.free(); c.mStale.free();
   c.mInst.free();
@@ -183,6 +184,7 @@ int rmx_initiate(rmx_ctx *ctx, int32_t *iterations, double *codelength, int64_t 
     if (c.N <= 0) return fail(ctx, RMX_ESTATE, "no network uploaded");
     RMX_HIP(hipSetDevice(c.device));
     c.level = 0;
+    c.sweep_counter = 0;   // a re-initiated network repeats its sweeps exactly
     steady_state(c, 200, 50, 1.0e-15);
     init_modules_singletons(c);
     module_totals(c, &c.sum_exit, &c.codelength);

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -95,6 +96,12 @@ struct DevBuf {
     cap_bytes = 0;
   }
   T *get() const { return p; }
+  void swap(DevBuf &o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+    std::swap(cap_bytes, o.cap_bytes);
+    std::swap(cache, o.cache);
+  }
 };
 
 // One level of the hierarchy: the original nodes (level 0) or the super
@@ -121,6 +128,7 @@ struct Level {
   int64_t hub_pool = 0;
   int32_t max_deg = 0;
   void free_all();
+  void swap(Level &o);
 };
 
 struct Timing {
@@ -168,6 +176,7 @@ struct Ctx {
   bool initiated = false;
 
   Level L0, L1;
+  Level L2;            // the next contraction is built here, then swapped with L1
   int level = 0;
   DevBuf<double> node_w, raw_w;   // inputs kept for re-initiation
   DevBuf<uint8_t> dangling;       // level-0 dangling flag
@@ -188,6 +197,7 @@ struct Ctx {
   DevBuf<long long> hub_out, hub_in;
   DevBuf<uint32_t> hub_plan, hub_bm;  // chunk prefixes of the hubs in a batch; best module per chunk
   DevBuf<double> hub_bd;              // best dL per chunk
+  DevBuf<uint32_t> big_list, big_plan;  // commit: movers with hub degree and their arc-chunk prefix
   DevBuf<uint32_t> counters;         // small device counters
   DevBuf<double> scal;               // device scalars: S, L, ...
   DevBuf<double> partial;            // block partials for reductions

@@ -2,16 +2,21 @@
 // SuperNode(Module, int), Node.cpp:76-106) and member bookkeeping
 // (Network::updateMembersInModule, Module.cpp:2881-2902).
 //
-// Contraction always works on the original arcs and the nodes' current
-// modules, as the reference does.  Relabel: super node id = rank of the
-// module among non-empty modules (exclusive scan of non-empty flags).  Arcs
-// between different super nodes get the key (src_sp, dst_sp); a stable LSD
-// radix sort by key keeps, inside every key, the original arc order
-// (source node ascending, then stored out-link order), which is the order in
-// which the reference's std::map sums them.  Each key segment is then summed
-// left to right by one thread, so the FP64 super-arc weights are bit-exact;
-// the fixed-point sums the sweeps use are carried beside them.  In-lists are
-// the transpose, again by a stable sort (ascending source).
+// Relabel: super node id = rank of the module among non-empty modules
+// (exclusive scan of non-empty flags).  Arcs between different super nodes
+// get the key (src_sp, dst_sp); a stable LSD radix sort by key keeps, inside
+// every key, the source level's arc order.  Each key segment is then summed
+// left to right by one thread; the fixed-point sums the sweeps use are
+// carried beside the FP64 sums.  In-lists are the transpose, again by a
+// stable sort (ascending source).
+//
+// The reference always contracts the original arcs (source node ascending,
+// then stored out-link order: the order its std::map sums them), so from
+// the node level the FP64 super-arc weights are bit-exact.  When the current
+// level is already a contraction, the new one is built from the current
+// super arcs instead (a fraction of the original arcs): the fixed-point sums
+// are exact either way, and the FP64 weights differ from summing the
+// original arcs by rounding only.
 #include "devutil.h"
 
 #include <vector>
@@ -91,7 +96,8 @@ __global__ void k_key_starts(const uint32_t *__restrict__ flag, const uint32_t *
 // one thread per (src, dst) super arc: sequential FP64 sum in arc order
 __global__ void k_reduce_segments(const uint32_t *__restrict__ start, int64_t n_seg, int64_t m,
                                   const uint64_t *__restrict__ keys, const uint32_t *__restrict__ arc,
-                                  const double *__restrict__ w, const float *__restrict__ wf, int mixed, int bits,
+                                  const double *__restrict__ w, const float *__restrict__ wf,
+                                  const long long *__restrict__ wx, int mixed, int bits,
                                   uint32_t *__restrict__ out_idx, double *__restrict__ out_w,
                                   long long *__restrict__ out_wx, uint32_t *__restrict__ src_cnt) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -104,7 +110,7 @@ __global__ void k_reduce_segments(const uint32_t *__restrict__ start, int64_t n_
     const uint32_t k = arc[i];
     const double f = w[k];
     sum += f;
-    sx += fx_of(mixed ? (double)wf[k] : f);
+    sx += wx ? wx[k] : fx_of(mixed ? (double)wf[k] : f);
   }
   const uint64_t key = keys[b];
   out_idx[s] = (uint32_t)(key & ((1ull << bits) - 1ull));
@@ -135,6 +141,12 @@ __global__ void k_propagate(const uint32_t *__restrict__ node_sp, const uint32_t
   if (i < n) mod[i] = smod[node_sp[i]];
 }
 
+// node -> new super node through the previous super node
+__global__ void k_compose(uint32_t *__restrict__ node_sp, const uint32_t *__restrict__ usp, int32_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) node_sp[i] = usp[node_sp[i]];
+}
+
 __global__ void k_copy_i32(const int32_t *__restrict__ a, int32_t *__restrict__ b, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = a[i];
@@ -162,11 +174,14 @@ void propagate_labels(Ctx &c) {
 void contract(Ctx &c) {
   hipStream_t s = c.stream;
   propagate_labels(c);
+  const bool from_super = c.level == 1;
   c.level = 0;
   Level &L0 = c.L0;
-  Level &L1 = c.L1;
+  Level &S = from_super ? c.L1 : c.L0;   // level whose arcs are contracted
+  Level &D = c.L2;                       // the new super level, swapped into L1 at the end
   const int32_t n = c.N;
-  const int64_t m = L0.m;
+  const int32_t ns_src = S.n;
+  const int64_t m = S.m;
 
   // relabel non-empty modules
   DevBuf<uint32_t> flag(c.cache), mod2sp(c.cache), total(c.cache);
@@ -179,28 +194,41 @@ void contract(Ctx &c) {
   uint32_t ns = 0;
   RMX_HIP(hipMemcpyAsync(&ns, total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   c.sync();
-  L1.n = (int32_t)ns;   // level arrays keep their blocks across contractions
-  L1.us.alloc(ns);
-  L1.nmem.alloc(ns);
-  L1.mod.alloc(ns);
-  k_super_stats<<<grid_for(n, 256), 256, 0, s>>>(flag.p, mod2sp.p, n, c.ms.p, c.mNodes.p, L1.us.p, L1.nmem.p,
-                                                 L1.mod.p, c.mUnits.p);
+  D.n = (int32_t)ns;   // level arrays keep their blocks across contractions
+  D.us.alloc(ns);
+  D.nmem.alloc(ns);
+  D.mod.alloc(ns);
+  k_super_stats<<<grid_for(n, 256), 256, 0, s>>>(flag.p, mod2sp.p, n, c.ms.p, c.mNodes.p, D.us.p, D.nmem.p,
+                                                 D.mod.p, c.mUnits.p);
   RMX_LAUNCH_CHECK();
-  c.node_sp.alloc(n);
-  k_node_sp<<<grid_for(n, 256), 256, 0, s>>>(L0.mod.p, mod2sp.p, n, c.node_sp.p);
-  RMX_LAUNCH_CHECK();
+  // unit of the source level -> new super node
+  DevBuf<uint32_t> usp_buf(c.cache);
+  const uint32_t *usp;
+  if (from_super) {
+    usp_buf.alloc(ns_src);
+    k_node_sp<<<grid_for(ns_src, 256), 256, 0, s>>>(S.mod.p, mod2sp.p, ns_src, usp_buf.p);
+    RMX_LAUNCH_CHECK();
+    k_compose<<<grid_for(n, 256), 256, 0, s>>>(c.node_sp.p, usp_buf.p, n);
+    RMX_LAUNCH_CHECK();
+    usp = usp_buf.p;
+  } else {
+    c.node_sp.alloc(n);
+    k_node_sp<<<grid_for(n, 256), 256, 0, s>>>(L0.mod.p, mod2sp.p, n, c.node_sp.p);
+    RMX_LAUNCH_CHECK();
+    usp = c.node_sp.p;
+  }
   c.sync();
   mod2sp.free();
 
-  // inter-super-node arcs in original arc order
+  // inter-super-node arcs in source arc order
   DevBuf<uint32_t> aflag(c.cache), apos(c.cache), atot(c.cache);
   aflag.alloc(m);
   apos.alloc(m);
   atot.alloc(1);
   uint32_t m_inter = 0;
   if (m > 0) {
-    k_inter_flags<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L0.out_ptr.p, L0.out_idx.p, n, c.node_sp.p,
-                                                                      aflag.p);
+    k_inter_flags<16><<<grid_for((int64_t)ns_src * 16, 256), 256, 0, s>>>(S.out_ptr.p, S.out_idx.p, ns_src, usp,
+                                                                           aflag.p);
     RMX_LAUNCH_CHECK();
     exclusive_scan_u32(aflag.p, apos.p, m, atot.p, c.scratch, s);
     RMX_HIP(hipMemcpyAsync(&m_inter, atot.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -212,8 +240,8 @@ void contract(Ctx &c) {
   keys.alloc(m_inter);
   arcs.alloc(m_inter);
   if (m_inter > 0) {
-    k_inter_keys<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L0.out_ptr.p, L0.out_idx.p, n, c.node_sp.p,
-                                                                     aflag.p, apos.p, bits, keys.p, arcs.p);
+    k_inter_keys<16><<<grid_for((int64_t)ns_src * 16, 256), 256, 0, s>>>(S.out_ptr.p, S.out_idx.p, ns_src, usp,
+                                                                          aflag.p, apos.p, bits, keys.p, arcs.p);
     RMX_LAUNCH_CHECK();
     radix_sort_pairs_u64(keys.p, arcs.p, m_inter, 0, 2 * bits, c.scratch, s);
   }
@@ -241,41 +269,43 @@ void contract(Ctx &c) {
     kflag.free();
     kpos.free();
   }
-  L1.m = m_super;
-  L1.out_idx.alloc(m_super);
-  L1.out_w.alloc(m_super);
-  L1.out_wx.alloc(m_super);
-  L1.out_wf.alloc(m_super);
+  D.m = m_super;
+  D.out_idx.alloc(m_super);
+  D.out_w.alloc(m_super);
+  D.out_wx.alloc(m_super);
+  D.out_wf.alloc(m_super);
   if (m_super > 0) {
-    k_reduce_segments<<<grid_for(m_super, 256), 256, 0, s>>>(kstart.p, m_super, m_inter, keys.p, arcs.p, L0.out_w.p,
-                                                              L0.out_wf.p, c.precision == 1 ? 1 : 0, bits,
-                                                              L1.out_idx.p, L1.out_w.p, L1.out_wx.p, src_cnt.p);
+    k_reduce_segments<<<grid_for(m_super, 256), 256, 0, s>>>(
+        kstart.p, m_super, m_inter, keys.p, arcs.p, S.out_w.p, S.out_wf.p, from_super ? S.out_wx.p : nullptr,
+        c.precision == 1 ? 1 : 0, bits, D.out_idx.p, D.out_w.p, D.out_wx.p, src_cnt.p);
     RMX_LAUNCH_CHECK();
-    k_to_float<<<grid_for(m_super, 256), 256, 0, s>>>(L1.out_w.p, L1.out_wf.p, m_super);
+    k_to_float<<<grid_for(m_super, 256), 256, 0, s>>>(D.out_w.p, D.out_wf.p, m_super);
     RMX_LAUNCH_CHECK();
   }
   c.sync();
   keys.free();
   arcs.free();
   kstart.free();
-  L1.out_ptr.alloc((int64_t)ns + 1);
-  exclusive_scan_u32(src_cnt.p, L1.out_ptr.p, (int64_t)ns + 1, nullptr, c.scratch, s);
+  usp_buf.free();
+  D.out_ptr.alloc((int64_t)ns + 1);
+  exclusive_scan_u32(src_cnt.p, D.out_ptr.p, (int64_t)ns + 1, nullptr, c.scratch, s);
   c.sync();
 
   // transpose
   DevBuf<uint32_t> in_of_out(c.cache);
-  build_in_csr(c, L1, &in_of_out);
-  L1.in_w.alloc(m_super);
-  L1.in_wx.alloc(m_super);
-  L1.in_wf.alloc(m_super);
+  build_in_csr(c, D, &in_of_out);
+  D.in_w.alloc(m_super);
+  D.in_wx.alloc(m_super);
+  D.in_wf.alloc(m_super);
   if (m_super > 0) {
-    k_scatter_in<<<grid_for(m_super, 256), 256, 0, s>>>(in_of_out.p, m_super, L1.out_w.p, L1.out_wx.p, L1.out_wf.p,
-                                                         L1.in_w.p, L1.in_wx.p, L1.in_wf.p);
+    k_scatter_in<<<grid_for(m_super, 256), 256, 0, s>>>(in_of_out.p, m_super, D.out_w.p, D.out_wx.p, D.out_wf.p,
+                                                         D.in_w.p, D.in_wx.p, D.in_wf.p);
     RMX_LAUNCH_CHECK();
   }
-  build_buckets(c, L1);
-  c.level = 1;
+  build_buckets(c, D);
   c.sync();
+  c.L1.swap(c.L2);
+  c.level = 1;
 }
 
 }  // namespace rmx

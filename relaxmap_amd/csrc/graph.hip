@@ -20,6 +20,19 @@ void Level::free_all() {
   n = 0; m = 0;
 }
 
+void Level::swap(Level &o) {
+  std::swap(n, o.n);
+  std::swap(m, o.m);
+  out_ptr.swap(o.out_ptr); out_idx.swap(o.out_idx); in_ptr.swap(o.in_ptr); in_idx.swap(o.in_idx);
+  out_w.swap(o.out_w); in_w.swap(o.in_w); out_wf.swap(o.out_wf); in_wf.swap(o.in_wf);
+  out_wx.swap(o.out_wx); in_wx.swap(o.in_wx);
+  p.swap(o.p); tau.swap(o.tau); us.swap(o.us);
+  nmem.swap(o.nmem); mod.swap(o.mod); bucket.swap(o.bucket); hub_off.swap(o.hub_off);
+  for (int b = 0; b <= kNumBuckets; ++b) std::swap(bucket_off[b], o.bucket_off[b]);
+  std::swap(hub_pool, o.hub_pool);
+  std::swap(max_deg, o.max_deg);
+}
+
 namespace {
 
 // src of every arc: one wave per row strides over the row's arcs
@@ -30,10 +43,18 @@ __global__ void k_arc_src(const uint32_t *__restrict__ ptr, int32_t n, uint32_t 
   for (uint32_t e = ptr[wave] + lane; e < ptr[wave + 1]; e += kWave) src[e] = (uint32_t)wave;
 }
 
-__global__ void k_degree_hist(const uint32_t *__restrict__ dst, int64_t m, uint32_t *__restrict__ cnt) {
+// in_ptr from the destinations sorted ascending: in_ptr[d] is the first
+// position whose destination is >= d, so position i fills every d in
+// (key[i-1], key[i]] (and position m fills the tail up to n)
+__global__ void k_ptr_from_sorted(const uint32_t *__restrict__ key, int64_t m, int32_t n,
+                                  uint32_t *__restrict__ ptr) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < m; i += stride) atomicAdd(&cnt[dst[i]], 1u);
+  for (; i <= m; i += stride) {
+    const int64_t lo = i == 0 ? 0 : (int64_t)key[i - 1] + 1;
+    const int64_t hi = i == m ? (int64_t)n : (int64_t)key[i];
+    for (int64_t d = lo; d <= hi; ++d) ptr[d] = (uint32_t)i;
+  }
 }
 
 // after the stable sort: in_idx[k] = src[e_k], in_of_out[e_k] = k
@@ -125,17 +146,17 @@ void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) {
   L.in_ptr.alloc(n + 1);
   L.in_idx.alloc(m);
   if (in_of_out) in_of_out->alloc(m);
-  DevBuf<uint32_t> cnt(c.cache);
-  cnt.alloc(n + 1);
-  fill_u32(cnt.p, 0u, n + 1, s);
-  if (m > 0) {
+  if (m == 0) {
+    fill_u32(L.in_ptr.p, 0u, n + 1, s);
+    c.sync();
+    return;
+  }
+  {
     DevBuf<uint32_t> src(c.cache), keys(c.cache), vals(c.cache);
     src.alloc(m);
     keys.alloc(m);
     vals.alloc(m);
     k_arc_src<<<grid_for((int64_t)n * kWave, 256), 256, 0, s>>>(L.out_ptr.p, n, src.p);
-    RMX_LAUNCH_CHECK();
-    k_degree_hist<<<gs_grid(m), 256, 0, s>>>(L.out_idx.p, m, cnt.p);
     RMX_LAUNCH_CHECK();
     RMX_HIP(hipMemcpyAsync(keys.p, L.out_idx.p, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, s));
     iota_u32(vals.p, m, s);
@@ -144,10 +165,10 @@ void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) {
     radix_sort_pairs_u32(keys.p, vals.p, m, 0, bits, c.scratch, s);
     k_in_from_sorted<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, m, L.in_idx.p, in_of_out ? in_of_out->p : nullptr);
     RMX_LAUNCH_CHECK();
+    k_ptr_from_sorted<<<gs_grid(m + 1), 256, 0, s>>>(keys.p, m, n, L.in_ptr.p);
+    RMX_LAUNCH_CHECK();
     c.sync();
   }
-  exclusive_scan_u32(cnt.p, L.in_ptr.p, n + 1, nullptr, c.scratch, s);
-  c.sync();
 }
 
 void build_buckets(Ctx &c, Level &L) {

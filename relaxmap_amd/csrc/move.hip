@@ -39,7 +39,7 @@ namespace rmx {
 namespace {
 
 enum { kS = 0, kL = 3, kNM = 7 };
-enum { cMovers = 0, cWant = 1, cMoved = 2, cVisitsLo = 4, cCandsLo = 6, cUnits = 10 };
+enum { cMovers = 0, cWant = 1, cMoved = 2, cVisitsLo = 4, cCandsLo = 6, cUnits = 10, cBig = 11 };
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kInvalid = 0xfffffffeu;
 
@@ -85,6 +85,7 @@ __device__ inline uint32_t batch_of(uint64_t seed, int batches, uint32_t u) {
 struct Best {
   double d;
   uint32_t m;
+  int i;   // slot of m in the (compacted) table, -1 for none or the empty module
 };
 
 __device__ inline bool better(double d, uint32_t m, const Best &b) { return d < b.d || (d == b.d && m < b.m); }
@@ -166,7 +167,8 @@ __device__ inline Best group_best(Best b) {
   for (int off = G / 2; off > 0; off >>= 1) {
     double d = __shfl_xor(b.d, off, G);
     uint32_t m = __shfl_xor(b.m, off, G);
-    if (better(d, m, b)) { b.d = d; b.m = m; }
+    int i = __shfl_xor(b.i, off, G);
+    if (better(d, m, b)) { b.d = d; b.m = m; b.i = i; }
   }
   return b;
 }
@@ -211,15 +213,23 @@ __device__ inline int lds_find(const uint32_t *keys, int H, uint32_t m) {
   }
 }
 
+// Lane mask of the lanes below this one in the wavefront.
+__device__ inline uint64_t lanes_below() {
+  const int wl = threadIdx.x & (kWave - 1);
+  return wl == 0 ? 0ull : (~0ull >> (kWave - wl));
+}
+
 // G lanes per unit (G <= 64: the group lives inside one wavefront), H keys.
+// After the flows are aggregated, the non-empty slots are compacted in place
+// to the front of the table (a slot only ever moves down, and every lane
+// reads its slot before any lane writes), so the FP64 evaluation runs on
+// full lanes instead of once per table stride with most lanes masked.
 template <int G, int H>
 __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
   constexpr int GPB = 256 / G;
   __shared__ uint32_t s_keys[GPB][H];
   __shared__ long long s_out[GPB][H];
   __shared__ long long s_in[GPB][H];
-  __shared__ unsigned long long s_vis[256 / kWave];
-  __shared__ unsigned long long s_cand[256 / kWave];
   const int gl = threadIdx.x / G;
   const int lane = threadIdx.x % G;
   const int64_t g = (int64_t)blockIdx.x * GPB + gl;
@@ -256,49 +266,71 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) visits = (oe - ob) + (ie - ib);
-    UnitCtx U;
     const uint32_t o = a.mod[u];
     const int so = lds_find(keys, H, o);
     const long long fo_o = so >= 0 ? fo[so] : 0, fi_o = so >= 0 ? fi[so] : 0;
+    // compact the non-empty slots to [0, nc) (the group's lanes of the wavefront)
+    const int wl = threadIdx.x & (kWave - 1);
+    const uint64_t gmask = G == kWave ? ~0ull : (((1ull << G) - 1ull) << (wl - lane));
+    int nc = 0;
+    for (int s0 = 0; s0 < H; s0 += G) {
+      const int s = s0 + lane;
+      const uint32_t key = keys[s];
+      const long long vo = fo[s], vi = fi[s];
+      const bool ne = key != kEmptyKey;
+      const uint64_t bal = __ballot(ne) & gmask;
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (ne) {
+        const int d = nc + (int)__popcll(bal & lanes_below());
+        keys[d] = key;
+        fo[d] = vo;
+        fi[d] = vi;
+      }
+      nc += (int)__popcll(bal);
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0) ncand = (unsigned long long)nc;
+    UnitCtx U;
     unit_ctx(a, u, fo_o, fi_o, U);
-    Best b{0.0, kNone};
-    for (int s = lane; s < H; s += G) {
-      const uint32_t m = keys[s];
-      if (m == kEmptyKey) continue;
-      ++ncand;
+    Best b{0.0, kNone, -1};
+    for (int i = lane; i < nc; i += G) {
+      const uint32_t m = keys[i];
       if (m == o) continue;
       if (!allowed(a, U, m)) continue;
-      const double dl = delta_for(U, a.ms[m], fo[s], fi[s]);
-      if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; }
+      const double dl = delta_for(U, a.ms[m], fo[i], fi[i]);
+      if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; b.i = i; }
     }
     if (lane == 0 && U.offer_empty) {
       const double dl = delta_empty(U);
-      if (better(dl, a.empty_key, b) && dl < 0.0) { b.d = dl; b.m = a.empty_key; }
+      if (better(dl, a.empty_key, b) && dl < 0.0) { b.d = dl; b.m = a.empty_key; b.i = -1; }
     }
     b = group_best<G>(b);
     if (lane == 0) {
-      long long fo_n = 0, fi_n = 0;
-      if (b.m != kNone && b.m != a.empty_key) {
-        const int sn = lds_find(keys, H, b.m);
-        fo_n = fo[sn];
-        fi_n = fi[sn];
-      }
+      const long long fo_n = b.i >= 0 ? fo[b.i] : 0, fi_n = b.i >= 0 ? fi[b.i] : 0;
       emit(a, u, b, fo_o, fi_o, fo_n, fi_n);
     }
   }
-  // edge visits, candidates and units: one atomic each per block
+  // edge visits, candidates and units: one atomic each per block (the
+  // partial sums reuse the first table once every group is done with it, so
+  // the kernel's LDS is exactly its tables)
+  constexpr int NW = 256 / kWave;
+  static_assert(H >= 2 * NW, "table too small for the block partials");
   visits = wave_sum(visits);
   ncand = wave_sum(ncand);
+  __syncthreads();
+  unsigned long long *red = (unsigned long long *)&s_out[0][0];
   if ((threadIdx.x & (kWave - 1)) == 0) {
-    s_vis[threadIdx.x / kWave] = visits;
-    s_cand[threadIdx.x / kWave] = ncand;
+    red[threadIdx.x / kWave] = visits;
+    red[NW + threadIdx.x / kWave] = ncand;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long t = 0, c = 0;
-    for (int i = 0; i < 256 / kWave; ++i) {
-      t += s_vis[i];
-      c += s_cand[i];
+    for (int i = 0; i < NW; ++i) {
+      t += red[i];
+      c += red[NW + i];
     }
     if (t) atomicAdd(a.visits, t);
     if (c) atomicAdd(a.cands, c);
@@ -307,123 +339,128 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
   if (work && lane == 0) atomicAdd(a.units, 1u);
 }
 
-// One 256-lane workgroup per unit; the hash table lives in LDS (up to H
-// keys, sized to the unit's degree) or, for hubs (GLOBAL), in the unit's own
-// region of a global pool.
-template <int H, bool GLOBAL>
+// One 256-lane workgroup per unit; the LDS hash table holds up to H keys,
+// sized to the unit's degree.  Candidates are compacted in place per
+// wavefront: wave w owns the slots s with (s mod 256) / 64 == w and moves
+// its j-th non-empty slot to its own j-th slot (a slot only moves down within
+// the wave's sequence, and the wave reads before it writes), then evaluates
+// its own list.  The kernel's LDS is exactly the table.
+__device__ inline int wave_slot(int j, int w) { return (j >> 6) * 256 + w * kWave + (j & (kWave - 1)); }
+
+template <int H>
 __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
-  constexpr int LH = GLOBAL ? 1 : H;
-  __shared__ uint32_t s_keys[LH];
-  __shared__ long long s_out[LH];
-  __shared__ long long s_in[LH];
-  __shared__ double s_bd[256 / kWave];
-  __shared__ uint32_t s_bm[256 / kWave];
-  __shared__ unsigned long long s_vis, s_cand;
-  if (threadIdx.x == 0) {
-    s_vis = 0;
-    s_cand = 0;
-  }
-  __syncthreads();
-  unsigned long long ncand = 0;
+  constexpr int NW = 256 / kWave;
+  static_assert(H >= 6 * NW, "table too small for the per-wave bests");
+  __shared__ __align__(16) uint32_t s_keys[H];
+  __shared__ long long s_out[H];
+  __shared__ long long s_in[H];
+  const int w = threadIdx.x / kWave;
+  const int wl = threadIdx.x & (kWave - 1);
+  unsigned long long vis_acc = 0, cand_acc = 0;   // thread 0
   for (int64_t g = blockIdx.x; g < a.list_n; g += gridDim.x) {
     const uint32_t pos = a.list[g];
     const uint32_t u = a.bucket[pos];
     const uint32_t ob = a.optr[u], oe = a.optr[u + 1];
     const uint32_t ib = a.iptr[u], ie = a.iptr[u + 1];
-    int cap;
-    uint32_t *keys;
-    long long *fo, *fi;
-    if (GLOBAL) {
-      const uint32_t h = pos - (uint32_t)a.hub_base;
-      const long long base = a.hub_off[h];
-      cap = (int)(a.hub_off[h + 1] - base);
-      keys = a.hkeys + base;
-      fo = a.hout + base;
-      fi = a.hin + base;
-    } else {
-      cap = 16;
-      while (cap < 2 * (int)((oe - ob) + (ie - ib)) + 2) cap <<= 1;
-      if (cap > H) cap = H;
-      keys = s_keys;
-      fo = s_out;
-      fi = s_in;
-    }
+    int cap = 256;
+    while (cap < 2 * (int)((oe - ob) + (ie - ib)) + 2) cap <<= 1;
+    if (cap > H) cap = H;
+    uint32_t *keys = s_keys;
+    long long *fo = s_out, *fi = s_in;
     for (int s = threadIdx.x; s < cap; s += 256) {
       keys[s] = kEmptyKey;
       fo[s] = 0;
       fi[s] = 0;
     }
-    if (GLOBAL) __threadfence();
     __syncthreads();
     for (uint32_t k = ob + threadIdx.x; k < oe; k += 256) {
       uint32_t m = a.mod[a.oidx[k]];
-      uint32_t s = lds_insert<GLOBAL>(keys, cap, m);
+      uint32_t s = lds_insert(keys, cap, m);
       atomicAdd((unsigned long long *)&fo[s], (unsigned long long)a.aout.fxw(k));
     }
     for (uint32_t k = ib + threadIdx.x; k < ie; k += 256) {
       uint32_t m = a.mod[a.iidx[k]];
-      uint32_t s = lds_insert<GLOBAL>(keys, cap, m);
+      uint32_t s = lds_insert(keys, cap, m);
       atomicAdd((unsigned long long *)&fi[s], (unsigned long long)a.ain.fxw(k));
     }
-    if (GLOBAL) __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
-      s_vis += (oe - ob) + (ie - ib);
+      vis_acc += (oe - ob) + (ie - ib);
       atomicAdd(a.units, 1u);
     }
-    UnitCtx U;
     const uint32_t o = a.mod[u];
     long long fo_o = 0, fi_o = 0;
     {
-      int so = lds_find<GLOBAL>(keys, cap, o);
+      int so = lds_find(keys, cap, o);
       if (so >= 0) {
-        fo_o = GLOBAL ? __hip_atomic_load(&fo[so], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : fo[so];
-        fi_o = GLOBAL ? __hip_atomic_load(&fi[so], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : fi[so];
+        fo_o = fo[so];
+        fi_o = fi[so];
       }
     }
+    __syncthreads();   // every lookup of o is done before the table is compacted
+    int nw = 0;        // this wave's candidates
+    for (int s0 = 0; s0 < cap; s0 += 256) {   // cap is a multiple of 256
+      const int s = s0 + w * kWave + wl;
+      const uint32_t key = keys[s];
+      const long long vo = fo[s], vi = fi[s];
+      const bool ne = key != kEmptyKey;
+      const uint64_t bal = __ballot(ne);
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (ne) {
+        const int d = wave_slot(nw + (int)__popcll(bal & lanes_below()), w);
+        keys[d] = key;
+        fo[d] = vo;
+        fi[d] = vi;
+      }
+      nw += (int)__popcll(bal);
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    UnitCtx U;
     unit_ctx(a, u, fo_o, fi_o, U);
-    Best b{0.0, kNone};
-    for (int s = threadIdx.x; s < cap; s += 256) {
-      const uint32_t m = GLOBAL ? __hip_atomic_load(&keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : keys[s];
-      if (m == kEmptyKey) continue;
-      ++ncand;
+    Best b{0.0, kNone, -1};
+    for (int j = wl; j < nw; j += kWave) {
+      const int i = wave_slot(j, w);
+      const uint32_t m = keys[i];
       if (m == o) continue;
       if (!allowed(a, U, m)) continue;
-      const long long vo = GLOBAL ? __hip_atomic_load(&fo[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : fo[s];
-      const long long vi = GLOBAL ? __hip_atomic_load(&fi[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : fi[s];
-      const double dl = delta_for(U, a.ms[m], vo, vi);
-      if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; }
+      const double dl = delta_for(U, a.ms[m], fo[i], fi[i]);
+      if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; b.i = i; }
     }
     if (threadIdx.x == 0 && U.offer_empty) {
       const double dl = delta_empty(U);
-      if (better(dl, a.empty_key, b) && dl < 0.0) { b.d = dl; b.m = a.empty_key; }
+      if (better(dl, a.empty_key, b) && dl < 0.0) { b.d = dl; b.m = a.empty_key; b.i = -1; }
     }
     b = group_best<kWave>(b);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-      s_bd[threadIdx.x / kWave] = b.d;
-      s_bm[threadIdx.x / kWave] = b.m;
+    const unsigned long long nc_w = (unsigned long long)nw;
+    __syncthreads();   // keys are not read again: the per-wave bests reuse them
+    double *r_bd = (double *)s_keys;                      // NW doubles
+    uint32_t *r_bm = s_keys + 2 * NW;                     // NW ids
+    int *r_bi = (int *)(s_keys + 3 * NW);                 // NW slots
+    unsigned long long *r_nc = (unsigned long long *)(s_keys + 4 * NW);   // NW counts
+    if (wl == 0) {
+      r_bd[w] = b.d;
+      r_bm[w] = b.m;
+      r_bi[w] = b.i;
+      r_nc[w] = nc_w;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      Best bb{s_bd[0], s_bm[0]};
-      for (int i = 1; i < 256 / kWave; ++i)
-        if (better(s_bd[i], s_bm[i], bb)) { bb.d = s_bd[i]; bb.m = s_bm[i]; }
-      long long fo_n = 0, fi_n = 0;
-      if (bb.m != kNone && bb.m != a.empty_key) {
-        const int sn = lds_find<GLOBAL>(keys, cap, bb.m);
-        fo_n = GLOBAL ? __hip_atomic_load(&fo[sn], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : fo[sn];
-        fi_n = GLOBAL ? __hip_atomic_load(&fi[sn], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : fi[sn];
+      Best bb{r_bd[0], r_bm[0], r_bi[0]};
+      cand_acc += r_nc[0];
+      for (int i = 1; i < NW; ++i) {
+        cand_acc += r_nc[i];
+        if (better(r_bd[i], r_bm[i], bb)) { bb.d = r_bd[i]; bb.m = r_bm[i]; bb.i = r_bi[i]; }
       }
+      const long long fo_n = bb.i >= 0 ? fo[bb.i] : 0, fi_n = bb.i >= 0 ? fi[bb.i] : 0;
       emit(a, u, bb, fo_o, fi_o, fo_n, fi_n);
     }
     __syncthreads();
   }
-  ncand = wave_sum(ncand);
-  if ((threadIdx.x & (kWave - 1)) == 0 && ncand) atomicAdd(&s_cand, ncand);
-  __syncthreads();
   if (threadIdx.x == 0) {
-    if (s_vis) atomicAdd(a.visits, s_vis);
-    if (s_cand) atomicAdd(a.cands, s_cand);
+    if (vis_acc) atomicAdd(a.visits, vis_acc);
+    if (cand_acc) atomicAdd(a.cands, cand_acc);
   }
 }
 
@@ -578,9 +615,15 @@ __device__ inline void hub_unit_ctx(const EvalArgs &a, uint32_t u, long long bas
   unit_ctx(a, u, fo_o, fi_o, U);
 }
 
+// one workgroup per slot chunk: the chunk's non-empty slots are listed in
+// LDS first, then evaluated densely
 __global__ __launch_bounds__(256) void k_hub_eval(EvalArgs a, HubPlan P) {
-  __shared__ double s_bd[256 / kWave];
-  __shared__ uint32_t s_bm[256 / kWave];
+  constexpr int NW = 256 / kWave;
+  __shared__ uint16_t s_list[kHubChunk];
+  __shared__ uint32_t s_wcnt[NW];
+  __shared__ double s_bd[NW];
+  __shared__ uint32_t s_bm[NW];
+  const int w = threadIdx.x / kWave;
   const uint32_t total = P.tot[1];
   unsigned long long ncand = 0;
   for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
@@ -594,32 +637,47 @@ __global__ __launch_bounds__(256) void k_hub_eval(EvalArgs a, HubPlan P) {
     hub_unit_ctx(a, u, base, cap, U, fo_o, fi_o);
     const int s0 = (int)(c - P.S[i]) * kHubChunk;
     const int s1 = min(cap, s0 + kHubChunk);
-    Best b{0.0, kNone};
-    for (int sl = s0 + threadIdx.x; sl < s1; sl += 256) {
+    int nc = 0;
+    for (int t0 = s0; t0 < s1; t0 += 256) {
+      const int sl = t0 + threadIdx.x;
+      const bool ne = sl < s1 && ld_key<true>(a.hkeys + base, (uint32_t)sl) != kEmptyKey;
+      const uint64_t bal = __ballot(ne);
+      if ((threadIdx.x & (kWave - 1)) == 0) s_wcnt[w] = (uint32_t)__popcll(bal);
+      __syncthreads();
+      int off = nc, tot = 0;
+      for (int q = 0; q < NW; ++q) {
+        if (q < w) off += (int)s_wcnt[q];
+        tot += (int)s_wcnt[q];
+      }
+      if (ne) s_list[off + (int)__popcll(bal & lanes_below())] = (uint16_t)(sl - s0);
+      nc += tot;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) ncand += (unsigned long long)nc;
+    Best b{0.0, kNone, -1};
+    for (int q = threadIdx.x; q < nc; q += 256) {
+      const int sl = s0 + (int)s_list[q];
       const uint32_t m = ld_key<true>(a.hkeys + base, (uint32_t)sl);
-      if (m == kEmptyKey) continue;
-      ++ncand;
       if (m == U.o || !allowed(a, U, m)) continue;
       const double dl = delta_for(U, a.ms[m], ld_ll(&a.hout[base + sl]), ld_ll(&a.hin[base + sl]));
       if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; }
     }
     b = group_best<kWave>(b);
     if ((threadIdx.x & (kWave - 1)) == 0) {
-      s_bd[threadIdx.x / kWave] = b.d;
-      s_bm[threadIdx.x / kWave] = b.m;
+      s_bd[w] = b.d;
+      s_bm[w] = b.m;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      Best bb{s_bd[0], s_bm[0]};
-      for (int q = 1; q < 256 / kWave; ++q)
+      Best bb{s_bd[0], s_bm[0], -1};
+      for (int q = 1; q < NW; ++q)
         if (better(s_bd[q], s_bm[q], bb)) { bb.d = s_bd[q]; bb.m = s_bm[q]; }
       P.bd[c] = bb.d;
       P.bm[c] = bb.m;
     }
     __syncthreads();
   }
-  ncand = wave_sum(ncand);
-  if ((threadIdx.x & (kWave - 1)) == 0 && ncand) atomicAdd(a.cands, ncand);
+  if (threadIdx.x == 0 && ncand) atomicAdd(a.cands, ncand);
 }
 
 // one wavefront per hub: best over its chunks (fixed order), empty option, emit
@@ -632,7 +690,7 @@ __global__ __launch_bounds__(256) void k_hub_finish(EvalArgs a, HubPlan P) {
     long long base;
     int cap;
     hub_of(a, (int)i, u, base, cap);
-    Best b{0.0, kNone};
+    Best b{0.0, kNone, -1};
     for (uint32_t c = P.S[i] + lane; c < P.S[i + 1]; c += kWave)
       if (better(P.bd[c], P.bm[c], b)) { b.d = P.bd[c]; b.m = P.bm[c]; }
     b = group_best<kWave>(b);
@@ -685,6 +743,9 @@ __global__ void k_resolve(const uint32_t *__restrict__ movers, const uint32_t *_
 //   X_o += -(W_u - A_o) + B_o,  X_n += (W_u - A_n) - B_n;
 // arcs u->v whose head moves in the same batch are corrected from the tail:
 //   X_o += w ([o != o_v] - [o != n_v]),  X_n += w ([n != n_v] - [n != o_v]).
+// The arcs of movers with more than big_deg arcs are left to k_commit_big,
+// which spreads them over many workgroups (all updates are exact integer
+// atomics, so the split does not change the result).
 template <int G>
 __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ movers,
                                                 const uint32_t *__restrict__ counters, const uint32_t *__restrict__ optr,
@@ -693,23 +754,28 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
                                                 const uint32_t *__restrict__ mod, const uint32_t *__restrict__ newmod,
                                                 const uint32_t *__restrict__ stamp, uint32_t cur,
                                                 const long long *__restrict__ agg, const Stat4 *__restrict__ us,
-                                                const int32_t *__restrict__ nmem, Stat4 *ms, int32_t *mNodes, int32_t *mUnits, uint8_t *act_next, int mark,
-                                                uint32_t *moved) {
+                                                const int32_t *__restrict__ nmem, Stat4 *ms, int32_t *mNodes,
+                                                int32_t *mUnits, uint8_t *act_next, int mark, uint32_t *moved,
+                                                uint32_t big_deg, uint32_t *big_list, uint32_t *big_cnt) {
   const uint32_t nm = counters[cMovers];
   const int lane = threadIdx.x % G;
   const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x / G;
+  uint32_t moved_here = 0;
   for (int64_t i = g0; i < nm; i += gstride) {
     const uint32_t u = movers[i];
     const uint32_t ob = optr[u], oe = optr[u + 1];
-    if (mark) {
+    const bool big = (oe - ob) + (iptr[u + 1] - iptr[u]) > big_deg;
+    if (big) {
+      if (lane == 0) big_list[atomicAdd(big_cnt, 1u)] = u;
+    } else if (mark) {
       for (uint32_t k = ob + lane; k < oe; k += G) act_next[oidx[k]] = 1;
       for (uint32_t k = iptr[u] + lane; k < iptr[u + 1]; k += G) act_next[iidx[k]] = 1;
     }
     if (stamp[u] != cur) continue;
     const uint32_t o = mod[u], n = newmod[u];
     long long dxo = 0, dxn = 0;
-    for (uint32_t k = ob + lane; k < oe; k += G) {
+    for (uint32_t k = ob + lane; k < (big ? ob : oe); k += G) {
       const uint32_t v = oidx[k];
       if (stamp[v] != cur) continue;
       const long long w = aout.fxw(k);
@@ -740,8 +806,96 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
       atomicAdd(&mNodes[n], k);
       atomicSub(&mUnits[o], 1);
       atomicAdd(&mUnits[n], 1);
-      atomicAdd(moved, (uint32_t)k);
+      moved_here += (uint32_t)k;
     }
+  }
+  moved_here = wave_sum(moved_here);
+  if ((threadIdx.x & (kWave - 1)) == 0 && moved_here) atomicAdd(moved, moved_here);
+}
+
+// Arc chunks of the deferred (high-degree) movers: prefix of chunk counts.
+constexpr int kCommitChunk = 4096;
+
+__global__ __launch_bounds__(256) void k_commit_plan(const uint32_t *__restrict__ big_list,
+                                                     const uint32_t *__restrict__ big_cnt,
+                                                     const uint32_t *__restrict__ optr,
+                                                     const uint32_t *__restrict__ iptr, uint32_t *__restrict__ A) {
+  __shared__ uint32_t lds[256 / kWave];
+  const int cnt = (int)*big_cnt;
+  uint32_t carry = 0;
+  if (threadIdx.x == 0) A[0] = 0;
+  for (int base = 0; base < cnt; base += 256) {
+    const int i = base + threadIdx.x;
+    uint32_t ac = 0;
+    if (i < cnt) {
+      const uint32_t u = big_list[i];
+      const uint32_t deg = (optr[u + 1] - optr[u]) + (iptr[u + 1] - iptr[u]);
+      ac = (deg + kCommitChunk - 1) / kCommitChunk;
+    }
+    uint32_t t;
+    const uint32_t inc = block_incl_scan(ac, lds, &t);
+    if (i < cnt) A[i + 1] = carry + inc;
+    carry += t;
+  }
+}
+
+// one workgroup per arc chunk of a deferred mover: activity marks and the
+// co-mover corrections of its out-arcs
+__global__ __launch_bounds__(256) void k_commit_big(const uint32_t *__restrict__ big_list,
+                                                    const uint32_t *__restrict__ big_cnt,
+                                                    const uint32_t *__restrict__ A, const uint32_t *__restrict__ optr,
+                                                    const uint32_t *__restrict__ oidx, const uint32_t *__restrict__ iptr,
+                                                    const uint32_t *__restrict__ iidx, ArcW aout,
+                                                    const uint32_t *__restrict__ mod,
+                                                    const uint32_t *__restrict__ newmod,
+                                                    const uint32_t *__restrict__ stamp, uint32_t cur, Stat4 *ms,
+                                                    uint8_t *act_next, int mark) {
+  __shared__ long long lds[2][256 / kWave];
+  const int cnt = (int)*big_cnt;
+  if (cnt == 0) return;
+  const uint32_t total = A[cnt];
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const int i = chunk_owner(A, cnt, c);
+    const uint32_t u = big_list[i];
+    const uint32_t ob = optr[u], od = optr[u + 1] - ob;
+    const uint32_t ib = iptr[u], deg = od + (iptr[u + 1] - ib);
+    const uint32_t j0 = (c - A[i]) * kCommitChunk;
+    const uint32_t j1 = min(deg, j0 + kCommitChunk);
+    const bool moves = stamp[u] == cur;
+    const uint32_t o = mod[u], n = newmod[u];
+    long long dxo = 0, dxn = 0;
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256) {
+      if (j < od) {
+        const uint32_t k = ob + j;
+        const uint32_t v = oidx[k];
+        if (mark) act_next[v] = 1;
+        if (moves && stamp[v] == cur) {
+          const long long w = aout.fxw(k);
+          const uint32_t ov = mod[v], nv = newmod[v];
+          dxo += w * ((long long)(o != ov) - (long long)(o != nv));
+          dxn += w * ((long long)(n != nv) - (long long)(n != ov));
+        }
+      } else if (mark) {
+        act_next[iidx[ib + (j - od)]] = 1;
+      }
+    }
+    dxo = wave_sum(dxo);
+    dxn = wave_sum(dxn);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      lds[0][threadIdx.x / kWave] = dxo;
+      lds[1][threadIdx.x / kWave] = dxn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && moves) {
+      long long so = 0, sn = 0;
+      for (int q = 0; q < 256 / kWave; ++q) {
+        so += lds[0][q];
+        sn += lds[1][q];
+      }
+      if (so) atomicAdd((unsigned long long *)&ms[o].x, (unsigned long long)so);
+      if (sn) atomicAdd((unsigned long long *)&ms[n].x, (unsigned long long)sn);
+    }
+    __syncthreads();
   }
 }
 
@@ -782,6 +936,7 @@ __global__ void k_hist256(const uint32_t *__restrict__ key, int64_t n, uint32_t 
 
 __global__ void k_reset_counters(uint32_t *counters) {
   if (threadIdx.x < 2) counters[threadIdx.x] = 0;
+  if (threadIdx.x == 2) counters[cBig] = 0;
 }
 
 __global__ void k_empty_flags(const int32_t *__restrict__ mNodes, int32_t n, uint32_t *__restrict__ flag) {
@@ -895,6 +1050,8 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       c.hub_bm.alloc(chunks);
     }
     if (c.hub_plan.n < 3 * ((int64_t)n_hubs + 1)) c.hub_plan.alloc(3 * ((int64_t)n_hubs + 1));
+    if (c.big_list.n < n_hubs) c.big_list.alloc(n_hubs);
+    if (c.big_plan.n < (int64_t)n_hubs + 1) c.big_plan.alloc((int64_t)n_hubs + 1);
   }
 
   // this sweep's units per (bucket, batch), inactive units last: one stable
@@ -972,7 +1129,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
         case 1: k_eval_group<16, 64><<<(unsigned)((cnt + 15) / 16), 256, 0, s>>>(a); break;
         case 2: k_eval_group<32, 256><<<(unsigned)((cnt + 7) / 8), 256, 0, s>>>(a); break;
         case 3: k_eval_group<64, 1024><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a); break;
-        case 4: k_eval_block<4096, false><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a); break;
+        case 4: k_eval_block<4096><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a); break;
         default: {
           HubPlan P{c.hub_plan.p, c.hub_plan.p + (cnt + 1), c.hub_plan.p + 2 * (cnt + 1), c.hub_bd.p, c.hub_bm.p};
           k_hub_plan<<<1, 256, 0, s>>>(a, P);
@@ -1000,17 +1157,28 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       uint32_t left = n_empty - empty_cursor;
       empty_cursor += n_want < left ? n_want : left;
     }
+    const uint32_t big_deg = n_hubs > 0 ? (uint32_t)kBucketMaxDeg[kNumBuckets - 2] : 0xffffffffu;
     k_commit<16><<<stride_grid((int64_t)n_movers * 16, 256), 256, 0, s>>>(
         c.movers.p, c.counters.p, L.out_ptr.p, L.out_idx.p, L.in_ptr.p, L.in_idx.p, aout, L.mod.p, c.newmod.p,
-        c.stamp.p, c.stamp_now, c.agg_buf.p, L.us.p, L.nmem.p, c.ms.p, c.mNodes.p, c.mUnits.p, c.act_next.p, prioritized ? 1 : 0, c.counters.p + cMoved);
+        c.stamp.p, c.stamp_now, c.agg_buf.p, L.us.p, L.nmem.p, c.ms.p, c.mNodes.p, c.mUnits.p, c.act_next.p,
+        prioritized ? 1 : 0, c.counters.p + cMoved, big_deg, c.big_list.p, c.counters.p + cBig);
     RMX_LAUNCH_CHECK();
+    if (n_hubs > 0) {
+      // only hubs (more than big_deg arcs) are deferred: the list holds at most n_hubs
+      k_commit_plan<<<1, 256, 0, s>>>(c.big_list.p, c.counters.p + cBig, L.out_ptr.p, L.in_ptr.p, c.big_plan.p);
+      RMX_LAUNCH_CHECK();
+      k_commit_big<<<2048, 256, 0, s>>>(c.big_list.p, c.counters.p + cBig, c.big_plan.p, L.out_ptr.p, L.out_idx.p,
+                                        L.in_ptr.p, L.in_idx.p, aout, L.mod.p, c.newmod.p, c.stamp.p, c.stamp_now,
+                                        c.ms.p, c.act_next.p, prioritized ? 1 : 0);
+      RMX_LAUNCH_CHECK();
+    }
     k_apply<<<stride_grid(n_movers, 256), 256, 0, s>>>(c.movers.p, c.counters.p, c.newmod.p, c.stamp.p,
                                                         c.stamp_now, L.mod.p);
     RMX_LAUNCH_CHECK();
     module_totals_async(c);   // S for the next batch
   }
   if (prioritized) {
-    std::swap(c.act.p, c.act_next.p);
+    c.act.swap(c.act_next);
     fill_u8(c.act_next.p, 0, c.N, s);
   }
   RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));

@@ -109,6 +109,42 @@ def test_contraction(key, pname):
 
 
 @pytest.mark.parametrize("key", ["lfr1k_mu03", "sbm1k", "lfr5k_mu05"])
+def test_contraction_from_super_level(key):
+    """Contracting an already contracted level (its super arcs summed again)
+    gives the structure of contracting the original arcs under the same node
+    partition (the oracle, which follows the reference); integer structure
+    bit-exact, weights 1e-13 relative (sums of partial sums round
+    differently), super-node statistics as for a fresh contraction."""
+    from oracle.pyoracle import OracleNet
+    with _gpu(key) as net:
+        for _ in range(2):
+            net.move(prioritized=True, seed=5)
+        net.update_members_in_module()
+        net.convert_modules_to_super_nodes()
+        for _ in range(3):
+            net.move(prioritized=True, seed=6)
+        net.update_members_in_module()
+        ns, ms = net.convert_modules_to_super_nodes()     # from the super level
+        s = net.super_nodes()
+        part = net.partition()
+        print(f"[gpu] contraction of contraction {key}: {ns} super nodes, {ms} super arcs", flush=True)
+    o = OracleNet.load(data_path(INPUTS[key])).initiate()
+    o.set_partition(part)
+    o.update_members()
+    o.contract()
+    r = o.supernodes()
+    assert ns == len(r["mod"]) and ms == len(r["out_idx"])
+    for k in ("out_ptr", "in_ptr", "out_idx", "in_idx", "mod", "n_members"):
+        assert np.array_equal(np.asarray(s[k], dtype=np.int64), np.asarray(r[k], dtype=np.int64)), k
+    assert np.allclose(s["out_w"], r["out_w"], rtol=1e-13, atol=0)
+    assert np.allclose(s["in_w"], r["in_w"], rtol=1e-13, atol=0)
+    assert np.allclose(s["size"], r["size"], rtol=1e-12, atol=1e-18)
+    assert np.allclose(s["tp"], r["tp"], rtol=1e-12, atol=1e-18)
+    assert np.allclose(s["dangling"], r["dangling"], rtol=1e-12, atol=1e-18)
+    assert np.allclose(s["exit"], r["exit"], rtol=1e-10, atol=1e-17)
+
+
+@pytest.mark.parametrize("key", ["lfr1k_mu03", "sbm1k", "lfr5k_mu05"])
 @pytest.mark.parametrize("prior", [True, False])
 def test_sweeps_consistent(key, prior):
     """The tracked codelength after every sweep equals the from-scratch

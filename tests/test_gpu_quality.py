@@ -105,3 +105,16 @@ def test_determinism_of_run():
             r = net.run(seed=9)
             out.append((net.partition(), r.codelength))
     assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+
+
+def test_rerun_on_same_network_repeats():
+    """initiate + run twice on one network gives the same partition."""
+    from relaxmap_amd import Network
+    with Network() as net:
+        net.load(data_path(INPUTS["lfr5k_mu05"]))
+        out = []
+        for _ in range(2):
+            net.initiate()
+            r = net.run(seed=3)
+            out.append((net.partition(), r.codelength))
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
