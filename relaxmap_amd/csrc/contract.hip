@@ -19,6 +19,7 @@
 // original arcs by rounding only.
 #include "devutil.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace rmx {
@@ -93,13 +94,14 @@ __global__ void k_key_starts(const uint32_t *__restrict__ flag, const uint32_t *
   if (i < m && flag[i]) start[pos[i]] = (uint32_t)i;
 }
 
-// one thread per (src, dst) super arc: sequential FP64 sum in arc order
+// one thread per (src, dst) super arc: sequential FP64 sum in arc order (in
+// mixed precision, from the node level: of the FP32 arc flows)
 __global__ void k_reduce_segments(const uint32_t *__restrict__ start, int64_t n_seg, int64_t m,
                                   const uint64_t *__restrict__ keys, const uint32_t *__restrict__ arc,
                                   const double *__restrict__ w, const float *__restrict__ wf,
                                   const long long *__restrict__ wx, int mixed, int bits,
                                   uint32_t *__restrict__ out_idx, double *__restrict__ out_w,
-                                  long long *__restrict__ out_wx, uint32_t *__restrict__ src_cnt) {
+                                  long long *__restrict__ out_wx) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_seg) return;
   const uint32_t b = start[s];
@@ -108,26 +110,31 @@ __global__ void k_reduce_segments(const uint32_t *__restrict__ start, int64_t n_
   long long sx = 0;
   for (uint32_t i = b; i < e; ++i) {
     const uint32_t k = arc[i];
-    const double f = w[k];
+    // mixed precision reads only the FP32 flow of a node-level arc: its
+    // FP64 sum and its fixed-point value (the one the sweeps use) both
+    // come from it
+    const double f = (mixed && !wx) ? (double)wf[k] : w[k];
     sum += f;
-    sx += wx ? wx[k] : fx_of(mixed ? (double)wf[k] : f);
+    sx += wx ? wx[k] : fx_of(f);
   }
   const uint64_t key = keys[b];
   out_idx[s] = (uint32_t)(key & ((1ull << bits) - 1ull));
   out_w[s] = sum;
   out_wx[s] = sx;
-  atomicAdd(&src_cnt[key >> bits], 1u);
 }
 
-__global__ void k_scatter_in(const uint32_t *__restrict__ in_of_out, int64_t m, const double *__restrict__ ow,
-                             const long long *__restrict__ owx, const float *__restrict__ owf, double *__restrict__ iw,
-                             long long *__restrict__ iwx, float *__restrict__ iwf) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  const uint32_t k = in_of_out[e];
-  iw[k] = ow[e];
-  iwx[k] = owx[e];
-  iwf[k] = owf[e];
+// out_ptr of the super level from the segments' sources (ascending): segment
+// i fills every super node in (src[i-1], src[i]], and position n_seg the tail
+__global__ void k_super_out_ptr(const uint32_t *__restrict__ start, int64_t n_seg,
+                                const uint64_t *__restrict__ keys, int bits, int32_t ns,
+                                uint32_t *__restrict__ ptr) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i <= n_seg; i += stride) {
+    const int64_t lo = i == 0 ? 0 : (int64_t)(keys[start[i - 1]] >> bits) + 1;
+    const int64_t hi = i == n_seg ? (int64_t)ns : (int64_t)(keys[start[i]] >> bits);
+    for (int64_t d = lo; d <= hi; ++d) ptr[d] = (uint32_t)i;
+  }
 }
 
 __global__ void k_to_float(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
@@ -251,9 +258,7 @@ void contract(Ctx &c) {
 
   // one super arc per distinct key
   uint32_t m_super = 0;
-  DevBuf<uint32_t> kflag(c.cache), kpos(c.cache), kstart(c.cache), src_cnt(c.cache);
-  src_cnt.alloc((int64_t)ns + 1);
-  fill_u32(src_cnt.p, 0u, (int64_t)ns + 1, s);
+  DevBuf<uint32_t> kflag(c.cache), kpos(c.cache), kstart(c.cache);
   if (m_inter > 0) {
     kflag.alloc(m_inter);
     kpos.alloc(m_inter);
@@ -277,31 +282,26 @@ void contract(Ctx &c) {
   if (m_super > 0) {
     k_reduce_segments<<<grid_for(m_super, 256), 256, 0, s>>>(
         kstart.p, m_super, m_inter, keys.p, arcs.p, S.out_w.p, S.out_wf.p, from_super ? S.out_wx.p : nullptr,
-        c.precision == 1 ? 1 : 0, bits, D.out_idx.p, D.out_w.p, D.out_wx.p, src_cnt.p);
+        c.precision == 1 ? 1 : 0, bits, D.out_idx.p, D.out_w.p, D.out_wx.p);
     RMX_LAUNCH_CHECK();
     k_to_float<<<grid_for(m_super, 256), 256, 0, s>>>(D.out_w.p, D.out_wf.p, m_super);
     RMX_LAUNCH_CHECK();
+  }
+  D.out_ptr.alloc((int64_t)ns + 1);
+  if (m_super > 0) {
+    k_super_out_ptr<<<(unsigned)std::min<int64_t>(16384, (m_super + 256) / 256), 256, 0, s>>>(
+        kstart.p, m_super, keys.p, bits, (int32_t)ns, D.out_ptr.p);
+    RMX_LAUNCH_CHECK();
+  } else {
+    fill_u32(D.out_ptr.p, 0u, (int64_t)ns + 1, s);
   }
   c.sync();
   keys.free();
   arcs.free();
   kstart.free();
   usp_buf.free();
-  D.out_ptr.alloc((int64_t)ns + 1);
-  exclusive_scan_u32(src_cnt.p, D.out_ptr.p, (int64_t)ns + 1, nullptr, c.scratch, s);
-  c.sync();
 
-  // transpose
-  DevBuf<uint32_t> in_of_out(c.cache);
-  build_in_csr(c, D, &in_of_out);
-  D.in_w.alloc(m_super);
-  D.in_wx.alloc(m_super);
-  D.in_wf.alloc(m_super);
-  if (m_super > 0) {
-    k_scatter_in<<<grid_for(m_super, 256), 256, 0, s>>>(in_of_out.p, m_super, D.out_w.p, D.out_wx.p, D.out_wf.p,
-                                                         D.in_w.p, D.in_wx.p, D.in_wf.p);
-    RMX_LAUNCH_CHECK();
-  }
+  build_in_csr_weighted(c, D);   // transpose
   build_buckets(c, D);
   c.sync();
   c.L1.swap(c.L2);

@@ -43,6 +43,24 @@ __global__ void k_arc_src(const uint32_t *__restrict__ ptr, int32_t n, uint32_t 
   for (uint32_t e = ptr[wave] + lane; e < ptr[wave + 1]; e += kWave) src[e] = (uint32_t)wave;
 }
 
+// after the stable sort, with the arc weights (contracted levels): every
+// in-arc gathers its source and weights from the out-arc it mirrors
+__global__ void k_in_gather(const uint32_t *__restrict__ sorted_e, const uint32_t *__restrict__ src, int64_t m,
+                            const double *__restrict__ ow, const long long *__restrict__ owx,
+                            uint32_t *__restrict__ in_idx, double *__restrict__ iw,
+                            long long *__restrict__ iwx, float *__restrict__ iwf) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; k < m; k += stride) {
+    const uint32_t e = sorted_e[k];
+    in_idx[k] = src[e];
+    const double f = ow[e];
+    iw[k] = f;
+    iwx[k] = owx[e];
+    iwf[k] = (float)f;   // the FP32 mirror of a contracted arc is its rounded FP64 weight
+  }
+}
+
 // in_ptr from the destinations sorted ascending: in_ptr[d] is the first
 // position whose destination is >= d, so position i fills every d in
 // (key[i-1], key[i]] (and position m fills the tail up to n)
@@ -139,13 +157,22 @@ void upload_graph(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const in
   c.initiated = false;
 }
 
-void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) {
+namespace {
+
+// in_of_out: the in-CSR position of every out-arc (level 0); weighted: the
+// in-arc weights are gathered from the out-arc weights (contracted levels)
+void transpose(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out, bool weighted) {
   hipStream_t s = c.stream;
   const int64_t m = L.m;
   const int32_t n = L.n;
   L.in_ptr.alloc(n + 1);
   L.in_idx.alloc(m);
   if (in_of_out) in_of_out->alloc(m);
+  if (weighted) {
+    L.in_w.alloc(m);
+    L.in_wx.alloc(m);
+    L.in_wf.alloc(m);
+  }
   if (m == 0) {
     fill_u32(L.in_ptr.p, 0u, n + 1, s);
     c.sync();
@@ -163,13 +190,24 @@ void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) {
     int bits = 1;
     while (bits < 32 && ((int64_t)1 << bits) < (int64_t)n) ++bits;
     radix_sort_pairs_u32(keys.p, vals.p, m, 0, bits, c.scratch, s);
-    k_in_from_sorted<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, m, L.in_idx.p, in_of_out ? in_of_out->p : nullptr);
+    if (weighted)
+      k_in_gather<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, m, L.out_w.p, L.out_wx.p, L.in_idx.p,
+                                             L.in_w.p, L.in_wx.p, L.in_wf.p);
+    else
+      k_in_from_sorted<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, m, L.in_idx.p,
+                                                  in_of_out ? in_of_out->p : nullptr);
     RMX_LAUNCH_CHECK();
     k_ptr_from_sorted<<<gs_grid(m + 1), 256, 0, s>>>(keys.p, m, n, L.in_ptr.p);
     RMX_LAUNCH_CHECK();
     c.sync();
   }
 }
+
+}  // namespace
+
+void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) { transpose(c, L, in_of_out, false); }
+
+void build_in_csr_weighted(Ctx &c, Level &L) { transpose(c, L, nullptr, true); }
 
 void build_buckets(Ctx &c, Level &L) {
   hipStream_t s = c.stream;

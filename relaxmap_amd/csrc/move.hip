@@ -470,7 +470,8 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
 // (clear, insert, evaluate) runs one workgroup per chunk, so a single huge
 // super node no longer serialises a launch; a final pass reduces each hub's
 // per-chunk best candidates in a fixed order.
-constexpr int kHubChunk = 4096;
+constexpr int kHubChunk = 4096;      // table slots per evaluation workgroup
+constexpr int kHubArcChunk = 4096;   // arcs per insertion workgroup
 
 __device__ inline int chunk_owner(const uint32_t *__restrict__ pre, int cnt, uint32_t c) {
   int lo = 0, hi = cnt;   // pre[lo] <= c < pre[hi]
@@ -534,7 +535,7 @@ __global__ __launch_bounds__(256) void k_hub_plan(EvalArgs a, HubPlan P) {
       int cap;
       hub_of(a, i, u, off, cap);
       const uint32_t deg = (a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u]);
-      ac = deg == 0 ? 1u : (deg + kHubChunk - 1) / kHubChunk;
+      ac = deg == 0 ? 1u : (deg + kHubArcChunk - 1) / kHubArcChunk;
       sc = ((uint32_t)cap + kHubChunk - 1) / kHubChunk;
     }
     uint32_t ta, ts;
@@ -571,6 +572,9 @@ __global__ __launch_bounds__(256) void k_hub_clear(EvalArgs a, HubPlan P) {
   }
 }
 
+// One workgroup per chunk of kHubArcChunk arcs of a hub, straight into the
+// hub's global table.  (Aggregating a chunk in LDS first was measured slower:
+// a 2048-arc chunk rarely repeats a module, and the table halves occupancy.)
 __global__ __launch_bounds__(256) void k_hub_insert(EvalArgs a, HubPlan P) {
   const uint32_t total = P.tot[0];
   for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
@@ -581,8 +585,8 @@ __global__ __launch_bounds__(256) void k_hub_insert(EvalArgs a, HubPlan P) {
     hub_of(a, i, u, base, cap);
     const uint32_t ob = a.optr[u], od = a.optr[u + 1] - ob;
     const uint32_t ib = a.iptr[u], deg = od + (a.iptr[u + 1] - ib);
-    const uint32_t j0 = (c - P.A[i]) * kHubChunk;
-    const uint32_t j1 = min(deg, j0 + kHubChunk);
+    const uint32_t j0 = (c - P.A[i]) * kHubArcChunk;
+    const uint32_t j1 = min(deg, j0 + kHubArcChunk);
     uint32_t *keys = a.hkeys + base;
     for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256) {
       if (j < od) {

@@ -166,6 +166,10 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_upsweep(const K *__restrict
   counts[(int64_t)threadIdx.x * n_tiles + blockIdx.x] = hist[threadIdx.x];
 }
 
+// Stable scatter of one tile: every element's rank within its digit is found
+// with wavefront ballots (tile order), the tile is reordered by digit in LDS,
+// and each digit's run is then written to its global range with consecutive
+// lanes on consecutive addresses.
 template <typename K>
 __global__ __launch_bounds__(kSortBlock) void k_sort_downsweep(const K *__restrict__ keys_in,
                                                                const uint32_t *__restrict__ vals_in,
@@ -174,48 +178,91 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_downsweep(const K *__restri
                                                                int shift, int64_t n_tiles,
                                                                const uint32_t *__restrict__ offsets) {
   constexpr int W = kSortBlock / kWave;
+  __shared__ K s_key[kSortTile];
+  __shared__ uint32_t s_val[kSortTile];
   __shared__ uint32_t tile_off[kRadix];
+  __shared__ uint32_t digit_start[kRadix];
   __shared__ uint32_t wave_cnt[W][kRadix];
   __shared__ uint32_t wave_off[W][kRadix];
+  __shared__ uint32_t wsum[W];
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
   tile_off[t] = offsets[(int64_t)t * n_tiles + blockIdx.x];
+  digit_start[t] = 0;
 #pragma unroll
   for (int i = 0; i < W; ++i) wave_cnt[i][t] = 0;
-  uint32_t running = 0;   // digit t's count in earlier steps
+  __syncthreads();
+  K key[kSortItems];
+  uint32_t val[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const int64_t j = base + (int64_t)i * kSortBlock + t;
+    const bool valid = j < n;
+    key[i] = valid ? keys_in[j] : K(0);
+    val[i] = valid ? vals_in[j] : 0u;
+    if (valid) atomicAdd(&digit_start[(uint32_t)(key[i] >> shift) & 0xffu], 1u);
+  }
+  __syncthreads();
+  {   // exclusive scan of the tile's digit counts (thread t holds digit t)
+    const uint32_t c = digit_start[t];
+    uint32_t x = c;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off, kWave);
+      if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) wsum[w] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int q = 0; q < w; ++q) pre += wsum[q];
+    digit_start[t] = pre + x - c;
+  }
   __syncthreads();
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  uint32_t running = 0;   // digit t's count in earlier steps
+#pragma unroll
   for (int i = 0; i < kSortItems; ++i) {
-    int64_t j = base + (int64_t)i * kSortBlock + t;
+    const int64_t j = base + (int64_t)i * kSortBlock + t;
     const bool valid = j < n;
-    K key = valid ? keys_in[j] : K(0);
-    uint32_t val = valid ? vals_in[j] : 0u;
-    uint32_t d = (uint32_t)(key >> shift) & 0xffu;
+    const uint32_t d = (uint32_t)(key[i] >> shift) & 0xffu;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      uint64_t bal = __ballot((d >> b) & 1u);
+      const uint64_t bal = __ballot((d >> b) & 1u);
       peers &= ((d >> b) & 1u) ? bal : ~bal;
     }
-    uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
     if (valid && rank == 0) wave_cnt[w][d] = (uint32_t)__popcll(peers);
     __syncthreads();
     {
       uint32_t run = running;
 #pragma unroll
       for (int q = 0; q < W; ++q) {
-        uint32_t c = wave_cnt[q][t];
+        const uint32_t cq = wave_cnt[q][t];
         wave_off[q][t] = run;
-        run += c;
+        run += cq;
         wave_cnt[q][t] = 0;
       }
       running = run;
     }
     __syncthreads();
     if (valid) {
-      uint32_t pos = tile_off[d] + wave_off[w][d] + rank;
-      keys_out[pos] = key;
-      vals_out[pos] = val;
+      const uint32_t p = digit_start[d] + wave_off[w][d] + rank;
+      s_key[p] = key[i];
+      s_val[p] = val[i];
+    }
+  }
+  __syncthreads();
+  const int64_t tile_n = (n - base) < kSortTile ? (n - base) : kSortTile;
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const int idx = i * kSortBlock + t;
+    if (idx < tile_n) {
+      const K k = s_key[idx];
+      const uint32_t d = (uint32_t)(k >> shift) & 0xffu;
+      const uint32_t pos = tile_off[d] + ((uint32_t)idx - digit_start[d]);
+      keys_out[pos] = k;
+      vals_out[pos] = s_val[idx];
     }
   }
 }
