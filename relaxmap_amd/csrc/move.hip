@@ -160,6 +160,39 @@ __device__ inline void emit(const EvalArgs &a, uint32_t u, const Best &b, long l
   }
 }
 
+// Lane mask of the lanes below this one in the wavefront.
+__device__ inline uint64_t lanes_below() {
+  const int wl = threadIdx.x & (kWave - 1);
+  return wl == 0 ? 0ull : (~0ull >> (kWave - wl));
+}
+
+// Called by every lane of a wavefront (uniform control flow); the lanes with
+// `active` set emit their unit as emit() does, with one atomic on the mover
+// counter per wavefront instead of one per mover (millions of movers per
+// batch would otherwise serialise on that one address).
+__device__ inline void emit_wave(const EvalArgs &a, bool active, uint32_t u, const Best &b, long long fo_o,
+                                 long long fi_o, long long fo_n, long long fi_n) {
+  const bool mv = active && b.d < a.vthresh && b.m != kNone;
+  const uint64_t bal = __ballot(mv);
+  if (bal == 0) return;
+  const int leader = __ffsll((unsigned long long)bal) - 1;
+  uint32_t base = 0;
+  if ((int)(threadIdx.x & (kWave - 1)) == leader) base = atomicAdd(&a.counters[cMovers], (uint32_t)__popcll(bal));
+  base = __shfl(base, leader, kWave);
+  if (!mv) return;
+  a.target[u] = b.m;
+  long long *g = a.agg + 4 * (int64_t)u;
+  g[0] = fo_o;
+  g[1] = fi_o;
+  g[2] = fo_n;
+  g[3] = fi_n;
+  a.movers[base + (uint32_t)__popcll(bal & lanes_below())] = u;
+  if (b.m == a.empty_key) {
+    a.want[u] = 1u;
+    atomicAdd(&a.counters[cWant], 1u);
+  }
+}
+
 // ------------------------------------------------------------------ groups
 template <int G>
 __device__ inline Best group_best(Best b) {
@@ -213,12 +246,6 @@ __device__ inline int lds_find(const uint32_t *keys, int H, uint32_t m) {
   }
 }
 
-// Lane mask of the lanes below this one in the wavefront.
-__device__ inline uint64_t lanes_below() {
-  const int wl = threadIdx.x & (kWave - 1);
-  return wl == 0 ? 0ull : (~0ull >> (kWave - wl));
-}
-
 // G lanes per unit (G <= 64: the group lives inside one wavefront), H keys.
 // After the flows are aggregated, the non-empty slots are compacted in place
 // to the front of the table (a slot only ever moves down, and every lane
@@ -243,6 +270,8 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
     u = a.bucket[a.list[g]];
     work = true;
   }
+  Best b{0.0, kNone, -1};
+  long long fo_o = 0, fi_o = 0, fo_n = 0, fi_n = 0;
   if (work) {
     for (int s = lane; s < H; s += G) {
       keys[s] = kEmptyKey;
@@ -268,7 +297,10 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
     if (lane == 0) visits = (oe - ob) + (ie - ib);
     const uint32_t o = a.mod[u];
     const int so = lds_find(keys, H, o);
-    const long long fo_o = so >= 0 ? fo[so] : 0, fi_o = so >= 0 ? fi[so] : 0;
+    if (so >= 0) {
+      fo_o = fo[so];
+      fi_o = fi[so];
+    }
     // compact the non-empty slots to [0, nc) (the group's lanes of the wavefront)
     const int wl = threadIdx.x & (kWave - 1);
     const uint64_t gmask = G == kWave ? ~0ull : (((1ull << G) - 1ull) << (wl - lane));
@@ -294,7 +326,6 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
     if (lane == 0) ncand = (unsigned long long)nc;
     UnitCtx U;
     unit_ctx(a, u, fo_o, fi_o, U);
-    Best b{0.0, kNone, -1};
     for (int i = lane; i < nc; i += G) {
       const uint32_t m = keys[i];
       if (m == o) continue;
@@ -307,36 +338,40 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
       if (better(dl, a.empty_key, b) && dl < 0.0) { b.d = dl; b.m = a.empty_key; b.i = -1; }
     }
     b = group_best<G>(b);
-    if (lane == 0) {
-      const long long fo_n = b.i >= 0 ? fo[b.i] : 0, fi_n = b.i >= 0 ? fi[b.i] : 0;
-      emit(a, u, b, fo_o, fi_o, fo_n, fi_n);
+    if (lane == 0 && b.i >= 0) {
+      fo_n = fo[b.i];
+      fi_n = fi[b.i];
     }
   }
+  emit_wave(a, work && lane == 0, u, b, fo_o, fi_o, fo_n, fi_n);
   // edge visits, candidates and units: one atomic each per block (the
   // partial sums reuse the first table once every group is done with it, so
   // the kernel's LDS is exactly its tables)
   constexpr int NW = 256 / kWave;
-  static_assert(H >= 2 * NW, "table too small for the block partials");
+  static_assert(H >= 3 * NW, "table too small for the block partials");
+  unsigned long long units = (work && lane == 0) ? 1ull : 0ull;   // a unit with no arcs is still one evaluation
   visits = wave_sum(visits);
   ncand = wave_sum(ncand);
+  units = wave_sum(units);
   __syncthreads();
   unsigned long long *red = (unsigned long long *)&s_out[0][0];
   if ((threadIdx.x & (kWave - 1)) == 0) {
     red[threadIdx.x / kWave] = visits;
     red[NW + threadIdx.x / kWave] = ncand;
+    red[2 * NW + threadIdx.x / kWave] = units;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long t = 0, c = 0;
+    unsigned long long t = 0, c = 0, nu = 0;
     for (int i = 0; i < NW; ++i) {
       t += red[i];
       c += red[NW + i];
+      nu += red[2 * NW + i];
     }
     if (t) atomicAdd(a.visits, t);
     if (c) atomicAdd(a.cands, c);
-    // a unit with no arcs is still one evaluation
+    if (nu) atomicAdd(a.units, (uint32_t)nu);
   }
-  if (work && lane == 0) atomicAdd(a.units, 1u);
 }
 
 // One 256-lane workgroup per unit; the LDS hash table holds up to H keys,
