@@ -45,6 +45,14 @@ const char *rmx_last_error(const rmx_ctx *ctx);
 int rmx_set_precision(rmx_ctx *ctx, int precision);
 int rmx_set_batches(rmx_ctx *ctx, int batches);
 
+/* How units with more than 2048 arcs are evaluated: 0 (default) partitions
+ * their arcs by neighbour module and aggregates each partition in LDS;
+ * 1 uses one global-memory hash table per unit (device atomics per arc);
+ * 2 partitions, then re-evaluates every hub as 0 does when a partition's
+ * table overflows (exercises that fallback).  All give identical moves.
+ * No reference counterpart. */
+int rmx_set_hub_path(rmx_ctx *ctx, int global_tables);
+
 /* Network construction: replaces the arc-list build of OmpRelaxmap.cpp:113-145
  * (after load_pajek_format_network / load_linkList_format_network,
  * FileIO.cpp:32-265).  out_ptr[n+1] / out_idx[m] / w[m] is the out-CSR with

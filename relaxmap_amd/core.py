@@ -69,6 +69,7 @@ def load_library():
         "rmx_last_error": (C.c_char_p, [vp]),
         "rmx_set_precision": (C.c_int, [vp, C.c_int]),
         "rmx_set_batches": (C.c_int, [vp, C.c_int]),
+        "rmx_set_hub_path": (C.c_int, [vp, C.c_int]),
         "rmx_graph_upload": (C.c_int, [vp, C.c_int32, C.c_int64, _I64P, _I32P, _F64P, _F64P]),
         "rmx_load_network": (C.c_int, [vp, C.c_char_p, _I32P, _I64P, _I32P, _I32P]),
         "rmx_initiate": (C.c_int, [vp, _I32P, _F64P, _I64P]),
@@ -123,6 +124,12 @@ class Network:
         self.m = 0
         self._check(L.rmx_set_precision(h, RMX_MIXED if precision == "mixed" else RMX_FP64))
         self._check(L.rmx_set_batches(h, int(batches)))
+
+    def set_hub_path(self, mode):
+        """Evaluate units with more than 2048 arcs by LDS partitions (0, the
+        default), with one global hash table each (1), or by partitions with
+        every hub sent through the overflow fallback (2); same moves."""
+        self._check(self._L.rmx_set_hub_path(self._h, int(mode)))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -133,6 +133,14 @@ int rmx_set_batches(rmx_ctx *ctx, int batches) {
   });
 }
 
+int rmx_set_hub_path(rmx_ctx *ctx, int global_tables) {
+  return guarded(ctx, [&] {
+    if (global_tables < 0 || global_tables > 2) return fail(ctx, RMX_EINVAL, "hub path must be 0, 1 or 2");
+    ctx->c.hub_global = global_tables;
+    return RMX_OK;
+  });
+}
+
 int rmx_graph_upload(rmx_ctx *ctx, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx,
                      const double *w, const double *node_w) {
   return guarded(ctx, [&] {

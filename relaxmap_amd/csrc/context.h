@@ -16,6 +16,9 @@ namespace rmx {
 constexpr int kNumBuckets = 6;
 // upper bound of (out + in) degree per bucket
 constexpr int kBucketMaxDeg[kNumBuckets] = {16, 32, 128, 512, 2048, 0x7fffffff};
+// Hubs are evaluated in partitions of about kHubPart arcs (split by a hash of
+// the neighbour's module), each aggregated in an LDS table.
+constexpr int kHubPart = 1024;
 
 // Device blocks kept for reuse within one context.  All work of a context
 // runs on one stream, so a block handed back while kernels that use it are
@@ -126,6 +129,8 @@ struct Level {
   // nextpow2(2 deg + 2) keys; hub_off[i] is the region of the i-th hub
   DevBuf<long long> hub_off;
   int64_t hub_pool = 0;
+  int64_t hub_arcs = 0;    // sum of the hubs' degrees
+  int64_t hub_parts = 0;   // partitions of all hubs (partitioned hub evaluation)
   int32_t max_deg = 0;
   void free_all();
   void swap(Level &o);
@@ -173,6 +178,7 @@ struct Ctx {
   int64_t E = 0;
   int precision = 0;   // 0 FP64, 1 mixed (FP32 bulk, FP64 accumulation)
   int batches = 8;     // sub-sweeps per sweep
+  int hub_global = 0;  // 1: hubs with global-memory tables instead of LDS partitions
   bool initiated = false;
 
   Level L0, L1;
@@ -198,6 +204,14 @@ struct Ctx {
   DevBuf<uint32_t> hub_plan, hub_bm;  // chunk prefixes of the hubs in a batch; best module per chunk
   DevBuf<double> hub_bd;              // best dL per chunk
   DevBuf<uint32_t> big_list, big_plan;  // commit: movers with hub degree and their arc-chunk prefix
+  // partitioned hub evaluation: per batch hub arc, per partition, per hub
+  DevBuf<uint32_t> hp_mtmp, hp_ekey;
+  DevBuf<long long> hp_ew;
+  DevBuf<uint32_t> hp_bcnt, hp_boff, hp_bm;
+  DevBuf<double> hp_bd;
+  DevBuf<long long> hp_bfo, hp_bfi;
+  DevBuf<uint32_t> hp_plan, hp_ovf;
+  DevBuf<long long> hp_own;
   DevBuf<uint32_t> counters;         // small device counters
   DevBuf<double> scal;               // device scalars: S, L, ...
   DevBuf<double> partial;            // block partials for reductions

@@ -17,6 +17,8 @@ void Level::free_all() {
   p.free(); tau.free(); us.free();
   nmem.free(); mod.free(); bucket.free(); hub_off.free();
   hub_pool = 0;
+  hub_arcs = 0;
+  hub_parts = 0;
   n = 0; m = 0;
 }
 
@@ -30,6 +32,8 @@ void Level::swap(Level &o) {
   nmem.swap(o.nmem); mod.swap(o.mod); bucket.swap(o.bucket); hub_off.swap(o.hub_off);
   for (int b = 0; b <= kNumBuckets; ++b) std::swap(bucket_off[b], o.bucket_off[b]);
   std::swap(hub_pool, o.hub_pool);
+  std::swap(hub_arcs, o.hub_arcs);
+  std::swap(hub_parts, o.hub_parts);
   std::swap(max_deg, o.max_deg);
 }
 
@@ -233,6 +237,8 @@ void build_buckets(Ctx &c, Level &L) {
   const int32_t nh = L.bucket_off[kNumBuckets] - L.bucket_off[kNumBuckets - 1];
   L.hub_off.alloc(nh + 1);
   L.hub_pool = 0;
+  L.hub_arcs = 0;
+  L.hub_parts = 0;
   if (nh > 0) {
     DevBuf<uint32_t> deg(c.cache);
     deg.alloc(nh);
@@ -247,6 +253,8 @@ void build_buckets(Ctx &c, Level &L) {
       long long cap = 1;
       while (cap < 2LL * hd[i] + 2) cap <<= 1;
       off[i + 1] = off[i] + cap;
+      L.hub_arcs += hd[i];
+      L.hub_parts += ((int64_t)hd[i] + kHubPart - 1) / kHubPart;
     }
     L.hub_pool = off[nh];
     RMX_HIP(hipMemcpyAsync(L.hub_off.p, off.data(), sizeof(long long) * (nh + 1), hipMemcpyHostToDevice, s));

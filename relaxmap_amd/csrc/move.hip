@@ -39,7 +39,7 @@ namespace rmx {
 namespace {
 
 enum { kS = 0, kL = 3, kNM = 7 };
-enum { cMovers = 0, cWant = 1, cMoved = 2, cVisitsLo = 4, cCandsLo = 6, cUnits = 10, cBig = 11 };
+enum { cMovers = 0, cWant = 1, cMoved = 2, cVisitsLo = 4, cCandsLo = 6, cUnits = 10, cBig = 11, cHubOvf = 12 };
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kInvalid = 0xfffffffeu;
 
@@ -544,6 +544,7 @@ struct HubPlan {
   uint32_t *tot;    // [0] arc chunks, [1] slot chunks
   double *bd;       // best dL per slot chunk
   uint32_t *bm;     // best module per slot chunk
+  const uint32_t *only;   // finish only the hubs flagged here (null: all)
 };
 
 __device__ inline void hub_of(const EvalArgs &a, int i, uint32_t &u, long long &base, int &cap) {
@@ -725,6 +726,7 @@ __global__ __launch_bounds__(256) void k_hub_finish(EvalArgs a, HubPlan P) {
   const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t wstride = (int64_t)gridDim.x * blockDim.x / kWave;
   for (int64_t i = w0; i < a.list_n; i += wstride) {
+    if (P.only && !P.only[i]) continue;
     uint32_t u;
     long long base;
     int cap;
@@ -747,6 +749,366 @@ __global__ __launch_bounds__(256) void k_hub_finish(EvalArgs a, HubPlan P) {
         fo_n = ld_ll(&a.hout[base + sn]);
         fi_n = ld_ll(&a.hin[base + sn]);
       }
+      emit(a, u, b, fo_o, fi_o, fo_n, fi_n);
+      atomicAdd(a.visits, (unsigned long long)((a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u])));
+      atomicAdd(a.units, 1u);
+    }
+  }
+}
+
+// ------------------------------------------------------- hubs, partitioned
+// The hubs of a batch are evaluated with no global atomic per arc: their
+// arcs are split into partitions of about kHubPart arcs by a hash of the
+// neighbour's module (all arcs into one module land in one partition), the
+// partitions are laid out contiguously (count, scan, scatter), and one
+// workgroup per partition aggregates its flows in an LDS table and evaluates
+// its candidates densely; a last pass takes every hub's best over its
+// partitions in a fixed order.  Should a partition ever hold more distinct
+// modules than its table (far beyond the expected kHubPart), its hub is
+// flagged and evaluated again by the global-table kernels above.
+constexpr int kHubTable = 2 * kHubPart;   // LDS slots of a partition
+constexpr int kHubHist = 4096;            // LDS bins of a chunk's histogram (more: global atomics)
+constexpr uint32_t kInBit = 0x80000000u;  // entry direction: in-arc
+
+struct HubParts {
+  uint32_t *A;      // arc-chunk prefix per hub (n_h + 1)
+  uint32_t *PB;     // partition prefix per hub (n_h + 1)
+  uint32_t *AB;     // arc prefix per hub (n_h + 1)
+  uint32_t *tot;    // [0] arc chunks, [1] partitions, [2] arcs
+  uint32_t *bcnt;   // entries per partition; then the scatter cursor
+  uint32_t *boff;   // first entry of each partition (scan of the counts)
+  uint32_t *mtmp;   // module of each batch hub arc (kInBit: in-arc)
+  uint32_t *ekey;   // entries grouped by partition (module | kInBit)
+  long long *ew;    // ... and their fixed-point flows
+  long long *own;   // per hub: flows to and from its own module
+  uint32_t *ovf;    // per hub: some partition overflowed
+  double *bd;       // per partition: best dL, module, and its flows
+  uint32_t *bm;
+  long long *bfo, *bfi;
+  int force_ovf;    // treat every partition as overflowed (tests the fallback)
+};
+
+__device__ inline uint32_t part_of(uint32_t m, uint32_t np) {
+  return (uint32_t)(((uint64_t)hash_u32(m ^ 0x9e3779b9u) * np) >> 32);
+}
+
+__device__ inline int lds_insert_bounded(uint32_t *keys, int H, uint32_t m) {
+  uint32_t h = hash_u32(m) & (uint32_t)(H - 1);
+  for (int probe = 0; probe < H; ++probe) {
+    const uint32_t k = keys[h];
+    if (k == m) return (int)h;
+    if (k == kEmptyKey) {
+      const uint32_t old = atomicCAS(&keys[h], kEmptyKey, m);
+      if (old == kEmptyKey || old == m) return (int)h;
+    }
+    h = (h + 1) & (uint32_t)(H - 1);
+  }
+  return -1;   // table full
+}
+
+__global__ __launch_bounds__(256) void k_hubp_plan(EvalArgs a, HubParts H) {
+  __shared__ uint32_t lds[256 / kWave];
+  uint32_t ca = 0, cp = 0, cb = 0;
+  if (threadIdx.x == 0) {
+    H.A[0] = 0;
+    H.PB[0] = 0;
+    H.AB[0] = 0;
+  }
+  for (int base = 0; base < a.list_n; base += 256) {
+    const int i = base + threadIdx.x;
+    uint32_t ac = 0, pc = 0, dc = 0;
+    if (i < a.list_n) {
+      const uint32_t u = a.bucket[a.list[i]];
+      const uint32_t deg = (a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u]);
+      ac = (deg + kHubArcChunk - 1) / kHubArcChunk;
+      pc = (deg + kHubPart - 1) / kHubPart;
+      dc = deg;
+    }
+    uint32_t ta, tp, tb;
+    const uint32_t ia = block_incl_scan(ac, lds, &ta);
+    const uint32_t ip = block_incl_scan(pc, lds, &tp);
+    const uint32_t ib = block_incl_scan(dc, lds, &tb);
+    if (i < a.list_n) {
+      H.A[i + 1] = ca + ia;
+      H.PB[i + 1] = cp + ip;
+      H.AB[i + 1] = cb + ib;
+    }
+    ca += ta;
+    cp += tp;
+    cb += tb;
+  }
+  if (threadIdx.x == 0) {
+    H.tot[0] = ca;
+    H.tot[1] = cp;
+    H.tot[2] = cb;
+  }
+}
+
+struct HubChunk {
+  int i;              // hub (position in the batch's hub list)
+  uint32_t u, o;      // unit and its module
+  uint32_t ob, od, ib, j0, j1, np;
+};
+
+__device__ inline HubChunk hub_chunk(const EvalArgs &a, const HubParts &H, uint32_t c) {
+  HubChunk h;
+  h.i = chunk_owner(H.A, a.list_n, c);
+  h.u = a.bucket[a.list[h.i]];
+  h.o = a.mod[h.u];
+  h.ob = a.optr[h.u];
+  h.od = a.optr[h.u + 1] - h.ob;
+  h.ib = a.iptr[h.u];
+  const uint32_t deg = h.od + (a.iptr[h.u + 1] - h.ib);
+  h.j0 = (c - H.A[h.i]) * kHubArcChunk;
+  h.j1 = min(deg, h.j0 + kHubArcChunk);
+  h.np = H.PB[h.i + 1] - H.PB[h.i];
+  return h;
+}
+
+// count pass: modules of the chunk's arcs, partition sizes, own-module flows
+__global__ __launch_bounds__(256) void k_hubp_count(EvalArgs a, HubParts H) {
+  __shared__ uint32_t hist[kHubHist];
+  __shared__ long long red[2][256 / kWave];
+  const uint32_t total = H.tot[0];
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const HubChunk h = hub_chunk(a, H, c);
+    const bool local = h.np <= (uint32_t)kHubHist;
+    if (local)
+      for (uint32_t b = threadIdx.x; b < h.np; b += 256) hist[b] = 0;
+    __syncthreads();
+    long long so = 0, si = 0;
+    uint32_t run_q = 0xffffffffu, run_n = 0;   // runs of one partition: one counter update each
+    for (uint32_t j = h.j0 + threadIdx.x;; j += 256) {
+      const bool more = j < h.j1;
+      uint32_t q = 0xffffffffu;
+      if (more) {
+        const bool out = j < h.od;
+        const uint32_t k = out ? h.ob + j : h.ib + (j - h.od);
+        const uint32_t m = a.mod[out ? a.oidx[k] : a.iidx[k]];
+        H.mtmp[H.AB[h.i] + j] = out ? m : (m | kInBit);
+        if (m == h.o) {
+          if (out) so += a.aout.fxw(k);
+          else si += a.ain.fxw(k);
+        }
+        q = part_of(m, h.np);
+      }
+      if (q != run_q) {
+        if (run_n) {
+          if (local) atomicAdd(&hist[run_q], run_n);
+          else atomicAdd(&H.bcnt[H.PB[h.i] + run_q], run_n);
+        }
+        run_q = q;
+        run_n = 0;
+      }
+      if (!more) break;
+      ++run_n;
+    }
+    so = wave_sum(so);
+    si = wave_sum(si);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      red[0][threadIdx.x / kWave] = so;
+      red[1][threadIdx.x / kWave] = si;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      long long to = 0, ti = 0;
+      for (int q = 0; q < 256 / kWave; ++q) {
+        to += red[0][q];
+        ti += red[1][q];
+      }
+      if (to) atomicAdd((unsigned long long *)&H.own[2 * h.i], (unsigned long long)to);
+      if (ti) atomicAdd((unsigned long long *)&H.own[2 * h.i + 1], (unsigned long long)ti);
+    }
+    if (local)
+      for (uint32_t b = threadIdx.x; b < h.np; b += 256)
+        if (hist[b]) atomicAdd(&H.bcnt[H.PB[h.i] + b], hist[b]);
+    __syncthreads();
+  }
+}
+
+// scatter pass: every arc to its partition (order inside a partition is
+// irrelevant: all sums are exact integers)
+__global__ __launch_bounds__(256) void k_hubp_scatter(EvalArgs a, HubParts H) {
+  constexpr int PER = kHubArcChunk / 256;
+  __shared__ uint32_t hcnt[kHubHist];
+  __shared__ uint32_t hbase[kHubHist];
+  const uint32_t total = H.tot[0];
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const HubChunk h = hub_chunk(a, H, c);
+    const bool local = h.np <= (uint32_t)kHubHist;
+    if (local)
+      for (uint32_t b = threadIdx.x; b < h.np; b += 256) hcnt[b] = 0;
+    __syncthreads();
+    uint32_t part[PER], rank[PER], key[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const uint32_t j = h.j0 + (uint32_t)(q * 256) + threadIdx.x;
+      part[q] = 0;
+      rank[q] = 0;
+      key[q] = 0;
+      if (j < h.j1) {
+        key[q] = H.mtmp[H.AB[h.i] + j];
+        part[q] = part_of(key[q] & ~kInBit, h.np);
+        rank[q] = local ? atomicAdd(&hcnt[part[q]], 1u) : atomicAdd(&H.bcnt[H.PB[h.i] + part[q]], 1u);
+      }
+    }
+    __syncthreads();
+    if (local)
+      for (uint32_t b = threadIdx.x; b < h.np; b += 256)
+        if (hcnt[b]) hbase[b] = atomicAdd(&H.bcnt[H.PB[h.i] + b], hcnt[b]);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const uint32_t j = h.j0 + (uint32_t)(q * 256) + threadIdx.x;
+      if (j < h.j1) {
+        const bool out = j < h.od;
+        const uint32_t k = out ? h.ob + j : h.ib + (j - h.od);
+        const uint32_t pos = H.boff[H.PB[h.i] + part[q]] + (local ? hbase[part[q]] : 0u) + rank[q];
+        H.ekey[pos] = key[q];
+        H.ew[pos] = out ? a.aout.fxw(k) : a.ain.fxw(k);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// one workgroup per partition: LDS aggregation, per-wave in-place compaction
+// (as in k_eval_block) and dense evaluation; best per partition
+__global__ __launch_bounds__(256) void k_hubp_eval(EvalArgs a, HubParts H) {
+  constexpr int NW = 256 / kWave;
+  __shared__ __align__(16) uint32_t s_keys[kHubTable];
+  __shared__ long long s_out[kHubTable];
+  __shared__ long long s_in[kHubTable];
+  const int w = threadIdx.x / kWave;
+  const int wl = threadIdx.x & (kWave - 1);
+  unsigned long long cand_acc = 0;   // thread 0
+  const uint32_t total = H.tot[1];
+  for (uint32_t p = blockIdx.x; p < total; p += gridDim.x) {
+    const int i = chunk_owner(H.PB, a.list_n, p);
+    const uint32_t u = a.bucket[a.list[i]];
+    const uint32_t e0 = H.boff[p], e1 = H.boff[p + 1];
+    for (int sl = threadIdx.x; sl < kHubTable; sl += 256) {
+      s_keys[sl] = kEmptyKey;
+      s_out[sl] = 0;
+      s_in[sl] = 0;
+    }
+    __syncthreads();
+    // A hub's arcs concentrate on few modules (its own community above all),
+    // so each lane first sums runs of equal modules in registers: one LDS
+    // update per run instead of one per entry on a contended slot.
+    uint32_t run_m = kEmptyKey;
+    long long run_o = 0, run_i = 0;
+    for (uint32_t e = e0 + threadIdx.x;; e += 256) {
+      const bool more = e < e1;
+      const uint32_t key = more ? H.ekey[e] : kEmptyKey;
+      const uint32_t m = key & ~kInBit;
+      if (!more || m != run_m) {
+        if (run_m != kEmptyKey) {
+          const int sl = lds_insert_bounded(s_keys, kHubTable, run_m);
+          if (sl >= 0) {
+            if (run_o) atomicAdd((unsigned long long *)&s_out[sl], (unsigned long long)run_o);
+            if (run_i) atomicAdd((unsigned long long *)&s_in[sl], (unsigned long long)run_i);
+          }
+        }
+        if (!more) break;
+        run_m = m;
+        run_o = 0;
+        run_i = 0;
+      }
+      if (key & kInBit) run_i += H.ew[e];
+      else run_o += H.ew[e];
+    }
+    __syncthreads();   // a failed insert leaves every slot occupied: detected below from the count
+    int nw = 0;
+    for (int s0 = 0; s0 < kHubTable; s0 += 256) {
+      const int sl = s0 + w * kWave + wl;
+      const uint32_t key = s_keys[sl];
+      const long long vo = s_out[sl], vi = s_in[sl];
+      const bool ne = key != kEmptyKey;
+      const uint64_t bal = __ballot(ne);
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (ne) {
+        const int d = wave_slot(nw + (int)__popcll(bal & lanes_below()), w);
+        s_keys[d] = key;
+        s_out[d] = vo;
+        s_in[d] = vi;
+      }
+      nw += (int)__popcll(bal);
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    UnitCtx U;
+    unit_ctx(a, u, H.own[2 * i], H.own[2 * i + 1], U);
+    Best b{0.0, kNone, -1};
+    for (int j = wl; j < nw; j += kWave) {
+      const int sl = wave_slot(j, w);
+      const uint32_t m = s_keys[sl];
+      if (m == U.o || !allowed(a, U, m)) continue;
+      const double dl = delta_for(U, a.ms[m], s_out[sl], s_in[sl]);
+      if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; b.i = sl; }
+    }
+    b = group_best<kWave>(b);
+    __syncthreads();   // keys are not read again: the per-wave bests reuse them
+    double *r_bd = (double *)s_keys;
+    uint32_t *r_bm = s_keys + 2 * NW;
+    int *r_bi = (int *)(s_keys + 3 * NW);
+    unsigned long long *r_nc = (unsigned long long *)(s_keys + 4 * NW);
+    if (wl == 0) {
+      r_bd[w] = b.d;
+      r_bm[w] = b.m;
+      r_bi[w] = b.i;
+      r_nc[w] = (unsigned long long)nw;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      Best bb{r_bd[0], r_bm[0], r_bi[0]};
+      unsigned long long nc = r_nc[0];
+      for (int q = 1; q < NW; ++q) {
+        nc += r_nc[q];
+        if (better(r_bd[q], r_bm[q], bb)) { bb.d = r_bd[q]; bb.m = r_bm[q]; bb.i = r_bi[q]; }
+      }
+      cand_acc += nc;
+      if (nc >= (unsigned long long)kHubTable || H.force_ovf) {
+        // the table filled up (an insert may have failed): the hub goes to the fallback
+        H.ovf[i] = 1u;
+        H.bd[p] = 0.0;
+        H.bm[p] = kNone;
+        atomicAdd(&a.counters[cHubOvf], 1u);
+      } else {
+        H.bd[p] = bb.d;
+        H.bm[p] = bb.m;
+        H.bfo[p] = bb.i >= 0 ? s_out[bb.i] : 0;
+        H.bfi[p] = bb.i >= 0 ? s_in[bb.i] : 0;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && cand_acc) atomicAdd(a.cands, cand_acc);
+}
+
+// one wavefront per hub: best over its partitions (fixed order), empty
+// option, emit; hubs flagged for overflow are left to the fallback
+__global__ __launch_bounds__(256) void k_hubp_finish(EvalArgs a, HubParts H) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t wstride = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t i = w0; i < a.list_n; i += wstride) {
+    if (H.ovf[i]) continue;
+    const uint32_t u = a.bucket[a.list[i]];
+    Best b{0.0, kNone, -1};
+    for (uint32_t p = H.PB[i] + lane; p < H.PB[i + 1]; p += kWave)
+      if (better(H.bd[p], H.bm[p], b)) { b.d = H.bd[p]; b.m = H.bm[p]; b.i = (int)p; }
+    b = group_best<kWave>(b);
+    if (lane == 0) {
+      const long long fo_o = H.own[2 * i], fi_o = H.own[2 * i + 1];
+      UnitCtx U;
+      unit_ctx(a, u, fo_o, fi_o, U);
+      if (U.offer_empty) {
+        const double dl = delta_empty(U);
+        if (better(dl, a.empty_key, b) && dl < 0.0) { b.d = dl; b.m = a.empty_key; b.i = -1; }
+      }
+      const long long fo_n = b.i >= 0 ? H.bfo[b.i] : 0, fi_n = b.i >= 0 ? H.bfi[b.i] : 0;
       emit(a, u, b, fo_o, fi_o, fo_n, fi_n);
       atomicAdd(a.visits, (unsigned long long)((a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u])));
       atomicAdd(a.units, 1u);
@@ -976,6 +1338,7 @@ __global__ void k_hist256(const uint32_t *__restrict__ key, int64_t n, uint32_t 
 __global__ void k_reset_counters(uint32_t *counters) {
   if (threadIdx.x < 2) counters[threadIdx.x] = 0;
   if (threadIdx.x == 2) counters[cBig] = 0;
+  if (threadIdx.x == 3) counters[cHubOvf] = 0;
 }
 
 __global__ void k_empty_flags(const int32_t *__restrict__ mNodes, int32_t n, uint32_t *__restrict__ flag) {
@@ -1091,6 +1454,25 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     if (c.hub_plan.n < 3 * ((int64_t)n_hubs + 1)) c.hub_plan.alloc(3 * ((int64_t)n_hubs + 1));
     if (c.big_list.n < n_hubs) c.big_list.alloc(n_hubs);
     if (c.big_plan.n < (int64_t)n_hubs + 1) c.big_plan.alloc((int64_t)n_hubs + 1);
+    // partitioned hub evaluation: sized for all hubs of the level at once
+    if (c.hp_mtmp.n < L.hub_arcs) {
+      c.hp_mtmp.alloc(L.hub_arcs);
+      c.hp_ekey.alloc(L.hub_arcs);
+      c.hp_ew.alloc(L.hub_arcs);
+    }
+    if (c.hp_bcnt.n < L.hub_parts + 1) {
+      c.hp_bcnt.alloc(L.hub_parts + 1);
+      c.hp_boff.alloc(L.hub_parts + 1);
+      c.hp_bm.alloc(L.hub_parts);
+      c.hp_bd.alloc(L.hub_parts);
+      c.hp_bfo.alloc(L.hub_parts);
+      c.hp_bfi.alloc(L.hub_parts);
+    }
+    if (c.hp_plan.n < 3 * ((int64_t)n_hubs + 1) + 4) c.hp_plan.alloc(3 * ((int64_t)n_hubs + 1) + 4);
+    if (c.hp_ovf.n < n_hubs) {
+      c.hp_ovf.alloc(n_hubs);
+      c.hp_own.alloc(2 * (int64_t)n_hubs);
+    }
   }
 
   // this sweep's units per (bucket, batch), inactive units last: one stable
@@ -1152,6 +1534,8 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
 
   ArcW aout = arcw_out(c, L);
   for (int b = 0; b < c.batches; ++b) {
+    const uint32_t *hub_list = nullptr;   // this batch's hubs, for the overflow fallback
+    int32_t hub_n = 0;
     a.batch = b;
     a.empty_avail = empty_cursor < n_empty ? 1 : 0;
     k_reset_counters<<<1, 64, 0, s>>>(c.counters.p);
@@ -1170,20 +1554,69 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
         case 3: k_eval_group<64, 1024><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a); break;
         case 4: k_eval_block<4096><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a); break;
         default: {
-          HubPlan P{c.hub_plan.p, c.hub_plan.p + (cnt + 1), c.hub_plan.p + 2 * (cnt + 1), c.hub_bd.p, c.hub_bm.p};
-          k_hub_plan<<<1, 256, 0, s>>>(a, P);
-          k_hub_clear<<<4096, 256, 0, s>>>(a, P);
-          k_hub_insert<<<4096, 256, 0, s>>>(a, P);
-          k_hub_eval<<<4096, 256, 0, s>>>(a, P);
-          k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, P);
+          if (c.hub_global == 1) {
+            HubPlan P{c.hub_plan.p, c.hub_plan.p + (cnt + 1), c.hub_plan.p + 2 * (cnt + 1), c.hub_bd.p,
+                      c.hub_bm.p, nullptr};
+            k_hub_plan<<<1, 256, 0, s>>>(a, P);
+            k_hub_clear<<<4096, 256, 0, s>>>(a, P);
+            k_hub_insert<<<4096, 256, 0, s>>>(a, P);
+            k_hub_eval<<<4096, 256, 0, s>>>(a, P);
+            k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, P);
+            break;
+          }
+          const int64_t NB = L.hub_parts;
+          HubParts H;
+          H.A = c.hp_plan.p;
+          H.PB = H.A + (cnt + 1);
+          H.AB = H.PB + (cnt + 1);
+          H.tot = H.AB + (cnt + 1);
+          H.bcnt = c.hp_bcnt.p;
+          H.boff = c.hp_boff.p;
+          H.mtmp = c.hp_mtmp.p;
+          H.ekey = c.hp_ekey.p;
+          H.ew = c.hp_ew.p;
+          H.own = c.hp_own.p;
+          H.ovf = c.hp_ovf.p;
+          H.bd = c.hp_bd.p;
+          H.bm = c.hp_bm.p;
+          H.bfo = c.hp_bfo.p;
+          H.bfi = c.hp_bfi.p;
+          H.force_ovf = c.hub_global == 2 ? 1 : 0;
+          k_hubp_plan<<<1, 256, 0, s>>>(a, H);
+          fill_u32(H.bcnt, 0u, NB + 1, s);
+          fill_i64(H.own, 0, 2 * (int64_t)cnt, s);
+          fill_u32(H.ovf, 0u, cnt, s);
+          k_hubp_count<<<4096, 256, 0, s>>>(a, H);
+          exclusive_scan_u32(H.bcnt, H.boff, NB + 1, nullptr, c.scratch, s);
+          fill_u32(H.bcnt, 0u, NB + 1, s);
+          k_hubp_scatter<<<4096, 256, 0, s>>>(a, H);
+          k_hubp_eval<<<8192, 256, 0, s>>>(a, H);
+          k_hubp_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, H);
+          hub_list = a.list;
+          hub_n = cnt;
           break;
         }
       }
       RMX_LAUNCH_CHECK();
       if (c.prof.on) profile_eval(c, bk, s);
     }
-    RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 2, hipMemcpyDeviceToHost, s));
+    RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
     c.sync();
+    if (hub_n > 0 && c.h_cnt[cHubOvf] > 0) {
+      // a partition table overflowed: those hubs again with global tables
+      a.list = hub_list;
+      a.list_n = hub_n;
+      HubPlan P{c.hub_plan.p, c.hub_plan.p + (hub_n + 1), c.hub_plan.p + 2 * (hub_n + 1), c.hub_bd.p, c.hub_bm.p,
+                c.hp_ovf.p};
+      k_hub_plan<<<1, 256, 0, s>>>(a, P);
+      k_hub_clear<<<4096, 256, 0, s>>>(a, P);
+      k_hub_insert<<<4096, 256, 0, s>>>(a, P);
+      k_hub_eval<<<4096, 256, 0, s>>>(a, P);
+      k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (hub_n + 3) / 4), 256, 0, s>>>(a, P);
+      RMX_LAUNCH_CHECK();
+      RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
+      c.sync();
+    }
     const uint32_t n_movers = c.h_cnt[cMovers], n_want = c.h_cnt[cWant];
     if (n_movers == 0) continue;
     ++c.stamp_now;
