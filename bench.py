@@ -29,6 +29,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "move-sweep GTEPS + time-to-converged codelength, mixed vs FP64, 500M edges"
 HBM_PEAK_GBS = 8000.0
+DEFAULT_NODES = 32 * 1024 * 1024
+DEFAULT_MEAN_DEG = 18.5   # calibrated so the generated network has ~529M arcs
 
 
 def log(msg):
@@ -144,18 +146,46 @@ def gpu_on_sample(args):
     return out
 
 
-def roofline_of(stats):
+TRAFFIC = os.path.join(ROOT, "profiles", "sweep_flow_kernels_chunglu32m_traffic.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` on the 32M-node workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated by a plain
+    copy of known size), or None."""
+    try:
+        lines = open(TRAFFIC).read().splitlines()
+    except OSError:
+        return None
+    ff = wf = None
+    for ln in lines:
+        if ln.startswith("fetch_factor"):
+            ff = float(ln.split()[1])
+        elif ln.startswith("write_factor"):
+            wf = float(ln.split()[1])
+        elif not ln.startswith("#"):
+            parts = ln.split("\t")
+            if len(parts) >= 4 and parts[0] == kernel and ff and wf:
+                return float(parts[2]) * 1024 * ff + float(parts[3]) * 1024 * wf
+    return None
+
+
+def roofline_of(stats, default_workload):
     if not stats:
         return None, {}
     table = {k: {"launches": n, "avg_ms": ms / max(n, 1), "total_ms": ms,
                  "achieved_GBs": (b / max(n, 1)) / (ms / max(n, 1) * 1e-3) / 1e9 if ms > 0 else 0.0}
              for k, (n, ms, b) in stats.items()}
-    name = max(stats, key=lambda k: stats[k][1])
+    single = [k for k in stats if not k.startswith("hub sequence")]   # one HIP kernel per entry
+    name = max(single, key=lambda k: stats[k][1])
     n, ms, b = stats[name]
     achieved = (b / n) / (ms / n * 1e-3) / 1e9
+    traffic = measured_traffic(name) if default_workload else None
     roof = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_launch": b / n, "avg_launch_ms": ms / n, "launches": n}
+    if traffic:
+        roof["traffic_over_algorithmic"] = round(traffic / (b / n), 2)
     return roof, table
 
 
@@ -164,8 +194,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nodes", type=int, default=32 * 1024 * 1024)
-    ap.add_argument("--mean-deg", type=float, default=18.5)
+    ap.add_argument("--nodes", type=int, default=DEFAULT_NODES)
+    ap.add_argument("--mean-deg", type=float, default=DEFAULT_MEAN_DEG)
     ap.add_argument("--batches", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -228,7 +258,7 @@ def main():
             out["nmi_planted"] = {"mixed": nmi(part_m, g.community), "fp64": nmi(part_d, g.community)}
     if not args.no_profile:
         stats = profile_pass(g, "mixed", args)
-        roof, table = roofline_of(stats)
+        roof, table = roofline_of(stats, args.nodes == DEFAULT_NODES and args.mean_deg == DEFAULT_MEAN_DEG)
         out["roofline"] = roof
         out["kernels"] = {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in
                           sorted(table.items(), key=lambda kv: -kv[1]["total_ms"])[:12]}

@@ -320,7 +320,9 @@ struct RowBins {
 // and the written result (4 + 8 + 8 + 8).
 template <int G, bool MIXED>
 void launch_spmv_bin(int bin, const RowBins &rb, const Level &L, Ctx &c, double *y) {
-  static const char *names[kNumRowBins] = {"spmv_g4", "spmv_g8", "spmv_g16", "spmv_g32", "spmv_g64", "spmv_block"};
+  // profiling entries carry the kernels' own names
+  static const char *names[kNumRowBins] = {"k_spmv<4, ", "k_spmv<8, ", "k_spmv<16, ", "k_spmv<32, ", "k_spmv<64, ",
+                                           "k_spmv_block<"};
   const int32_t cnt = rb.off[bin + 1] - rb.off[bin];
   if (cnt <= 0) return;
   const uint32_t *rows = rb.rows.p + rb.off[bin];
@@ -337,7 +339,7 @@ void launch_spmv_bin(int bin, const RowBins &rb, const Level &L, Ctx &c, double 
   RMX_LAUNCH_CHECK();
   if (c.prof.on) {
     const double per_arc = MIXED ? 12.0 : 20.0;
-    c.prof.end(c.stream, std::string(names[bin]) + (MIXED ? "_fp32" : "_fp64"),
+    c.prof.end(c.stream, std::string(names[bin]) + (MIXED ? "true>" : "false>"),
                (double)rb.arcs[bin] * per_arc + (double)cnt * 28.0);
   }
 }

@@ -1364,8 +1364,11 @@ unsigned stride_grid(int64_t items, int per_block) {
 // its weight and its module (4 + w + 4 bytes); per unit its statistics, module
 // and CSR bounds (48 bytes); per candidate module its four statistics (32).
 static void profile_eval(Ctx &c, int bucket, hipStream_t s) {
-  static const char *names[kNumBuckets] = {"eval_g8_h32", "eval_g16_h64", "eval_g32_h256", "eval_g64_h1024",
-                                           "eval_block_h4096", "eval_block_hub"};
+  // entries carry the kernels' own names (both levels together); the hub
+  // bucket is a sequence of kernels and is marked as such
+  static const char *names[kNumBuckets] = {"k_eval_group<8, 32>", "k_eval_group<16, 64>", "k_eval_group<32, 256>",
+                                           "k_eval_group<64, 1024>", "k_eval_block<4096>",
+                                           "hub sequence: k_hubp_plan..k_hubp_finish"};
   const float ms = c.prof.stop(s);   // end event right after the launch
   uint32_t before[16];
   std::memcpy(before, c.h_cnt, sizeof(before));
@@ -1376,7 +1379,7 @@ static void profile_eval(Ctx &c, int bucket, hipStream_t s) {
   const double cands = u64(c.h_cnt, cCandsLo) - u64(before, cCandsLo);
   const double units = (double)c.h_cnt[cUnits] - (double)before[cUnits];
   const double wbytes = (c.level == 0 && c.precision == 1) ? 4.0 : 8.0;
-  KernelStat &k = c.prof.get(std::string(names[bucket]) + (c.level == 0 ? "_nodes" : "_super"));
+  KernelStat &k = c.prof.get(names[bucket]);
   k.launches += 1;
   k.ms += ms;
   k.bytes += visits * (8.0 + wbytes) + units * 48.0 + cands * 32.0;
