@@ -104,7 +104,10 @@ int rmx_move_sweep(rmx_ctx *ctx, int prioritized, double vthresh, uint64_t seed,
 /* Network::updateMembersInModule (Module.cpp:2881-2902). */
 int rmx_update_members(rmx_ctx *ctx);
 /* Network::convertModulesToSuperNodes (Module.cpp:3008-3077): contracts the
- * original arcs by the current node modules; later sweeps move super nodes. */
+ * current level by the current modules (at the node level the original arcs,
+ * as the reference does; after a contraction its super arcs, which gives the
+ * same exact fixed-point sums and FP64 weights equal up to rounding); later
+ * sweeps move the super nodes. */
 int rmx_contract(rmx_ctx *ctx, int32_t *n_super, int64_t *m_super);
 /* Super-node statistics after rmx_contract (SuperNode(Module, int),
  * Node.cpp:76-106): flow, exit, teleport, dangling flow, module index and

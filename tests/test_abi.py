@@ -57,3 +57,19 @@ def test_cli_usage():
         pytest.skip("CLI not built")
     r = subprocess.run([exe], capture_output=True, text=True)
     assert "Call: ./relaxmap <seed>" in r.stdout
+
+
+def test_integration_binding_compiles(tmp_path):
+    """The C++ binding shown in INTEGRATION.md compiles against the header."""
+    import shutil
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```cpp\n(.*?)```", text, re.S)
+    assert blocks, "no C++ binding in INTEGRATION.md"
+    src = tmp_path / "gpu_network.cpp"
+    src.write_text(blocks[0] + "\nint main() { GpuNetwork *p = nullptr; (void)p; return 0; }\n")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                        str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
