@@ -120,6 +120,22 @@ int rmx_get_super_arcs(rmx_ctx *ctx, int64_t *out_ptr, int32_t *out_idx, double 
 /* Back to moving original nodes (node modules follow their super nodes). */
 int rmx_to_node_level(rmx_ctx *ctx);
 
+/* generate_sub_modules (OmpRelaxmap.cpp:500-632): every module's induced
+ * subgraph (generate_network_from_module, OmpRelaxmap.cpp:640-783) is
+ * partitioned from singletons as its own network, with its own codelength
+ * (partition_module_network, OmpRelaxmap.cpp:436-498: sweeps until one gains
+ * less than `threshold` bits, at most max_iter).  All modules are solved
+ * together on the GPU.  *n_sub is the number of sub-modules. */
+int rmx_generate_sub_modules(rmx_ctx *ctx, double threshold, int32_t max_iter, uint64_t seed, int32_t *n_sub);
+/* Sub-module label per node after rmx_generate_sub_modules (a node id of the
+ * same parent module). */
+int rmx_get_sub_modules(rmx_ctx *ctx, int32_t *sub);
+/* Network::generateSuperNodesFromSubModules (Module.cpp:3083-3158): every
+ * sub-module becomes a super node in its parent module; later sweeps move
+ * them (the coarse tune).  The super-node exit is the consistent
+ * (alpha P + beta D)(1 - T) + beta X, not the reference's per-member sum. */
+int rmx_contract_sub_modules(rmx_ctx *ctx, int32_t *n_super, int64_t *m_super);
+
 /* The CLI's optimisation: main()'s super-step loop (OmpRelaxmap.cpp:181-219)
  * around stochastic_greedy_partition (OmpRelaxmap.cpp:263-429). */
 typedef struct {
@@ -141,6 +157,7 @@ typedef struct {
   double time_partition_s;    /* sweeps + contraction (OmpRelaxmap.cpp:222) */
   double time_sweeps_s;
   double time_contract_s;
+  double time_coarse_s;       /* sub-module partitions of the coarse tunes */
 } rmx_partition_result;
 
 int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_partition_result *res);

@@ -36,7 +36,7 @@ class _Result(C.Structure):
     _fields_ = [("codelength", C.c_double), ("computed_codelength", C.c_double), ("n_module", C.c_int32),
                 ("super_steps", C.c_int32), ("sweeps", C.c_int64), ("edge_visits", C.c_int64),
                 ("time_partition_s", C.c_double), ("time_sweeps_s", C.c_double),
-                ("time_contract_s", C.c_double)]
+                ("time_contract_s", C.c_double), ("time_coarse_s", C.c_double)]
 
 
 @dataclass
@@ -50,6 +50,7 @@ class PartitionResult:
     time_partition_s: float
     time_sweeps_s: float
     time_contract_s: float
+    time_coarse_s: float
 
 
 def load_library():
@@ -70,6 +71,9 @@ def load_library():
         "rmx_set_precision": (C.c_int, [vp, C.c_int]),
         "rmx_set_batches": (C.c_int, [vp, C.c_int]),
         "rmx_set_hub_path": (C.c_int, [vp, C.c_int]),
+        "rmx_generate_sub_modules": (C.c_int, [vp, C.c_double, C.c_int32, C.c_uint64, _I32P]),
+        "rmx_get_sub_modules": (C.c_int, [vp, _I32P]),
+        "rmx_contract_sub_modules": (C.c_int, [vp, _I32P, _I64P]),
         "rmx_graph_upload": (C.c_int, [vp, C.c_int32, C.c_int64, _I64P, _I32P, _F64P, _F64P]),
         "rmx_load_network": (C.c_int, [vp, C.c_char_p, _I32P, _I64P, _I32P, _I32P]),
         "rmx_initiate": (C.c_int, [vp, _I32P, _F64P, _I64P]),
@@ -247,6 +251,27 @@ class Network:
         self.n_super, self.m_super = ns.value, ms.value
         return self.n_super, self.m_super
 
+    def generate_sub_modules(self, threshold=1e-3, max_iter=10, seed=1):
+        """generate_sub_modules (OmpRelaxmap.cpp:500-632): every module's induced
+        subgraph partitioned as its own network; returns the sub-module count."""
+        k = C.c_int32()
+        self._check(self._L.rmx_generate_sub_modules(self._h, float(threshold), int(max_iter), int(seed),
+                                                     C.byref(k)))
+        return k.value
+
+    def sub_modules(self):
+        """Sub-module label per node after generate_sub_modules."""
+        a = np.empty(self.n, dtype=np.int32)
+        self._check(self._L.rmx_get_sub_modules(self._h, _p(a, _I32P)))
+        return a
+
+    def generate_super_nodes_from_sub_modules(self):
+        """Network::generateSuperNodesFromSubModules (Module.cpp:3083-3158)."""
+        ns, ms = C.c_int32(), C.c_int64()
+        self._check(self._L.rmx_contract_sub_modules(self._h, C.byref(ns), C.byref(ms)))
+        self.n_super, self.m_super = ns.value, ms.value
+        return self.n_super, self.m_super
+
     def super_nodes(self):
         s, m = self.n_super, self.m_super
         size, ex, tp, dg = (np.empty(s) for _ in range(4))
@@ -276,7 +301,8 @@ class Network:
         r = _Result()
         self._check(self._L.rmx_partition(self._h, C.byref(o), C.byref(r)))
         return PartitionResult(r.codelength, r.computed_codelength, r.n_module, r.super_steps, r.sweeps,
-                               r.edge_visits, r.time_partition_s, r.time_sweeps_s, r.time_contract_s)
+                               r.edge_visits, r.time_partition_s, r.time_sweeps_s, r.time_contract_s,
+                               r.time_coarse_s)
 
     def write_tree(self, path):
         self._check(self._L.rmx_write_tree(self._h, str(path).encode()))

@@ -54,6 +54,7 @@ void free_ctx_device(Ctx &c) {
   c.L0.free_all();
   c.L1.free_all();
   c.L2.free_all();
+  c.Lsub.free_all();
   c.node_w.free(); c.raw_w.free(); c.dangling.free(); c.in_of_out.free(); c.node_sp.free();
   c.ms.free(); c.mNodes.free(); c.mUnits.free(); c.mStale.free();
   c.mInst.free();
@@ -386,7 +387,7 @@ struct Driver {
   const rmx_partition_opts &o;
   rmx_partition_result &r;
   double t0;
-  double t_sweeps = 0, t_contract = 0;
+  double t_sweeps = 0, t_contract = 0, t_coarse = 0;
   uint64_t sweep_no = 0;
   const double ln2 = std::log(2.0);
 
@@ -415,10 +416,12 @@ struct Driver {
     const double thr = o.threshold;
     if (fine) to_node_level(c);
     else {
-      // coarse tune: move the current modules as units
+      // coarse tune (main(), OmpRelaxmap.cpp:212-217): the sub-modules of
+      // every module become the moving units, starting in their modules
       double a = now_s();
-      contract(c);
-      t_contract += now_s() - a;
+      generate_sub_modules(c, thr, o.max_iter, o.seed ^ (0x9e3779b97f4a7c15ull * ++sweep_no));
+      contract_sub_modules(c);
+      t_coarse += now_s() - a;
     }
     reset_active(c);
     if (o.verbose) std::printf("%d, ", c.cur().n);
@@ -470,6 +473,42 @@ void rmx::to_node_level(Ctx &c) {
   RMX_HIP(hipMemcpyAsync(c.mUnits.p, c.mNodes.p, sizeof(int32_t) * c.N, hipMemcpyDeviceToDevice, c.stream));
 }
 
+extern "C" int rmx_generate_sub_modules(rmx_ctx *ctx, double threshold, int32_t max_iter, uint64_t seed,
+                                        int32_t *n_sub) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    if (max_iter < 0) return fail(ctx, RMX_EINVAL, "max_iter < 0");
+    RMX_HIP(hipSetDevice(c.device));
+    const int32_t k = rmx::generate_sub_modules(c, threshold, max_iter, seed);
+    if (n_sub) *n_sub = k;
+    return RMX_OK;
+  });
+}
+
+extern "C" int rmx_get_sub_modules(rmx_ctx *ctx, int32_t *sub) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (c.n_sub <= 0) return fail(ctx, RMX_ESTATE, "rmx_generate_sub_modules first");
+    if (!sub) return fail(ctx, RMX_EINVAL, "null output");
+    RMX_HIP(hipMemcpyAsync(sub, c.Lsub.mod.p, sizeof(uint32_t) * c.N, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    return RMX_OK;
+  });
+}
+
+extern "C" int rmx_contract_sub_modules(rmx_ctx *ctx, int32_t *n_super, int64_t *m_super) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (c.n_sub <= 0) return fail(ctx, RMX_ESTATE, "rmx_generate_sub_modules first");
+    RMX_HIP(hipSetDevice(c.device));
+    rmx::contract_sub_modules(c);
+    if (n_super) *n_super = c.L1.n;
+    if (m_super) *m_super = c.L1.m;
+    return RMX_OK;
+  });
+}
+
 extern "C" int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_partition_result *res) {
   return guarded(ctx, [&] {
     Ctx &c = ctx->c;
@@ -502,6 +541,7 @@ extern "C" int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_p
     res->time_partition_s = now_s() - t_start;
     res->time_sweeps_s = d.t_sweeps;
     res->time_contract_s = d.t_contract;
+    res->time_coarse_s = d.t_coarse;
     res->codelength = c.codelength;
     res->n_module = c.n_module;
     res->super_steps = step;

@@ -217,6 +217,19 @@ struct Ctx {
   DevBuf<double> partial;            // block partials for reductions
   uint32_t stamp_now = 0;
 
+  // coarse tune (sub-module partition of every module as its own instance):
+  // the intra-module level, sub-module statistics, per-instance totals
+  int sub_mode = 0;                  // sweeps run on the sub-module instances
+  Level Lsub;
+  DevBuf<Stat4> sub_ms;
+  DevBuf<int32_t> sub_nodes, sub_units, sub_stale;
+  DevBuf<uint32_t> sub_par;          // instance (parent module) of every node
+  DevBuf<uint32_t> inst_nodes, inst_start;   // nodes grouped by instance, segment starts
+  int32_t n_inst = 0;                // instances (non-empty parent modules)
+  DevBuf<double> inst_S, inst_L, inst_Lprev, inst_C;   // per parent module id
+  DevBuf<uint8_t> inst_act;
+  int32_t n_sub = 0;                 // sub-modules after the last partition
+
   // steady-state buffers
   DevBuf<double> x, y;
   DevBuf<float> xf;
@@ -269,6 +282,14 @@ void ensure_sweep_buffers(Ctx &c);
 int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int64_t *edge_visits);
 void reset_active(Ctx &c);
 int64_t n_active(Ctx &c);
+
+// ----- coarse.hip
+// generate_sub_modules (OmpRelaxmap.cpp:500-632): every module's induced
+// subgraph is partitioned as its own network; returns the sub-module count.
+int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t seed);
+// Network::generateSuperNodesFromSubModules (Module.cpp:3083-3158)
+void contract_sub_modules(Ctx &c);
+void instance_totals_async(Ctx &c);   // per-instance S and L of the sub-module partition
 
 // ----- contract.hip
 void update_members(Ctx &c);

@@ -62,6 +62,8 @@ struct EvalArgs {
   int empty_avail;
   uint32_t empty_key;
   const double *scal;
+  const double *S_inst;      // per-instance S (sub-module partition), indexed by inst[u]; null: scal[kS]
+  const uint32_t *inst;
   double vthresh;
   uint32_t *target;
   long long *agg;        // per unit: flows to/from the old and the new module
@@ -130,7 +132,7 @@ __device__ inline void unit_ctx(const EvalArgs &a, uint32_t u, long long fo_o, l
   const double outo = kBeta * fx_to(fo_o) + U.tele_out * fx_to(so.t - su.t);
   const double ino = kBeta * fx_to(fi_o) + (kAlpha * fx_to(so.p - su.p) + kBeta * fx_to(so.d - su.d)) * U.td;
   const double q1 = qo - U.qu + outo + ino;
-  const double S = a.scal[kS];
+  const double S = a.S_inst ? a.S_inst[a.inst[u]] : a.scal[kS];
   U.C = S + q1 - qo;
   U.K = -plogp(S) - 2.0 * plogp(q1) + 2.0 * plogp(qo) + plogp(q1 + fx_to(so.p - su.p)) - plogp(qo + fx_to(so.p));
   const int32_t units_o = a.mUnits[U.o];
@@ -1421,9 +1423,11 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   const int32_t n = L.n;
   const uint64_t sweep_seed = mix64(seed ^ mix64(++c.sweep_counter));
 
-  // empty modules available to this sweep, ascending id
+  // empty modules available to this sweep, ascending id (none in a
+  // sub-module partition: its stale member lists never exceed one member,
+  // so the reference never offers the option there)
   uint32_t n_empty = 0;
-  {
+  if (!c.sub_mode) {
     k_empty_flags<<<grid_for(c.N, 256), 256, 0, s>>>(c.mNodes.p, c.N, c.rank_tmp.p);
     RMX_LAUNCH_CHECK();
     exclusive_scan_u32(c.rank_tmp.p, c.target.p, c.N, c.counters.p + 8, c.scratch, s);
@@ -1434,7 +1438,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   fill_u32(c.counters.p + cUnits, 0u, 1, s);
   RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
   c.sync();
-  n_empty = c.h_cnt[8];
+  n_empty = c.sub_mode ? 0u : c.h_cnt[8];
   uint32_t empty_cursor = 0;
   if (c.prof.on) {   // h_cnt mirrors the counters from here on (profiling deltas)
     RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
@@ -1521,6 +1525,8 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   a.node_level = c.level == 0 ? 1 : 0;
   a.empty_key = (uint32_t)c.N + 1u;
   a.scal = c.scal.p;
+  a.S_inst = c.sub_mode ? c.inst_S.p : nullptr;
+  a.inst = c.sub_mode ? c.sub_par.p : nullptr;
   a.vthresh = prioritized ? vthresh : 0.0;
   a.target = c.target.p;
   a.agg = c.agg_buf.p;
@@ -1650,14 +1656,20 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     k_apply<<<stride_grid(n_movers, 256), 256, 0, s>>>(c.movers.p, c.counters.p, c.newmod.p, c.stamp.p,
                                                         c.stamp_now, L.mod.p);
     RMX_LAUNCH_CHECK();
-    module_totals_async(c);   // S for the next batch
+    if (c.sub_mode) instance_totals_async(c);   // S of every instance for the next batch
+    else module_totals_async(c);               // S for the next batch
   }
   if (prioritized) {
     c.act.swap(c.act_next);
     fill_u8(c.act_next.p, 0, c.N, s);
   }
   RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
-  module_totals(c, &c.sum_exit, &c.codelength);
+  if (c.sub_mode) {
+    instance_totals_async(c);
+    c.sync();
+  } else {
+    module_totals(c, &c.sum_exit, &c.codelength);
+  }
   const uint32_t moved = c.h_cnt[cMoved];
   unsigned long long vis = (unsigned long long)c.h_cnt[cVisitsLo] | ((unsigned long long)c.h_cnt[cVisitsLo + 1] << 32);
   if (edge_visits) *edge_visits = (int64_t)vis;
