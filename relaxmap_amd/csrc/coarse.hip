@@ -32,7 +32,7 @@ namespace rmx {
 
 namespace {
 
-// arcs whose ends share a module
+// arcs whose ends share a module (of an out- or an in-CSR)
 template <int G>
 __global__ void k_intra_flags(const uint32_t *__restrict__ optr, const uint32_t *__restrict__ oidx, int32_t n,
                               const uint32_t *__restrict__ par, uint32_t *__restrict__ flag) {
@@ -63,18 +63,6 @@ __global__ void k_intra_copy(const uint32_t *__restrict__ flag, const uint32_t *
     sidx[j] = oidx[k];
     sw[j] = w[k];
     swf[j] = wf[k];
-  }
-}
-
-// in-arc weights of the intra level through the in-CSR positions
-__global__ void k_scatter_w(const uint32_t *__restrict__ in_of_out, int64_t m, const double *__restrict__ ow,
-                            const float *__restrict__ owf, double *__restrict__ iw, float *__restrict__ iwf) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; e < m; e += stride) {
-    const uint32_t k = in_of_out[e];
-    iw[k] = ow[e];
-    iwf[k] = owf[e];
   }
 }
 
@@ -295,12 +283,29 @@ int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t se
     }
     c.sync();
   }
-  {
-    DevBuf<uint32_t> in_of_out(c.cache);
-    build_in_csr(c, S, &in_of_out);
+  {   // in-lists: the node level's in-lists filtered the same way (their
+      // ascending-tail order carries over, so no sort is needed)
+    DevBuf<uint32_t> flag(c.cache), apos(c.cache), tot(c.cache);
+    flag.alloc(m);
+    apos.alloc(m);
+    tot.alloc(1);
+    uint32_t m_in2 = 0;
+    if (m > 0) {
+      k_intra_flags<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L0.in_ptr.p, L0.in_idx.p, n, c.sub_par.p,
+                                                                        flag.p);
+      RMX_LAUNCH_CHECK();
+      exclusive_scan_u32(flag.p, apos.p, m, tot.p, c.scratch, s);
+      RMX_HIP(hipMemcpyAsync(&m_in2, tot.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      c.sync();
+    }
+    if (m_in2 != m_in) throw HipError("intra-module in- and out-arcs disagree");
+    S.in_ptr.alloc((int64_t)n + 1);
+    S.in_idx.alloc(m_in);
+    k_intra_ptr<<<grid_for((int64_t)n + 1, 256), 256, 0, s>>>(L0.in_ptr.p, n, m, apos.p, m_in, S.in_ptr.p);
+    RMX_LAUNCH_CHECK();
     if (m_in > 0) {
-      k_scatter_w<<<(unsigned)std::min<int64_t>(16384, ((int64_t)m_in + 255) / 256), 256, 0, s>>>(
-          in_of_out.p, m_in, S.out_w.p, S.out_wf.p, S.in_w.p, S.in_wf.p);
+      k_intra_copy<<<(unsigned)std::min<int64_t>(16384, (m + 255) / 256), 256, 0, s>>>(
+          flag.p, apos.p, m, L0.in_idx.p, L0.in_w.p, L0.in_wf.p, S.in_idx.p, S.in_w.p, S.in_wf.p);
       RMX_LAUNCH_CHECK();
     }
     c.sync();

@@ -12,6 +12,12 @@
 // with more than 256 in-arcs get a 256-lane workgroup.  Mixed precision
 // streams FP32 weights and an FP32 copy of x, accumulating in FP64; the
 // residual, the normalisation and D stay FP64.
+//
+// The iteration runs on a relabelled copy of the in-CSR: nodes in order of
+// descending out-degree, so the sources read most often share a compact,
+// cache-resident part of x (the gathers of x dominate the traffic).  Every
+// row keeps its arcs in their original order, so each row sum is the same
+// as on the original labels; p is mapped back after the last iteration.
 #include "context.h"
 
 #include <algorithm>
@@ -308,6 +314,66 @@ __global__ void k_singletons(int32_t n, uint32_t *__restrict__ mod, int32_t *__r
   mStale[i] = 1;
 }
 
+// The pull side of the iteration: in-CSR, weights, teleport, dangling flags.
+struct PullCSR {
+  int32_t n = 0;
+  const uint32_t *ptr = nullptr, *idx = nullptr;
+  const double *w = nullptr;
+  const float *wf = nullptr;
+  const double *tau = nullptr;
+  const uint8_t *dangling = nullptr;
+};
+
+__global__ void k_out_degree_key(const uint32_t *__restrict__ optr, int32_t n, uint32_t *__restrict__ key) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) key[i] = ~(optr[i + 1] - optr[i]);   // ascending key = descending degree
+}
+
+__global__ void k_invert(const uint32_t *__restrict__ inv, int32_t n, uint32_t *__restrict__ perm) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) perm[inv[r]] = (uint32_t)r;
+}
+
+// per new row: in-degree, teleport weight and dangling flag of its node
+__global__ void k_perm_rows(const uint32_t *__restrict__ inv, int32_t n, const uint32_t *__restrict__ iptr,
+                            const double *__restrict__ tau, const uint8_t *__restrict__ dang,
+                            uint32_t *__restrict__ deg, double *__restrict__ ptau, uint8_t *__restrict__ pdang) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t v = inv[r];
+  deg[r] = iptr[v + 1] - iptr[v];
+  ptau[r] = tau[v];
+  pdang[r] = dang[v];
+}
+
+// G lanes per new row: the node's in-arcs in their order, tails relabelled
+template <int G, bool MIXED>
+__global__ void k_perm_arcs(const uint32_t *__restrict__ inv, const uint32_t *__restrict__ perm, int32_t n,
+                            const uint32_t *__restrict__ iptr, const uint32_t *__restrict__ iidx,
+                            const double *__restrict__ iw, const float *__restrict__ iwf,
+                            const uint32_t *__restrict__ pptr, uint32_t *__restrict__ pidx, double *__restrict__ pw,
+                            float *__restrict__ pwf) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = t / G;
+  const int lane = (int)(t % G);
+  if (r >= n) return;
+  const uint32_t v = inv[r];
+  const uint32_t b = iptr[v], e = iptr[v + 1], o = pptr[r];
+  for (uint32_t k = b + lane; k < e; k += G) {
+    const uint32_t j = o + (k - b);
+    pidx[j] = perm[iidx[k]];
+    if (MIXED) pwf[j] = iwf[k];
+    else pw[j] = iw[k];
+  }
+}
+
+// p on the original labels
+__global__ void k_unpermute(const double *__restrict__ xp, const uint32_t *__restrict__ perm, int32_t n,
+                            double *__restrict__ x) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) x[v] = xp[perm[v]];
+}
+
 // Row bins of the pull SpMV (in-degree classes), stable in row order.
 struct RowBins {
   DevBuf<uint32_t> rows;
@@ -319,7 +385,7 @@ struct RowBins {
 // and the gathered flow (4 + w + x); per row the row id, its CSR bounds, tau
 // and the written result (4 + 8 + 8 + 8).
 template <int G, bool MIXED>
-void launch_spmv_bin(int bin, const RowBins &rb, const Level &L, Ctx &c, double *y) {
+void launch_spmv_bin(int bin, const RowBins &rb, const PullCSR &L, Ctx &c, double *y) {
   // profiling entries carry the kernels' own names
   static const char *names[kNumRowBins] = {"k_spmv<4, ", "k_spmv<8, ", "k_spmv<16, ", "k_spmv<32, ", "k_spmv<64, ",
                                            "k_spmv_block<"};
@@ -330,11 +396,11 @@ void launch_spmv_bin(int bin, const RowBins &rb, const Level &L, Ctx &c, double 
   if (G > 0) {
     constexpr int GG = G > 0 ? G : 1;
     k_spmv<GG, MIXED><<<grid_for((int64_t)cnt * GG, 256), 256, 0, c.stream>>>(
-        rows, cnt, L.in_ptr.p, L.in_idx.p, L.in_w.p, L.in_wf.p, c.x.p, c.xf.p, L.tau.p, c.scal.p, y);
+        rows, cnt, L.ptr, L.idx, L.w, L.wf, c.x.p, c.xf.p, L.tau, c.scal.p, y);
   } else {
     unsigned g = cnt < 4096 ? (unsigned)cnt : 4096u;
-    k_spmv_block<MIXED><<<g, 256, 0, c.stream>>>(rows, cnt, L.in_ptr.p, L.in_idx.p, L.in_w.p, L.in_wf.p, c.x.p,
-                                                 c.xf.p, L.tau.p, c.scal.p, y);
+    k_spmv_block<MIXED><<<g, 256, 0, c.stream>>>(rows, cnt, L.ptr, L.idx, L.w, L.wf, c.x.p, c.xf.p, L.tau,
+                                                 c.scal.p, y);
   }
   RMX_LAUNCH_CHECK();
   if (c.prof.on) {
@@ -345,7 +411,7 @@ void launch_spmv_bin(int bin, const RowBins &rb, const Level &L, Ctx &c, double 
 }
 
 template <bool MIXED>
-void spmv_all(Ctx &c, const Level &L, const RowBins &rb, double *y) {
+void spmv_all(Ctx &c, const PullCSR &L, const RowBins &rb, double *y) {
   launch_spmv_bin<4, MIXED>(0, rb, L, c, y);
   launch_spmv_bin<8, MIXED>(1, rb, L, c, y);
   launch_spmv_bin<16, MIXED>(2, rb, L, c, y);
@@ -356,13 +422,13 @@ void spmv_all(Ctx &c, const Level &L, const RowBins &rb, double *y) {
 
 }  // namespace
 
-static void make_row_bins(Ctx &c, const Level &L, RowBins &rb) {
+static void make_row_bins(Ctx &c, const PullCSR &L, RowBins &rb) {
   hipStream_t s = c.stream;
   DevBuf<uint32_t> key(c.cache), cnt(c.cache);
   key.alloc(L.n);
   cnt.alloc(32);
   fill_u32(cnt.p, 0u, 32, s);
-  k_row_bin<<<(unsigned)std::min<int64_t>(2048, (L.n + 255) / 256), 256, 0, s>>>(L.in_ptr.p, L.n, key.p, cnt.p,
+  k_row_bin<<<(unsigned)std::min<int64_t>(2048, (L.n + 255) / 256), 256, 0, s>>>(L.ptr, L.n, key.p, cnt.p,
                                                (unsigned long long *)(cnt.p + 16));
   RMX_LAUNCH_CHECK();
   rb.rows.alloc(L.n);
@@ -413,14 +479,60 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   k_normalize<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, c.raw_w.p, c.in_of_out.p, L.out_w.p,
                                                                   L.in_w.p, L.in_wf.p, c.dangling.p);
   RMX_LAUNCH_CHECK();
+  // the relabelled pull copy (descending out-degree), built on the device
+  DevBuf<uint32_t> inv(c.cache), perm(c.cache), pptr(c.cache), pidx(c.cache), pdeg(c.cache);
+  DevBuf<double> pw(c.cache), ptau(c.cache);
+  DevBuf<float> pwf(c.cache);
+  DevBuf<uint8_t> pdang(c.cache);
+  PullCSR F;
+  F.n = n;
+  {
+    inv.alloc(n);
+    perm.alloc(n);
+    pptr.alloc((int64_t)n + 1);
+    pdeg.alloc((int64_t)n + 1);
+    ptau.alloc(n);
+    pdang.alloc(n);
+    pidx.alloc(m);
+    if (mixed) pwf.alloc(m);
+    else pw.alloc(m);
+    DevBuf<uint32_t> key(c.cache);
+    key.alloc(n);
+    k_out_degree_key<<<grid_for(n, 256), 256, 0, s>>>(L.out_ptr.p, n, key.p);
+    RMX_LAUNCH_CHECK();
+    iota_u32(inv.p, n, s);
+    radix_sort_pairs_u32(key.p, inv.p, n, 0, 32, c.scratch, s);   // stable: ties by node id
+    k_invert<<<grid_for(n, 256), 256, 0, s>>>(inv.p, n, perm.p);
+    RMX_LAUNCH_CHECK();
+    fill_u32(pdeg.p + n, 0u, 1, s);
+    k_perm_rows<<<grid_for(n, 256), 256, 0, s>>>(inv.p, n, L.in_ptr.p, L.tau.p, c.dangling.p, pdeg.p, ptau.p,
+                                                 pdang.p);
+    RMX_LAUNCH_CHECK();
+    exclusive_scan_u32(pdeg.p, pptr.p, (int64_t)n + 1, nullptr, c.scratch, s);
+    if (m > 0) {
+      if (mixed)
+        k_perm_arcs<16, true><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(
+            inv.p, perm.p, n, L.in_ptr.p, L.in_idx.p, L.in_w.p, L.in_wf.p, pptr.p, pidx.p, nullptr, pwf.p);
+      else
+        k_perm_arcs<16, false><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(
+            inv.p, perm.p, n, L.in_ptr.p, L.in_idx.p, L.in_w.p, L.in_wf.p, pptr.p, pidx.p, pw.p, nullptr);
+      RMX_LAUNCH_CHECK();
+    }
+    F.ptr = pptr.p;
+    F.idx = pidx.p;
+    F.w = pw.p;
+    F.wf = pwf.p;
+    F.tau = ptau.p;
+    F.dangling = pdang.p;
+  }
   RowBins rb;
-  make_row_bins(c, L, rb);
+  make_row_bins(c, F, rb);
 
   hipEvent_t e0, e1;
   RMX_HIP(hipEventCreate(&e0));
   RMX_HIP(hipEventCreate(&e1));
   RMX_HIP(hipEventRecord(e0, s));
-  k_init_x<<<kRedGrid, kRedBlock, 0, s>>>(c.x.p, c.xf.p, c.dangling.p, n, part + 2 * kRedGrid, part + kRedGrid);
+  k_init_x<<<kRedGrid, kRedBlock, 0, s>>>(c.x.p, c.xf.p, F.dangling, n, part + 2 * kRedGrid, part + kRedGrid);
   RMX_LAUNCH_CHECK();
   k_final_rd<<<1, kRedBlock, 0, s>>>(part + kRedGrid, part + 2 * kRedGrid, c.scal.p);
   RMX_LAUNCH_CHECK();
@@ -431,12 +543,12 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   // FP64 test then also stops when ten iterations no longer halve it.
   std::vector<double> hist;
   while (it < max_iter) {
-    if (mixed) spmv_all<true>(c, L, rb, c.y.p);
-    else spmv_all<false>(c, L, rb, c.y.p);
+    if (mixed) spmv_all<true>(c, F, rb, c.y.p);
+    else spmv_all<false>(c, F, rb, c.y.p);
     if (c.prof.on) c.prof.begin(s);
     k_sum_partials<<<kRedGrid, kRedBlock, 0, s>>>(c.y.p, n, part);
     RMX_LAUNCH_CHECK();
-    k_norm_step<<<kRedGrid, kRedBlock, 0, s>>>(c.y.p, c.x.p, c.xf.p, c.dangling.p, n, part, part + kRedGrid,
+    k_norm_step<<<kRedGrid, kRedBlock, 0, s>>>(c.y.p, c.x.p, c.xf.p, F.dangling, n, part, part + kRedGrid,
                                                part + 2 * kRedGrid);
     RMX_LAUNCH_CHECK();
     k_final_rd<<<1, kRedBlock, 0, s>>>(part + kRedGrid, part + 2 * kRedGrid, c.scal.p);
@@ -462,6 +574,11 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   RMX_HIP(hipEventDestroy(e1));
   c.iterations = it;
   c.resid = r;
+  // p back on the original labels
+  k_unpermute<<<grid_for(n, 256), 256, 0, s>>>(c.x.p, perm.p, n, c.y.p);
+  RMX_LAUNCH_CHECK();
+  c.x.swap(c.y);
+  c.sync();   // the relabelled copy goes back to the cache
 
   // flows, node statistics and the literal node exits
   k_flows<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, L.out_w.p, c.in_of_out.p, L.in_w.p,
