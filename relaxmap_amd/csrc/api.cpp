@@ -356,7 +356,13 @@ int rmx_get_super_arcs(rmx_ctx *ctx, int64_t *out_ptr, int32_t *out_idx, double 
       if (out_idx) RMX_HIP(hipMemcpyAsync(out_idx, c.L1.out_idx.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, c.stream));
       if (out_w) RMX_HIP(hipMemcpyAsync(out_w, c.L1.out_w.p, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
       if (in_idx) RMX_HIP(hipMemcpyAsync(in_idx, c.L1.in_idx.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, c.stream));
-      if (in_w) RMX_HIP(hipMemcpyAsync(in_w, c.L1.in_w.p, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+      if (in_w) {   // derived from the out-arcs on request (the sweeps never read them)
+        DevBuf<double> tmp(c.cache);
+        tmp.alloc(m);
+        super_in_weights(c, c.L1, tmp.p);
+        RMX_HIP(hipMemcpyAsync(in_w, tmp.p, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+      }
     }
     c.sync();
     for (int32_t i = 0; i <= n; ++i) {

@@ -137,11 +137,6 @@ __global__ void k_super_out_ptr(const uint32_t *__restrict__ start, int64_t n_se
   }
 }
 
-__global__ void k_to_float(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = (float)a[i];
-}
-
 __global__ void k_propagate(const uint32_t *__restrict__ node_sp, const uint32_t *__restrict__ smod, int32_t n,
                             uint32_t *__restrict__ mod) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -278,13 +273,11 @@ void contract(Ctx &c) {
   D.out_idx.alloc(m_super);
   D.out_w.alloc(m_super);
   D.out_wx.alloc(m_super);
-  D.out_wf.alloc(m_super);
+
   if (m_super > 0) {
     k_reduce_segments<<<grid_for(m_super, 256), 256, 0, s>>>(
         kstart.p, m_super, m_inter, keys.p, arcs.p, S.out_w.p, S.out_wf.p, from_super ? S.out_wx.p : nullptr,
         c.precision == 1 ? 1 : 0, bits, D.out_idx.p, D.out_w.p, D.out_wx.p);
-    RMX_LAUNCH_CHECK();
-    k_to_float<<<grid_for(m_super, 256), 256, 0, s>>>(D.out_w.p, D.out_wf.p, m_super);
     RMX_LAUNCH_CHECK();
   }
   D.out_ptr.alloc((int64_t)ns + 1);

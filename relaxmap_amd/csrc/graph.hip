@@ -48,20 +48,37 @@ __global__ void k_arc_src(const uint32_t *__restrict__ ptr, int32_t n, uint32_t 
 }
 
 // after the stable sort, with the arc weights (contracted levels): every
-// in-arc gathers its source and weights from the out-arc it mirrors
+// in-arc gathers its source and its exact fixed-point weight from the out-arc
+// it mirrors (the sweeps read only those on a contracted level; the FP64
+// in-weights are derived on request, see super_in_weights)
 __global__ void k_in_gather(const uint32_t *__restrict__ sorted_e, const uint32_t *__restrict__ src, int64_t m,
-                            const double *__restrict__ ow, const long long *__restrict__ owx,
-                            uint32_t *__restrict__ in_idx, double *__restrict__ iw,
-                            long long *__restrict__ iwx, float *__restrict__ iwf) {
+                            const long long *__restrict__ owx, uint32_t *__restrict__ in_idx,
+                            long long *__restrict__ iwx) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; k < m; k += stride) {
     const uint32_t e = sorted_e[k];
     in_idx[k] = src[e];
-    const double f = ow[e];
-    iw[k] = f;
     iwx[k] = owx[e];
-    iwf[k] = (float)f;   // the FP32 mirror of a contracted arc is its rounded FP64 weight
+  }
+}
+
+// FP64 weight of every in-arc (v <- u): the weight of u's out-arc to v,
+// found by binary search in u's out-list (sorted by head)
+__global__ void k_in_w_lookup(const uint32_t *__restrict__ iptr, const uint32_t *__restrict__ iidx, int32_t n,
+                              const uint32_t *__restrict__ optr, const uint32_t *__restrict__ oidx,
+                              const double *__restrict__ ow, double *__restrict__ iw) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  for (uint32_t k = iptr[v]; k < iptr[v + 1]; ++k) {
+    const uint32_t u = iidx[k];
+    uint32_t lo = optr[u], hi = optr[u + 1];
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (oidx[mid] < (uint32_t)v) lo = mid + 1;
+      else hi = mid;
+    }
+    iw[k] = (lo < optr[u + 1] && oidx[lo] == (uint32_t)v) ? ow[lo] : 0.0;
   }
 }
 
@@ -172,11 +189,7 @@ void transpose(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out, bool weighted) {
   L.in_ptr.alloc(n + 1);
   L.in_idx.alloc(m);
   if (in_of_out) in_of_out->alloc(m);
-  if (weighted) {
-    L.in_w.alloc(m);
-    L.in_wx.alloc(m);
-    L.in_wf.alloc(m);
-  }
+  if (weighted) L.in_wx.alloc(m);
   if (m == 0) {
     fill_u32(L.in_ptr.p, 0u, n + 1, s);
     c.sync();
@@ -195,8 +208,7 @@ void transpose(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out, bool weighted) {
     while (bits < 32 && ((int64_t)1 << bits) < (int64_t)n) ++bits;
     radix_sort_pairs_u32(keys.p, vals.p, m, 0, bits, c.scratch, s);
     if (weighted)
-      k_in_gather<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, m, L.out_w.p, L.out_wx.p, L.in_idx.p,
-                                             L.in_w.p, L.in_wx.p, L.in_wf.p);
+      k_in_gather<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, m, L.out_wx.p, L.in_idx.p, L.in_wx.p);
     else
       k_in_from_sorted<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, m, L.in_idx.p,
                                                   in_of_out ? in_of_out->p : nullptr);
@@ -212,6 +224,13 @@ void transpose(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out, bool weighted) {
 void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) { transpose(c, L, in_of_out, false); }
 
 void build_in_csr_weighted(Ctx &c, Level &L) { transpose(c, L, nullptr, true); }
+
+void super_in_weights(Ctx &c, const Level &L, double *in_w) {
+  if (L.m == 0) return;
+  k_in_w_lookup<<<grid_for(L.n, 256), 256, 0, c.stream>>>(L.in_ptr.p, L.in_idx.p, L.n, L.out_ptr.p, L.out_idx.p,
+                                                          L.out_w.p, in_w);
+  RMX_LAUNCH_CHECK();
+}
 
 void build_buckets(Ctx &c, Level &L) {
   hipStream_t s = c.stream;
