@@ -79,6 +79,10 @@ struct EvalArgs {
   const long long *hub_off;
 };
 
+// Every unit draws its batch independently.  (Drawing by blocks of
+// consecutive ids would coalesce the per-unit reads, but on networks whose
+// communities occupy id ranges a whole community then moves in one batch
+// against stale statistics, which was measured to raise the codelength.)
 __device__ inline uint32_t batch_of(uint64_t seed, int batches, uint32_t u) {
   if (batches <= 1) return 0;
   return (uint32_t)(mix64(seed ^ ((uint64_t)u * 0x9e3779b97f4a7c15ull)) % (uint64_t)batches);
