@@ -96,31 +96,66 @@ struct Best {
 
 __device__ inline bool better(double d, uint32_t m, const Best &b) { return d < b.d || (d == b.d && m < b.m); }
 
-// Per-unit constants of the evaluation.  With C = S + q1 - q_o and
-// K = -phi(S) - 2 phi(q1) + 2 phi(q_o) + phi(q1 + P_o - p) - phi(q_o + P_o),
-// a candidate m costs
-//   dL = phi(C + q2 - q_m) - 2 phi(q2) + 2 phi(q_m) + phi(q2 + P_m + p) - phi(q_m + P_m) + K.
+// Per-unit constants of the evaluation.  With d1 = q1 - q_o and
+// Kr = -2 phi(q1) + 2 phi(q_o) + phi(q1 + P_o - p) - phi(q_o + P_o)
+// (the unit's own module), a candidate m with q2 = q_m + d2 costs
+//   dL = phi(S + d1 + d2) - phi(S) - 2 phi(q2) + 2 phi(q_m) + phi(q2 + P_m + p) - phi(q_m + P_m) + Kr.
 struct UnitCtx {
   uint32_t o;
   long long p;
   double tele_out;    // alpha p + beta d
-  double td, qu, C, K;
+  double td, qu;
+  double S, d1, Kr, phiS;
   bool single, offer_empty;
 };
 
-__device__ inline double delta_for(const UnitCtx &U, const Stat4 &M, long long fo, long long fi) {
-  const double qm = module_exit(M.p, M.t, M.d, M.x);
+// d2 = q2 - q_m of a move into module M with flows fo (to) and fi (from) it
+__device__ inline double cand_d2(const UnitCtx &U, const Stat4 &M, long long fo, long long fi) {
   const double flows = kBeta * (fx_to(fo) + fx_to(fi));
   const double tele = U.tele_out * fx_to(M.t) + (kAlpha * fx_to(M.p) + kBeta * fx_to(M.d)) * U.td;
-  const double q2 = qm + U.qu - flows - tele;
-  return plogp(U.C + q2 - qm) - 2.0 * plogp(q2) + 2.0 * plogp(qm) + plogp(q2 + fx_to(M.p + U.p)) -
-         plogp(qm + fx_to(M.p)) + U.K;
+  return U.qu - flows - tele;
+}
+
+__device__ inline double delta_for(const UnitCtx &U, const Stat4 &M, long long fo, long long fi) {
+  const double qm = module_exit(M.p, M.t, M.d, M.x);
+  const double d2 = cand_d2(U, M, fo, fi);
+  const double q2 = qm + d2;
+  return plogp(U.S + U.d1 + d2) - U.phiS - 2.0 * plogp(q2) + 2.0 * plogp(qm) + plogp(q2 + fx_to(M.p + U.p)) -
+         plogp(qm + fx_to(M.p)) + U.Kr;
+}
+
+// Mixed precision evaluates the candidates in FP32, in difference form so
+// that no large terms cancel:
+//   dL = dphi(S, d1 + d2) - 2 dphi(q_m, d2) + dphi(q_m + P_m, d2 + p) + Kr,
+//   dphi(x, d) = phi(x + d) - phi(x) = d ln(x + d) + x log1p(d / x),
+// with d1 = q1 - q_o, d2 = q2 - q_m (computed in FP64 from the exact
+// statistics) and Kr the unit's own-module terms.  Only the choice of the
+// move is approximate: commits and the tracked codelength stay exact.
+__device__ inline float dphi32(float x, float d) {
+  const float y = x + d;
+  if (!(x > 0.f)) return y > 0.f ? y * __logf(y) : 0.f;
+  if (!(y > 0.f)) return -x * __logf(x);
+  return d * __logf(y) + x * log1pf(d / x);
+}
+
+__device__ inline double delta_for32(const UnitCtx &U, const Stat4 &M, long long fo, long long fi) {
+  const double qm = module_exit(M.p, M.t, M.d, M.x);
+  const float d2 = (float)cand_d2(U, M, fo, fi);
+  const float fq = (float)qm;
+  const float t = dphi32((float)U.S, (float)U.d1 + d2) - 2.f * dphi32(fq, d2) +
+                  dphi32(fq + (float)fx_to(M.p), d2 + (float)fx_to(U.p));
+  return (double)t + U.Kr;
+}
+
+template <bool F32>
+__device__ inline double cand_delta(const UnitCtx &U, const Stat4 &M, long long fo, long long fi) {
+  return F32 ? delta_for32(U, M, fo, fi) : delta_for(U, M, fo, fi);
 }
 
 __device__ inline double delta_empty(const UnitCtx &U) {
-  // zero statistics and no flows to the new module (Module.cpp:1220-1223)
+  // zero statistics and no flows to the new module (Module.cpp:1220-1223): d2 = q_u
   const double q2 = U.qu;
-  return plogp(U.C + q2) - 2.0 * plogp(q2) + plogp(q2 + fx_to(U.p)) + U.K;
+  return plogp(U.S + U.d1 + q2) - U.phiS - 2.0 * plogp(q2) + plogp(q2 + fx_to(U.p)) + U.Kr;
 }
 
 __device__ inline void unit_ctx(const EvalArgs &a, uint32_t u, long long fo_o, long long fi_o, UnitCtx &U) {
@@ -137,8 +172,10 @@ __device__ inline void unit_ctx(const EvalArgs &a, uint32_t u, long long fo_o, l
   const double ino = kBeta * fx_to(fi_o) + (kAlpha * fx_to(so.p - su.p) + kBeta * fx_to(so.d - su.d)) * U.td;
   const double q1 = qo - U.qu + outo + ino;
   const double S = a.S_inst ? a.S_inst[a.inst[u]] : a.scal[kS];
-  U.C = S + q1 - qo;
-  U.K = -plogp(S) - 2.0 * plogp(q1) + 2.0 * plogp(qo) + plogp(q1 + fx_to(so.p - su.p)) - plogp(qo + fx_to(so.p));
+  U.Kr = -2.0 * plogp(q1) + 2.0 * plogp(qo) + plogp(q1 + fx_to(so.p - su.p)) - plogp(qo + fx_to(so.p));
+  U.S = S;
+  U.phiS = plogp(S);
+  U.d1 = q1 - qo;
   const int32_t units_o = a.mUnits[U.o];
   U.single = units_o == 1;
   U.offer_empty = a.empty_avail && (a.node_level ? a.mStale[U.o] > 1 : units_o > 1);
@@ -257,7 +294,7 @@ __device__ inline int lds_find(const uint32_t *keys, int H, uint32_t m) {
 // to the front of the table (a slot only ever moves down, and every lane
 // reads its slot before any lane writes), so the FP64 evaluation runs on
 // full lanes instead of once per table stride with most lanes masked.
-template <int G, int H>
+template <int G, int H, bool F32>
 __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
   constexpr int GPB = 256 / G;
   __shared__ uint32_t s_keys[GPB][H];
@@ -336,7 +373,7 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
       const uint32_t m = keys[i];
       if (m == o) continue;
       if (!allowed(a, U, m)) continue;
-      const double dl = delta_for(U, a.ms[m], fo[i], fi[i]);
+      const double dl = cand_delta<F32>(U, a.ms[m], fo[i], fi[i]);
       if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; b.i = i; }
     }
     if (lane == 0 && U.offer_empty) {
@@ -388,7 +425,7 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
 // its own list.  The kernel's LDS is exactly the table.
 __device__ inline int wave_slot(int j, int w) { return (j >> 6) * 256 + w * kWave + (j & (kWave - 1)); }
 
-template <int H>
+template <int H, bool F32>
 __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
   constexpr int NW = 256 / kWave;
   static_assert(H >= 6 * NW, "table too small for the per-wave bests");
@@ -466,7 +503,7 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
       const uint32_t m = keys[i];
       if (m == o) continue;
       if (!allowed(a, U, m)) continue;
-      const double dl = delta_for(U, a.ms[m], fo[i], fi[i]);
+      const double dl = cand_delta<F32>(U, a.ms[m], fo[i], fi[i]);
       if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; b.i = i; }
     }
     if (threadIdx.x == 0 && U.offer_empty) {
@@ -663,6 +700,7 @@ __device__ inline void hub_unit_ctx(const EvalArgs &a, uint32_t u, long long bas
 
 // one workgroup per slot chunk: the chunk's non-empty slots are listed in
 // LDS first, then evaluated densely
+template <bool F32>
 __global__ __launch_bounds__(256) void k_hub_eval(EvalArgs a, HubPlan P) {
   constexpr int NW = 256 / kWave;
   __shared__ uint16_t s_list[kHubChunk];
@@ -705,7 +743,7 @@ __global__ __launch_bounds__(256) void k_hub_eval(EvalArgs a, HubPlan P) {
       const int sl = s0 + (int)s_list[q];
       const uint32_t m = ld_key<true>(a.hkeys + base, (uint32_t)sl);
       if (m == U.o || !allowed(a, U, m)) continue;
-      const double dl = delta_for(U, a.ms[m], ld_ll(&a.hout[base + sl]), ld_ll(&a.hin[base + sl]));
+      const double dl = cand_delta<F32>(U, a.ms[m], ld_ll(&a.hout[base + sl]), ld_ll(&a.hin[base + sl]));
       if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; }
     }
     b = group_best<kWave>(b);
@@ -980,6 +1018,7 @@ __global__ __launch_bounds__(256) void k_hubp_scatter(EvalArgs a, HubParts H) {
 
 // one workgroup per partition: LDS aggregation, per-wave in-place compaction
 // (as in k_eval_block) and dense evaluation; best per partition
+template <bool F32>
 __global__ __launch_bounds__(256) void k_hubp_eval(EvalArgs a, HubParts H) {
   constexpr int NW = 256 / kWave;
   __shared__ __align__(16) uint32_t s_keys[kHubTable];
@@ -1051,7 +1090,7 @@ __global__ __launch_bounds__(256) void k_hubp_eval(EvalArgs a, HubParts H) {
       const int sl = wave_slot(j, w);
       const uint32_t m = s_keys[sl];
       if (m == U.o || !allowed(a, U, m)) continue;
-      const double dl = delta_for(U, a.ms[m], s_out[sl], s_in[sl]);
+      const double dl = cand_delta<F32>(U, a.ms[m], s_out[sl], s_in[sl]);
       if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; b.i = sl; }
     }
     b = group_best<kWave>(b);
@@ -1546,6 +1585,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   a.hub_off = L.hub_off.p;
 
   ArcW aout = arcw_out(c, L);
+  const bool f32 = c.precision == 1;   // mixed precision: candidates in FP32
   for (int b = 0; b < c.batches; ++b) {
     const uint32_t *hub_list = nullptr;   // this batch's hubs, for the overflow fallback
     int32_t hub_n = 0;
@@ -1561,11 +1601,26 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       a.list_n = cnt;
       if (c.prof.on) c.prof.begin(s);
       switch (bk) {
-        case 0: k_eval_group<8, 32><<<(unsigned)((cnt + 31) / 32), 256, 0, s>>>(a); break;
-        case 1: k_eval_group<16, 64><<<(unsigned)((cnt + 15) / 16), 256, 0, s>>>(a); break;
-        case 2: k_eval_group<32, 256><<<(unsigned)((cnt + 7) / 8), 256, 0, s>>>(a); break;
-        case 3: k_eval_group<64, 1024><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a); break;
-        case 4: k_eval_block<4096><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a); break;
+        case 0:
+          if (f32) k_eval_group<8, 32, true><<<(unsigned)((cnt + 31) / 32), 256, 0, s>>>(a);
+          else k_eval_group<8, 32, false><<<(unsigned)((cnt + 31) / 32), 256, 0, s>>>(a);
+          break;
+        case 1:
+          if (f32) k_eval_group<16, 64, true><<<(unsigned)((cnt + 15) / 16), 256, 0, s>>>(a);
+          else k_eval_group<16, 64, false><<<(unsigned)((cnt + 15) / 16), 256, 0, s>>>(a);
+          break;
+        case 2:
+          if (f32) k_eval_group<32, 256, true><<<(unsigned)((cnt + 7) / 8), 256, 0, s>>>(a);
+          else k_eval_group<32, 256, false><<<(unsigned)((cnt + 7) / 8), 256, 0, s>>>(a);
+          break;
+        case 3:
+          if (f32) k_eval_group<64, 1024, true><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
+          else k_eval_group<64, 1024, false><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
+          break;
+        case 4:
+          if (f32) k_eval_block<4096, true><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
+          else k_eval_block<4096, false><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
+          break;
         default: {
           if (c.hub_global == 1) {
             HubPlan P{c.hub_plan.p, c.hub_plan.p + (cnt + 1), c.hub_plan.p + 2 * (cnt + 1), c.hub_bd.p,
@@ -1573,7 +1628,8 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
             k_hub_plan<<<1, 256, 0, s>>>(a, P);
             k_hub_clear<<<4096, 256, 0, s>>>(a, P);
             k_hub_insert<<<4096, 256, 0, s>>>(a, P);
-            k_hub_eval<<<4096, 256, 0, s>>>(a, P);
+            if (f32) k_hub_eval<true><<<4096, 256, 0, s>>>(a, P);
+            else k_hub_eval<false><<<4096, 256, 0, s>>>(a, P);
             k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, P);
             break;
           }
@@ -1603,7 +1659,8 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
           exclusive_scan_u32(H.bcnt, H.boff, NB + 1, nullptr, c.scratch, s);
           fill_u32(H.bcnt, 0u, NB + 1, s);
           k_hubp_scatter<<<4096, 256, 0, s>>>(a, H);
-          k_hubp_eval<<<8192, 256, 0, s>>>(a, H);
+          if (f32) k_hubp_eval<true><<<8192, 256, 0, s>>>(a, H);
+          else k_hubp_eval<false><<<8192, 256, 0, s>>>(a, H);
           k_hubp_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, H);
           hub_list = a.list;
           hub_n = cnt;
@@ -1624,7 +1681,8 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       k_hub_plan<<<1, 256, 0, s>>>(a, P);
       k_hub_clear<<<4096, 256, 0, s>>>(a, P);
       k_hub_insert<<<4096, 256, 0, s>>>(a, P);
-      k_hub_eval<<<4096, 256, 0, s>>>(a, P);
+      if (f32) k_hub_eval<true><<<4096, 256, 0, s>>>(a, P);
+            else k_hub_eval<false><<<4096, 256, 0, s>>>(a, P);
       k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (hub_n + 3) / 4), 256, 0, s>>>(a, P);
       RMX_LAUNCH_CHECK();
       RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
