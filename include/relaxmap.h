@@ -145,6 +145,9 @@ typedef struct {
   int32_t prior;      /* 1: prioritized sweeps ("prior"), 0: "normal" */
   uint64_t seed;
   int32_t verbose;    /* 1: print the reference's progress lines to stdout */
+  int32_t attempts;   /* > 1: independent runs from singletons with seeds seed, seed + 1, ...;
+                         the partition with the lowest codelength is kept (the CLI's
+                         <# attempts>, parsed but unused by the reference, OmpRelaxmap.cpp:50) */
 } rmx_partition_opts;
 
 typedef struct {
@@ -158,6 +161,7 @@ typedef struct {
   double time_sweeps_s;
   double time_contract_s;
   double time_coarse_s;       /* sub-module partitions of the coarse tunes */
+  int32_t best_attempt;       /* the attempt kept (0-based) */
 } rmx_partition_result;
 
 int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_partition_result *res);

@@ -29,14 +29,14 @@ def library_path():
 
 class _Opts(C.Structure):
     _fields_ = [("threshold", C.c_double), ("vthresh", C.c_double), ("max_iter", C.c_int32),
-                ("prior", C.c_int32), ("seed", C.c_uint64), ("verbose", C.c_int32)]
+                ("prior", C.c_int32), ("seed", C.c_uint64), ("verbose", C.c_int32), ("attempts", C.c_int32)]
 
 
 class _Result(C.Structure):
     _fields_ = [("codelength", C.c_double), ("computed_codelength", C.c_double), ("n_module", C.c_int32),
                 ("super_steps", C.c_int32), ("sweeps", C.c_int64), ("edge_visits", C.c_int64),
                 ("time_partition_s", C.c_double), ("time_sweeps_s", C.c_double),
-                ("time_contract_s", C.c_double), ("time_coarse_s", C.c_double)]
+                ("time_contract_s", C.c_double), ("time_coarse_s", C.c_double), ("best_attempt", C.c_int32)]
 
 
 @dataclass
@@ -51,6 +51,7 @@ class PartitionResult:
     time_sweeps_s: float
     time_contract_s: float
     time_coarse_s: float
+    best_attempt: int = 0
 
 
 def load_library():
@@ -295,14 +296,16 @@ class Network:
         self._check(self._L.rmx_to_node_level(self._h))
 
     # ------------------------------------------------------------ driver
-    def run(self, threshold=1e-3, vthresh=0.0, max_iter=10, prior=True, seed=1, verbose=False):
-        """main()'s super-step loop (OmpRelaxmap.cpp:181-219)."""
-        o = _Opts(float(threshold), float(vthresh), int(max_iter), 1 if prior else 0, int(seed), 1 if verbose else 0)
+    def run(self, threshold=1e-3, vthresh=0.0, max_iter=10, prior=True, seed=1, verbose=False, attempts=1):
+        """main()'s super-step loop (OmpRelaxmap.cpp:181-219); with attempts > 1,
+        independent runs with seeds seed, seed + 1, ... keep the lowest codelength."""
+        o = _Opts(float(threshold), float(vthresh), int(max_iter), 1 if prior else 0, int(seed), 1 if verbose else 0,
+                  int(attempts))
         r = _Result()
         self._check(self._L.rmx_partition(self._h, C.byref(o), C.byref(r)))
         return PartitionResult(r.codelength, r.computed_codelength, r.n_module, r.super_steps, r.sweeps,
                                r.edge_visits, r.time_partition_s, r.time_sweeps_s, r.time_contract_s,
-                               r.time_coarse_s)
+                               r.time_coarse_s, r.best_attempt)
 
     def write_tree(self, path):
         self._check(self._L.rmx_write_tree(self._h, str(path).encode()))

@@ -255,6 +255,7 @@ int rmx_set_partition(rmx_ctx *ctx, const int32_t *part) {
     for (int32_t i = 0; i < c.N; ++i)
       if (part[i] < 0 || part[i] >= c.N) return fail(ctx, RMX_EINVAL, "module index out of range");
     c.level = 0;
+    c.sweep_counter = 0;   // runs from an installed partition repeat exactly
     DevBuf<uint32_t> d(c.cache);
     d.alloc(c.N);
     RMX_HIP(hipMemcpyAsync(d.p, part, sizeof(uint32_t) * c.N, hipMemcpyHostToDevice, c.stream));
@@ -515,6 +516,41 @@ extern "C" int rmx_contract_sub_modules(rmx_ctx *ctx, int32_t *n_super, int64_t 
   });
 }
 
+namespace {
+
+// One run of main()'s super-step loop (OmpRelaxmap.cpp:181-219) from the
+// current partition.
+void run_super_steps(Ctx &c, const rmx_partition_opts &o, rmx_partition_result &r, Driver &d) {
+  const double ln2 = std::log(2.0);
+  bool fine = true;
+  int step = 1;
+  double old = c.codelength;
+  d.greedy(fine);
+  if (o.verbose) std::printf("SuperStep [%d] - codeLength = %g in %d modules.\n", step, c.codelength / ln2, c.n_module);
+  bool next = !((old - c.codelength) / ln2 < o.threshold);
+  while (next) {
+    old = c.codelength;
+    d.greedy(fine);
+    ++step;
+    if (o.verbose)
+      std::printf("SuperStep [%d] - codeLength = %g in %d modules.\n", step, c.codelength / ln2, c.n_module);
+    if ((old - c.codelength) / ln2 < o.threshold) next = false;
+    fine = !fine;
+  }
+  r.super_steps += step;
+}
+
+// back to singleton modules on the node level (a fresh attempt)
+void reset_to_singletons(Ctx &c) {
+  propagate_labels(c);
+  c.sweep_counter = 0;   // as after rmx_initiate
+  init_modules_singletons(c);
+  module_totals(c, &c.sum_exit, &c.codelength);
+  reset_active(c);
+}
+
+}  // namespace
+
 extern "C" int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_partition_result *res) {
   return guarded(ctx, [&] {
     Ctx &c = ctx->c;
@@ -523,25 +559,36 @@ extern "C" int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_p
     if (opts->max_iter < 0) return fail(ctx, RMX_EINVAL, "max_iter < 0");
     std::memset(res, 0, sizeof(*res));
     RMX_HIP(hipSetDevice(c.device));
-    Driver d{c, *opts, *res, now_s()};
     const double ln2 = std::log(2.0);
     const double t_start = now_s();
-    // main(): OmpRelaxmap.cpp:181-219
-    bool fine = true;
-    int step = 1;
-    double old = c.codelength;
-    d.greedy(fine);
-    if (opts->verbose)
-      std::printf("SuperStep [%d] - codeLength = %g in %d modules.\n", step, c.codelength / ln2, c.n_module);
-    bool next = !((old - c.codelength) / ln2 < opts->threshold);
-    while (next) {
-      old = c.codelength;
-      d.greedy(fine);
-      ++step;
-      if (opts->verbose)
-        std::printf("SuperStep [%d] - codeLength = %g in %d modules.\n", step, c.codelength / ln2, c.n_module);
-      if ((old - c.codelength) / ln2 < opts->threshold) next = false;
-      fine = !fine;
+    const int attempts = opts->attempts > 1 ? opts->attempts : 1;
+    DevBuf<uint32_t> best(c.cache);
+    double best_L = 0.0;
+    Driver d{c, *opts, *res, now_s()};
+    for (int a = 0; a < attempts; ++a) {
+      rmx_partition_opts oa = *opts;
+      oa.seed = opts->seed + (uint64_t)a;
+      Driver da{c, oa, *res, now_s()};
+      if (a > 0) reset_to_singletons(c);
+      run_super_steps(c, oa, *res, da);
+      d.t_sweeps += da.t_sweeps;
+      d.t_contract += da.t_contract;
+      d.t_coarse += da.t_coarse;
+      if (attempts > 1) {
+        propagate_labels(c);
+        const double L = calculate_codelength(c);
+        if (opts->verbose) std::printf("Attempt %d: Computed Code Length = %g\n", a + 1, L / ln2);
+        if (a == 0 || L < best_L) {
+          best_L = L;
+          res->best_attempt = a;
+          best.alloc(c.N);
+          RMX_HIP(hipMemcpyAsync(best.p, c.L0.mod.p, sizeof(uint32_t) * c.N, hipMemcpyDeviceToDevice, c.stream));
+        }
+      }
+    }
+    if (attempts > 1 && res->best_attempt != attempts - 1) {
+      rmx::to_node_level(c);
+      set_partition(c, best.p);   // the kept attempt, with fresh module statistics
     }
     c.sync();
     res->time_partition_s = now_s() - t_start;
@@ -550,7 +597,6 @@ extern "C" int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_p
     res->time_coarse_s = d.t_coarse;
     res->codelength = c.codelength;
     res->n_module = c.n_module;
-    res->super_steps = step;
     res->computed_codelength = calculate_codelength(c);
     c.timing.sweep_ms = d.t_sweeps * 1e3;
     c.timing.contract_ms = d.t_contract * 1e3;

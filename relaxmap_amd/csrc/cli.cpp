@@ -4,7 +4,9 @@
 //            <vThresh> <maxIter> <outDir> <prior/normal> [selflinks]
 //
 // <# threads> is accepted and ignored (the sweeps run on one GPU);
-// <# attempts> is parsed but, as in the reference, one attempt is made.
+// <# attempts> > 1 runs that many independent partitions (seeds seed,
+// seed + 1, ...) and keeps the one with the lowest codelength (the reference
+// parses it but makes one attempt, OmpRelaxmap.cpp:50).
 // Outputs <outDir>/<name>.tree and <outDir>/<name>.clu.
 #include <chrono>
 #include <cmath>
@@ -32,6 +34,7 @@ int main(int argc, char **argv) {
   }
   const unsigned long seed = std::strtoul(argv[1], nullptr, 10);
   const std::string file = argv[2];
+  const int attempts = std::atoi(argv[4]);
   const double threshold = std::atof(argv[5]);
   const double vthresh = std::atof(argv[6]);
   const int max_iter = std::atoi(argv[7]);
@@ -87,6 +90,7 @@ int main(int argc, char **argv) {
   o.prior = prior ? 1 : 0;
   o.seed = seed;
   o.verbose = 1;
+  o.attempts = attempts > 1 ? attempts : 1;
   rmx_partition_result r;
   if (rmx_partition(ctx, &o, &r) != RMX_OK) return die(ctx, "partition");
   double t_end = now_s();
