@@ -170,6 +170,29 @@ def measured_traffic(kernel):
     return None
 
 
+def config4(args):
+    """64 directed SBMs (500-50k nodes, 10-200 blocks, Poisson(12), mu in
+    [0.1, 0.5]) x 8 seeds = 512 independent instances, partitioned by
+    concurrent contexts: instances/s and the per-network codelength spread
+    over seeds."""
+    from relaxmap_amd import synth
+    from relaxmap_amd.batch import run_instances
+    graphs = synth.config4_graphs()
+    seeds = list(range(1, 9))
+    run_instances(graphs[:4], seeds[:1], workers=args.workers)   # warm-up
+    res, dt = run_instances(graphs, seeds, workers=args.workers)
+    spread = []
+    for gi in range(len(graphs)):
+        L = np.array([res[(gi, s)][0] for s in seeds])
+        spread.append(float((L.max() - L.min()) / L.min()))
+    out = {"instances": len(res), "networks": len(graphs), "seeds": len(seeds), "workers": args.workers,
+           "nodes": int(sum(g.n for g in graphs)), "arcs": int(sum(g.m for g in graphs)), "seconds": dt,
+           "instances_per_s": len(res) / dt, "codelength_rel_spread_over_seeds": {
+               "median": float(np.median(spread)), "max": float(np.max(spread))}}
+    log(f"config 4: {len(res)} instances in {dt:.2f} s ({len(res) / dt:.1f} instances/s)")
+    return out
+
+
 def roofline_of(stats, default_workload):
     if not stats:
         return None, {}
@@ -201,6 +224,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
+    ap.add_argument("--no-config4", action="store_true", help="skip the 512-instance batch measurement")
+    ap.add_argument("--workers", type=int, default=8, help="concurrent contexts of the batch measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the hot path here is a single-GPU build "
@@ -263,6 +288,8 @@ def main():
         out["kernels"] = {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in
                           sorted(table.items(), key=lambda kv: -kv[1]["total_ms"])[:12]}
     del g
+    if not args.no_config4:
+        out["config4"] = config4(args)
     if not args.no_cpu:
         base, info = cpu_baseline(args)
         out["cpu_baseline"] = base
