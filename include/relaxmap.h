@@ -67,6 +67,22 @@ int rmx_graph_upload(rmx_ctx *ctx, int32_t n, int64_t m, const int64_t *out_ptr,
 int rmx_load_network(rmx_ctx *ctx, const char *path, int32_t *n, int64_t *m, int32_t *n_self_links,
                      int32_t *n_double_links);
 
+/* Host-side network files, no GPU needed.  rmx_net_read parses a .net or
+ * .txt file with the reference's rules (FileIO.cpp:32-265, then the
+ * self-loop and w <= 0 removal of OmpRelaxmap.cpp:117-135) on `threads` host
+ * threads (0: all), or reads a .rmxcsr binary written by rmx_net_write.  The
+ * arrays stay valid until rmx_net_free.  No reference counterpart. */
+typedef struct rmx_net rmx_net;
+int rmx_net_read(const char *path, int threads, rmx_net **net, char *err, int err_len);
+void rmx_net_free(rmx_net *net);
+int rmx_net_info(const rmx_net *net, int32_t *n, int64_t *m, int32_t *n_self_links, int32_t *n_double_links);
+int rmx_net_arrays(const rmx_net *net, const int64_t **out_ptr, const int32_t **out_idx, const double **w,
+                   const double **node_w);
+const char *rmx_net_name(const rmx_net *net, int32_t i);
+int rmx_net_write(const rmx_net *net, const char *path);
+/* Uploads a parsed network (rmx_graph_upload) and keeps its names for the writers. */
+int rmx_net_upload(rmx_ctx *ctx, const rmx_net *net);
+
 /* Network::initiate (Module.cpp:210-333): dangling nodes, row
  * normalisation, steady state (Network::calculateSteadyState,
  * Module.cpp:340-469: at least 50, at most 200 iterations, L1 change

@@ -5,6 +5,6 @@ The product is the C ABI in include/relaxmap.h (librelaxmap.so) and the
 ``relaxmap`` command line; this package is a thin ctypes view of that ABI
 whose names follow the reference's ``Network`` class (Module.h:99-200).
 """
-from .core import Network, PartitionResult, device_count, library_path, load_library  # noqa: F401
+from .core import Network, NetworkFile, PartitionResult, device_count, library_path, load_library  # noqa: F401
 
-__all__ = ["Network", "PartitionResult", "device_count", "library_path", "load_library"]
+__all__ = ["Network", "NetworkFile", "PartitionResult", "device_count", "library_path", "load_library"]

@@ -21,6 +21,7 @@ _F64P = C.POINTER(C.c_double)
 
 RMX_FP64 = 0
 RMX_MIXED = 1
+RMX_EIO = -5
 
 
 def library_path():
@@ -72,6 +73,13 @@ def load_library():
         "rmx_set_precision": (C.c_int, [vp, C.c_int]),
         "rmx_set_batches": (C.c_int, [vp, C.c_int]),
         "rmx_set_hub_path": (C.c_int, [vp, C.c_int]),
+        "rmx_net_read": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(vp), C.c_char_p, C.c_int]),
+        "rmx_net_free": (None, [vp]),
+        "rmx_net_info": (C.c_int, [vp, _I32P, _I64P, _I32P, _I32P]),
+        "rmx_net_arrays": (C.c_int, [vp, C.POINTER(_I64P), C.POINTER(_I32P), C.POINTER(_F64P), C.POINTER(_F64P)]),
+        "rmx_net_name": (C.c_char_p, [vp, C.c_int32]),
+        "rmx_net_write": (C.c_int, [vp, C.c_char_p]),
+        "rmx_net_upload": (C.c_int, [vp, vp]),
         "rmx_generate_sub_modules": (C.c_int, [vp, C.c_double, C.c_int32, C.c_uint64, _I32P]),
         "rmx_get_sub_modules": (C.c_int, [vp, _I32P]),
         "rmx_contract_sub_modules": (C.c_int, [vp, _I32P, _I64P]),
@@ -112,6 +120,55 @@ def device_count():
 
 def _p(a, t):
     return a.ctypes.data_as(t)
+
+
+class NetworkFile:
+    """A network file parsed on the host (no GPU): .net / .txt with the
+    reference's rules on several threads, or a .rmxcsr binary."""
+
+    def __init__(self, path, threads=0):
+        L = load_library()
+        self._L = L
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = L.rmx_net_read(str(path).encode(), int(threads), C.byref(h), err, 512)
+        if rc != 0:
+            raise (IOError if rc == RMX_EIO else ValueError)(err.value.decode() or f"status {rc}")
+        self._h = h
+        n, m, ns, nd = C.c_int32(), C.c_int64(), C.c_int32(), C.c_int32()
+        L.rmx_net_info(h, C.byref(n), C.byref(m), C.byref(ns), C.byref(nd))
+        self.n, self.m, self.n_self_links, self.n_double_links = n.value, m.value, ns.value, nd.value
+
+    def arrays(self):
+        """(out_ptr, out_idx, w, node_w) as copies."""
+        p, i, w, nw = _I64P(), _I32P(), _F64P(), _F64P()
+        self._L.rmx_net_arrays(self._h, C.byref(p), C.byref(i), C.byref(w), C.byref(nw))
+        n, m = self.n, self.m
+        return (np.ctypeslib.as_array(p, shape=(n + 1,)).copy(),
+                np.ctypeslib.as_array(i, shape=(max(m, 1),))[:m].copy(),
+                np.ctypeslib.as_array(w, shape=(max(m, 1),))[:m].copy(),
+                np.ctypeslib.as_array(nw, shape=(max(n, 1),))[:n].copy())
+
+    def name(self, i):
+        r = self._L.rmx_net_name(self._h, int(i))
+        return None if r is None else r.decode()
+
+    def write(self, path):
+        if self._L.rmx_net_write(self._h, str(path).encode()) != 0:
+            raise IOError(f"cannot write {path}")
+
+    def upload_to(self, net):
+        net._check(self._L.rmx_net_upload(net._h, self._h))
+        net.n, net.m = self.n, self.m
+        return net
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.rmx_net_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Network:

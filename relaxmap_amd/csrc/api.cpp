@@ -168,6 +168,81 @@ int rmx_graph_upload(rmx_ctx *ctx, int32_t n, int64_t m, const int64_t *out_ptr,
   });
 }
 
+struct rmx_net {
+  rmx::NetFile f;
+};
+
+int rmx_net_read(const char *path, int threads, rmx_net **net, char *err, int err_len) {
+  if (!net || !path) return RMX_EINVAL;
+  *net = nullptr;
+  try {
+    set_parse_threads(threads > 0 ? (unsigned)threads : 0u);
+    auto *h = new rmx_net;
+    std::string e;
+    const int rc = read_network(path, h->f, e);
+    set_parse_threads(0);
+    if (rc != 0) {
+      if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", e.c_str());
+      delete h;
+      return rc == -2 ? RMX_EIO : RMX_EINVAL;
+    }
+    *net = h;
+    return RMX_OK;
+  } catch (const std::exception &x) {
+    set_parse_threads(0);
+    if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", x.what());
+    return RMX_EINVAL;
+  }
+}
+
+void rmx_net_free(rmx_net *net) { delete net; }
+
+int rmx_net_info(const rmx_net *net, int32_t *n, int64_t *m, int32_t *n_self_links, int32_t *n_double_links) {
+  if (!net) return RMX_EINVAL;
+  if (n) *n = net->f.n;
+  if (m) *m = (int64_t)net->f.idx.size();
+  if (n_self_links) *n_self_links = net->f.n_self;
+  if (n_double_links) *n_double_links = net->f.n_double;
+  return RMX_OK;
+}
+
+int rmx_net_arrays(const rmx_net *net, const int64_t **out_ptr, const int32_t **out_idx, const double **w,
+                   const double **node_w) {
+  if (!net) return RMX_EINVAL;
+  if (out_ptr) *out_ptr = net->f.ptr.data();
+  if (out_idx) *out_idx = net->f.idx.data();
+  if (w) *w = net->f.w.data();
+  if (node_w) *node_w = net->f.node_w.data();
+  return RMX_OK;
+}
+
+const char *rmx_net_name(const rmx_net *net, int32_t i) {
+  if (!net || i < 0 || i >= net->f.n) return nullptr;
+  return net->f.names[(size_t)i].c_str();
+}
+
+int rmx_net_write(const rmx_net *net, const char *path) {
+  if (!net || !path) return RMX_EINVAL;
+  try {
+    std::string e;
+    return write_binary(path, net->f, e) == 0 ? RMX_OK : RMX_EIO;
+  } catch (...) {
+    return RMX_EIO;
+  }
+}
+
+int rmx_net_upload(rmx_ctx *ctx, const rmx_net *net) {
+  return guarded(ctx, [&] {
+    if (!net) return fail(ctx, RMX_EINVAL, "null network");
+    const rmx::NetFile &f = net->f;
+    int r = rmx_graph_upload(ctx, f.n, (int64_t)f.idx.size(), f.ptr.data(), f.idx.data(), f.w.data(),
+                             f.node_w.data());
+    if (r != RMX_OK) return r;
+    ctx->names = f.names;
+    return RMX_OK;
+  });
+}
+
 int rmx_load_network(rmx_ctx *ctx, const char *path, int32_t *n, int64_t *m, int32_t *n_self, int32_t *n_double) {
   return guarded(ctx, [&] {
     if (!path) return fail(ctx, RMX_EINVAL, "null path");

@@ -1,7 +1,8 @@
 // relaxmap: the ompRelaxmap command line (OmpRelaxmap.cpp:41-252) on the GPU.
 //
-//   relaxmap <seed> <network.net|.txt> <# threads> <# attempts> <threshold>
+//   relaxmap <seed> <network.net|.txt|.rmxcsr> <# threads> <# attempts> <threshold>
 //            <vThresh> <maxIter> <outDir> <prior/normal> [selflinks]
+//   relaxmap --convert <network.net|.txt> <out.rmxcsr>
 //
 // <# threads> is accepted and ignored (the sweeps run on one GPU);
 // <# attempts> > 1 runs that many independent partitions (seeds seed,
@@ -27,6 +28,27 @@ static int die(rmx_ctx *ctx, const char *what) {
 }
 
 int main(int argc, char **argv) {
+  if (argc == 4 && std::string(argv[1]) == "--convert") {
+    // relaxmap --convert <network.net|.txt> <out.rmxcsr>: parse once, keep the binary CSR
+    char err[512] = {0};
+    rmx_net *net = nullptr;
+    double t = now_s();
+    if (rmx_net_read(argv[2], 0, &net, err, sizeof(err)) != RMX_OK) {
+      std::printf("cannot read %s: %s\n", argv[2], err);
+      return 1;
+    }
+    int32_t n = 0, ns = 0, nd = 0;
+    int64_t m = 0;
+    rmx_net_info(net, &n, &m, &ns, &nd);
+    std::printf("parsed %s: %d nodes, %lld arcs in %.2f s\n", argv[2], n, (long long)m, now_s() - t);
+    const int rc = rmx_net_write(net, argv[3]);
+    rmx_net_free(net);
+    if (rc != RMX_OK) {
+      std::printf("cannot write %s\n", argv[3]);
+      return 1;
+    }
+    return 0;
+  }
   if (argc < 10) {
     std::printf("Call: ./relaxmap <seed> <network.net> <# threads> <# attempts> <threshold> <vThresh> <maxIter> "
                 "<outDir> <prior/normal>  [selflinks]\n");

@@ -20,7 +20,11 @@ struct NetFile {
 };
 
 // 0 on success; -1 malformed (the reference exits), -2 unreadable.
+// .net and .txt follow the reference's parsers; .rmxcsr is the binary CSR
+// written by write_binary.
 int read_network(const std::string &path, NetFile &net, std::string &err);
+int write_binary(const std::string &path, const NetFile &net, std::string &err);
+void set_parse_threads(unsigned t);   // 0: all hardware threads
 int write_tree(const std::string &path, double codelength_nats, const std::vector<uint32_t> &mod,
                const std::vector<double> &p, const std::vector<std::string> &names, std::string &err);
 int write_clu(const std::string &path, const std::vector<uint32_t> &mod, std::string &err);
