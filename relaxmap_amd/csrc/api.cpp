@@ -58,7 +58,7 @@ void free_ctx_device(Ctx &c) {
   c.node_w.free(); c.raw_w.free(); c.dangling.free(); c.in_of_out.free(); c.node_sp.free();
   c.ms.free(); c.mNodes.free(); c.mUnits.free(); c.mStale.free();
   c.mInst.free();
-  c.act.free(); c.act_next.free(); c.target.free(); c.newmod.free(); c.stamp.free(); c.movers.free();
+  c.act.free(); c.act_next.free(); c.target.free(); c.newmod.free(); c.stamp.free(); c.movers.free(); c.movers_st.free();
   c.empty_list.free(); c.want_empty.free(); c.rank_tmp.free(); c.agg_buf.free();
   c.batch_keys.free(); c.batch_list.free();
   c.hub_keys.free(); c.hub_out.free(); c.hub_in.free(); c.hub_plan.free(); c.hub_bm.free(); c.hub_bd.free();

@@ -212,7 +212,9 @@ struct Ctx {
   DevBuf<long long> hp_bfo, hp_bfi;
   DevBuf<uint32_t> hp_plan, hp_ovf;
   DevBuf<long long> hp_own;
-  DevBuf<uint32_t> counters;         // small device counters
+  DevBuf<uint32_t> counters;         // small device counters, then kStripes copies of the launch counters
+  DevBuf<uint32_t> movers_st;        // movers per counter copy, mv_cap each
+  uint32_t mv_cap = 0;
   DevBuf<double> scal;               // device scalars: S, L, ...
   DevBuf<double> partial;            // block partials for reductions
   uint32_t stamp_now = 0;

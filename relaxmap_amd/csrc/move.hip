@@ -39,7 +39,29 @@ namespace rmx {
 namespace {
 
 enum { kS = 0, kL = 3, kNM = 7 };
-enum { cMovers = 0, cWant = 1, cMoved = 2, cVisitsLo = 4, cCandsLo = 6, cUnits = 10, cBig = 11, cHubOvf = 12 };
+enum { cMovers = 0, cWant = 1, cMoved = 2, cVisitsLo = 4, cCandsLo = 6, cUnits = 10, cBig = 11, cHubOvf = 12,
+       cErr = 13 };
+
+// Counters that every workgroup of a large launch updates (movers, empty
+// requests, moved nodes, edge visits, candidates, units) are kept in
+// kStripes copies 128 bytes apart; a workgroup uses copy (block id mod
+// kStripes).  Device-scope atomics on one address serialise at about 11 ns
+// each on MI355X, so a single counter bumped by every workgroup of an
+// 85k-workgroup launch costs about a millisecond per update per workgroup;
+// spread over the copies they proceed in parallel.  Each copy appends its
+// movers to its own region of movers_st (mv_cap entries: at most
+// n/64 + 1024 movers of one batch fall on one copy, since every launch maps
+// at most ceil(grid/kStripes) workgroups of bounded unit count to a copy).
+// k_gather_movers appends the copies' movers to the dense mover list and
+// k_fold_stripes adds the copies to `counters` and clears them.
+constexpr int kStripes = 128;
+constexpr int kStripeWords = 32;
+constexpr int kStripeBase = 512;   // offset of the copies in the counter buffer
+constexpr int kCounterWords = kStripeBase + kStripes * kStripeWords;
+
+__device__ inline uint32_t *my_stripe(uint32_t *stripes) {
+  return stripes + (blockIdx.x & (uint32_t)(kStripes - 1)) * (uint32_t)kStripeWords;
+}
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kInvalid = 0xfffffffeu;
 
@@ -67,12 +89,11 @@ struct EvalArgs {
   double vthresh;
   uint32_t *target;
   long long *agg;        // per unit: flows to/from the old and the new module
-  uint32_t *movers;
+  uint32_t *movers_st;   // movers per counter copy, mv_cap entries each
+  uint32_t mv_cap;
   uint32_t *counters;
+  uint32_t *stripes;     // kStripes copies of the per-launch counters
   uint32_t *want;
-  unsigned long long *visits;
-  unsigned long long *cands;
-  uint32_t *units;
   // hub tables: region [hub_off[g], hub_off[g+1]) of the pool for hub g
   uint32_t *hkeys;
   long long *hout, *hin;
@@ -87,6 +108,19 @@ __device__ inline uint32_t batch_of(uint64_t seed, int batches, uint32_t u) {
   if (batches <= 1) return 0;
   return (uint32_t)(mix64(seed ^ ((uint64_t)u * 0x9e3779b97f4a7c15ull)) % (uint64_t)batches);
 }
+
+__device__ inline void put_mover(const EvalArgs &a, uint32_t slot, uint32_t u) {
+  if (slot < a.mv_cap) a.movers_st[(size_t)(blockIdx.x & (uint32_t)(kStripes - 1)) * a.mv_cap + slot] = u;
+  else atomicOr(&a.counters[cErr], 1u);   // cannot happen within the bound above; reported as an error
+}
+
+__device__ inline void add_visits(const EvalArgs &a, unsigned long long v) {
+  atomicAdd((unsigned long long *)(my_stripe(a.stripes) + cVisitsLo), v);
+}
+__device__ inline void add_cands(const EvalArgs &a, unsigned long long v) {
+  atomicAdd((unsigned long long *)(my_stripe(a.stripes) + cCandsLo), v);
+}
+__device__ inline void add_units(const EvalArgs &a, uint32_t v) { atomicAdd(my_stripe(a.stripes) + cUnits, v); }
 
 struct Best {
   double d;
@@ -195,11 +229,11 @@ __device__ inline void emit(const EvalArgs &a, uint32_t u, const Best &b, long l
   g[1] = fi_o;
   g[2] = fo_n;
   g[3] = fi_n;
-  uint32_t i = atomicAdd(&a.counters[cMovers], 1u);
-  a.movers[i] = u;
+  uint32_t *st = my_stripe(a.stripes);
+  put_mover(a, atomicAdd(&st[cMovers], 1u), u);
   if (b.m == a.empty_key) {
     a.want[u] = 1u;
-    atomicAdd(&a.counters[cWant], 1u);
+    atomicAdd(&st[cWant], 1u);
   }
 }
 
@@ -211,16 +245,16 @@ __device__ inline uint64_t lanes_below() {
 
 // Called by every lane of a wavefront (uniform control flow); the lanes with
 // `active` set emit their unit as emit() does, with one atomic on the mover
-// counter per wavefront instead of one per mover (millions of movers per
-// batch would otherwise serialise on that one address).
+// counter per wavefront instead of one per mover.
 __device__ inline void emit_wave(const EvalArgs &a, bool active, uint32_t u, const Best &b, long long fo_o,
                                  long long fi_o, long long fo_n, long long fi_n) {
   const bool mv = active && b.d < a.vthresh && b.m != kNone;
   const uint64_t bal = __ballot(mv);
   if (bal == 0) return;
   const int leader = __ffsll((unsigned long long)bal) - 1;
+  uint32_t *st = my_stripe(a.stripes);
   uint32_t base = 0;
-  if ((int)(threadIdx.x & (kWave - 1)) == leader) base = atomicAdd(&a.counters[cMovers], (uint32_t)__popcll(bal));
+  if ((int)(threadIdx.x & (kWave - 1)) == leader) base = atomicAdd(&st[cMovers], (uint32_t)__popcll(bal));
   base = __shfl(base, leader, kWave);
   if (!mv) return;
   a.target[u] = b.m;
@@ -229,10 +263,10 @@ __device__ inline void emit_wave(const EvalArgs &a, bool active, uint32_t u, con
   g[1] = fi_o;
   g[2] = fo_n;
   g[3] = fi_n;
-  a.movers[base + (uint32_t)__popcll(bal & lanes_below())] = u;
+  put_mover(a, base + (uint32_t)__popcll(bal & lanes_below()), u);
   if (b.m == a.empty_key) {
     a.want[u] = 1u;
-    atomicAdd(&a.counters[cWant], 1u);
+    atomicAdd(&st[cWant], 1u);
   }
 }
 
@@ -411,9 +445,9 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
       c += red[NW + i];
       nu += red[2 * NW + i];
     }
-    if (t) atomicAdd(a.visits, t);
-    if (c) atomicAdd(a.cands, c);
-    if (nu) atomicAdd(a.units, (uint32_t)nu);
+    if (t) add_visits(a, t);
+    if (c) add_cands(a, c);
+    if (nu) add_units(a, (uint32_t)nu);
   }
 }
 
@@ -464,7 +498,7 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
     __syncthreads();
     if (threadIdx.x == 0) {
       vis_acc += (oe - ob) + (ie - ib);
-      atomicAdd(a.units, 1u);
+      add_units(a, 1u);
     }
     const uint32_t o = a.mod[u];
     long long fo_o = 0, fi_o = 0;
@@ -537,8 +571,8 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    if (vis_acc) atomicAdd(a.visits, vis_acc);
-    if (cand_acc) atomicAdd(a.cands, cand_acc);
+    if (vis_acc) add_visits(a, vis_acc);
+    if (cand_acc) add_cands(a, cand_acc);
   }
 }
 
@@ -761,7 +795,7 @@ __global__ __launch_bounds__(256) void k_hub_eval(EvalArgs a, HubPlan P) {
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0 && ncand) atomicAdd(a.cands, ncand);
+  if (threadIdx.x == 0 && ncand) add_cands(a, ncand);
 }
 
 // one wavefront per hub: best over its chunks (fixed order), empty option, emit
@@ -794,8 +828,8 @@ __global__ __launch_bounds__(256) void k_hub_finish(EvalArgs a, HubPlan P) {
         fi_n = ld_ll(&a.hin[base + sn]);
       }
       emit(a, u, b, fo_o, fi_o, fo_n, fi_n);
-      atomicAdd(a.visits, (unsigned long long)((a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u])));
-      atomicAdd(a.units, 1u);
+      add_visits(a, (unsigned long long)((a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u])));
+      add_units(a, 1u);
     }
   }
 }
@@ -1129,7 +1163,7 @@ __global__ __launch_bounds__(256) void k_hubp_eval(EvalArgs a, HubParts H) {
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0 && cand_acc) atomicAdd(a.cands, cand_acc);
+  if (threadIdx.x == 0 && cand_acc) add_cands(a, cand_acc);
 }
 
 // one wavefront per hub: best over its partitions (fixed order), empty
@@ -1155,8 +1189,8 @@ __global__ __launch_bounds__(256) void k_hubp_finish(EvalArgs a, HubParts H) {
       }
       const long long fo_n = b.i >= 0 ? H.bfo[b.i] : 0, fi_n = b.i >= 0 ? H.bfi[b.i] : 0;
       emit(a, u, b, fo_o, fi_o, fo_n, fi_n);
-      atomicAdd(a.visits, (unsigned long long)((a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u])));
-      atomicAdd(a.units, 1u);
+      add_visits(a, (unsigned long long)((a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u])));
+      add_units(a, 1u);
     }
   }
 }
@@ -1201,7 +1235,7 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
                                                 const uint32_t *__restrict__ stamp, uint32_t cur,
                                                 const long long *__restrict__ agg, const Stat4 *__restrict__ us,
                                                 const int32_t *__restrict__ nmem, Stat4 *ms, int32_t *mNodes,
-                                                int32_t *mUnits, uint8_t *act_next, int mark, uint32_t *moved,
+                                                int32_t *mUnits, uint8_t *act_next, int mark, uint32_t *stripes,
                                                 uint32_t big_deg, uint32_t *big_list, uint32_t *big_cnt) {
   const uint32_t nm = counters[cMovers];
   const int lane = threadIdx.x % G;
@@ -1256,7 +1290,7 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
     }
   }
   moved_here = wave_sum(moved_here);
-  if ((threadIdx.x & (kWave - 1)) == 0 && moved_here) atomicAdd(moved, moved_here);
+  if ((threadIdx.x & (kWave - 1)) == 0 && moved_here) atomicAdd(my_stripe(stripes) + cMoved, moved_here);
 }
 
 // Arc chunks of the deferred (high-degree) movers: prefix of chunk counts.
@@ -1380,6 +1414,66 @@ __global__ void k_hist256(const uint32_t *__restrict__ key, int64_t n, uint32_t 
   if (h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
 }
 
+// Appends the movers held by the counter copies to the dense mover list,
+// after the movers already there (counters[cMovers]).
+__global__ __launch_bounds__(256) void k_gather_movers(const uint32_t *__restrict__ counters,
+                                                       const uint32_t *__restrict__ stripes,
+                                                       const uint32_t *__restrict__ mv_st, uint32_t cap,
+                                                       uint32_t *__restrict__ movers) {
+  __shared__ uint32_t pre[kStripes + 1];
+  if (threadIdx.x < kStripes) pre[threadIdx.x + 1] = min(stripes[threadIdx.x * kStripeWords + cMovers], cap);
+  if (threadIdx.x == 0) pre[0] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int j = 0; j < kStripes; ++j) pre[j + 1] += pre[j];
+  __syncthreads();
+  const uint32_t base = counters[cMovers], total = pre[kStripes];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    int lo = 0, hi = kStripes;   // pre[lo] <= i < pre[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (pre[mid] <= i) lo = mid;
+      else hi = mid;
+    }
+    movers[base + i] = mv_st[(size_t)lo * cap + (i - pre[lo])];
+  }
+}
+
+// Adds the counter copies to `counters` (one workgroup of kStripes lanes,
+// after k_gather_movers) and clears them.
+__global__ __launch_bounds__(kStripes) void k_fold_stripes(uint32_t *__restrict__ counters,
+                                                           uint32_t *__restrict__ stripes, uint32_t cap) {
+  constexpr int NW = kStripes / kWave;
+  __shared__ unsigned long long part[6][NW];
+  uint32_t *st = stripes + threadIdx.x * kStripeWords;
+  unsigned long long v[6] = {min(st[cMovers], cap), st[cWant], st[cMoved],
+                             *(const unsigned long long *)(st + cVisitsLo),
+                             *(const unsigned long long *)(st + cCandsLo), st[cUnits]};
+  st[cMovers] = 0;
+  st[cWant] = 0;
+  st[cMoved] = 0;
+  *(unsigned long long *)(st + cVisitsLo) = 0;
+  *(unsigned long long *)(st + cCandsLo) = 0;
+  st[cUnits] = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    v[k] = wave_sum(v[k]);
+    if ((threadIdx.x & (kWave - 1)) == 0) part[k][threadIdx.x / kWave] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 6; ++k)
+      for (int w = 0; w < NW; ++w) t[k] += part[k][w];
+    counters[cMovers] += (uint32_t)t[0];
+    counters[cWant] += (uint32_t)t[1];
+    counters[cMoved] += (uint32_t)t[2];
+    *(unsigned long long *)(counters + cVisitsLo) += t[3];
+    *(unsigned long long *)(counters + cCandsLo) += t[4];
+    counters[cUnits] += (uint32_t)t[5];
+  }
+}
+
 __global__ void k_reset_counters(uint32_t *counters) {
   if (threadIdx.x < 2) counters[threadIdx.x] = 0;
   if (threadIdx.x == 2) counters[cBig] = 0;
@@ -1403,6 +1497,15 @@ unsigned stride_grid(int64_t items, int per_block) {
   return (unsigned)(g < 1 ? 1 : g);
 }
 
+// Folds the counter copies into the counters (movers into the dense list).
+void fold_counters(Ctx &c, hipStream_t s) {
+  k_gather_movers<<<512, 256, 0, s>>>(c.counters.p, c.counters.p + kStripeBase, c.movers_st.p, c.mv_cap,
+                                      c.movers.p);
+  RMX_LAUNCH_CHECK();
+  k_fold_stripes<<<1, kStripes, 0, s>>>(c.counters.p, c.counters.p + kStripeBase, c.mv_cap);
+  RMX_LAUNCH_CHECK();
+}
+
 }  // namespace
 
 // Algorithmic bytes of one evaluation launch: per arc visit the neighbour id,
@@ -1417,6 +1520,7 @@ static void profile_eval(Ctx &c, int bucket, hipStream_t s) {
   const float ms = c.prof.stop(s);   // end event right after the launch
   uint32_t before[16];
   std::memcpy(before, c.h_cnt, sizeof(before));
+  fold_counters(c, s);
   RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
   c.sync();
   auto u64 = [](const uint32_t *h, int i) { return (double)((uint64_t)h[i] | ((uint64_t)h[i + 1] << 32)); };
@@ -1439,10 +1543,12 @@ void ensure_sweep_buffers(Ctx &c) {
   c.newmod.alloc(n);
   c.stamp.alloc(n);
   c.movers.alloc(n);
+  c.mv_cap = (uint32_t)((n + 63) / 64 + 1024);
+  c.movers_st.alloc((int64_t)kStripes * c.mv_cap);
   c.empty_list.alloc(n);
   c.want_empty.alloc(n);
   c.rank_tmp.alloc(n);
-  c.counters.alloc(16 + 256);
+  c.counters.alloc(kCounterWords);
   c.agg_buf.alloc(4 * (int64_t)n);
   c.batch_keys.alloc(n);
   c.batch_list.alloc(n);
@@ -1450,7 +1556,7 @@ void ensure_sweep_buffers(Ctx &c) {
   fill_u8(c.act_next.p, 0, n, c.stream);
   fill_u32(c.stamp.p, 0u, n, c.stream);
   fill_u32(c.want_empty.p, 0u, n, c.stream);
-  fill_u32(c.counters.p, 0u, 16 + 256, c.stream);
+  fill_u32(c.counters.p, 0u, kCounterWords, c.stream);
 }
 
 void reset_active(Ctx &c) {
@@ -1479,6 +1585,8 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   }
   fill_u32(c.counters.p + 2, 0u, 6, s);   // moved, visits, candidates
   fill_u32(c.counters.p + cUnits, 0u, 1, s);
+  fill_u32(c.counters.p + cErr, 0u, 1, s);
+  fill_u32(c.counters.p + kStripeBase, 0u, kStripes * kStripeWords, s);
   RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
   c.sync();
   n_empty = c.sub_mode ? 0u : c.h_cnt[8];
@@ -1573,12 +1681,11 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   a.vthresh = prioritized ? vthresh : 0.0;
   a.target = c.target.p;
   a.agg = c.agg_buf.p;
-  a.movers = c.movers.p;
+  a.movers_st = c.movers_st.p;
+  a.mv_cap = c.mv_cap;
   a.counters = c.counters.p;
+  a.stripes = c.counters.p + kStripeBase;
   a.want = c.want_empty.p;
-  a.visits = (unsigned long long *)(c.counters.p + cVisitsLo);
-  a.cands = (unsigned long long *)(c.counters.p + cCandsLo);
-  a.units = c.counters.p + cUnits;
   a.hkeys = c.hub_keys.p;
   a.hout = c.hub_out.p;
   a.hin = c.hub_in.p;
@@ -1670,8 +1777,10 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       RMX_LAUNCH_CHECK();
       if (c.prof.on) profile_eval(c, bk, s);
     }
+    fold_counters(c, s);
     RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
     c.sync();
+    if (c.h_cnt[cErr]) throw HipError("move sweep: mover list of a counter copy overflowed");
     if (hub_n > 0 && c.h_cnt[cHubOvf] > 0) {
       // a partition table overflowed: those hubs again with global tables
       a.list = hub_list;
@@ -1685,8 +1794,10 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
             else k_hub_eval<false><<<4096, 256, 0, s>>>(a, P);
       k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (hub_n + 3) / 4), 256, 0, s>>>(a, P);
       RMX_LAUNCH_CHECK();
+      fold_counters(c, s);
       RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
       c.sync();
+      if (c.h_cnt[cErr]) throw HipError("move sweep: mover list of a counter copy overflowed");
     }
     const uint32_t n_movers = c.h_cnt[cMovers], n_want = c.h_cnt[cWant];
     if (n_movers == 0) continue;
@@ -1704,7 +1815,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     k_commit<16><<<stride_grid((int64_t)n_movers * 16, 256), 256, 0, s>>>(
         c.movers.p, c.counters.p, L.out_ptr.p, L.out_idx.p, L.in_ptr.p, L.in_idx.p, aout, L.mod.p, c.newmod.p,
         c.stamp.p, c.stamp_now, c.agg_buf.p, L.us.p, L.nmem.p, c.ms.p, c.mNodes.p, c.mUnits.p, c.act_next.p,
-        prioritized ? 1 : 0, c.counters.p + cMoved, big_deg, c.big_list.p, c.counters.p + cBig);
+        prioritized ? 1 : 0, c.counters.p + kStripeBase, big_deg, c.big_list.p, c.counters.p + cBig);
     RMX_LAUNCH_CHECK();
     if (n_hubs > 0) {
       // only hubs (more than big_deg arcs) are deferred: the list holds at most n_hubs
@@ -1725,6 +1836,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     c.act.swap(c.act_next);
     fill_u8(c.act_next.p, 0, c.N, s);
   }
+  fold_counters(c, s);   // moved nodes of the last batch
   RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
   if (c.sub_mode) {
     instance_totals_async(c);
