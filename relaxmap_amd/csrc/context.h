@@ -264,7 +264,6 @@ struct Ctx {
 void upload_graph(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx,
                   const double *w, const double *node_w);
 void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out);
-void build_in_csr_weighted(Ctx &c, Level &L);   // in-arc fixed-point weights from the out-arcs
 void super_in_weights(Ctx &c, const Level &L, double *in_w);   // FP64 in-weights of a contracted level
 void build_buckets(Ctx &c, Level &L);
 

@@ -3,12 +3,14 @@
 // (Network::updateMembersInModule, Module.cpp:2881-2902).
 //
 // Relabel: super node id = rank of the module among non-empty modules
-// (exclusive scan of non-empty flags).  Arcs between different super nodes
-// get the key (src_sp, dst_sp); a stable LSD radix sort by key keeps, inside
-// every key, the source level's arc order.  Each key segment is then summed
-// left to right by one thread; the fixed-point sums the sweeps use are
-// carried beside the FP64 sums.  In-lists are the transpose, again by a
-// stable sort (ascending source).
+// (exclusive scan of non-empty flags).  Every arc gets the key
+// (src_sp, dst_sp), or a sentinel above every valid key when both ends lie in
+// one super node; a stable LSD radix sort by key keeps, inside every key, the
+// source level's arc order, and moves the sentinels to the end.  Each key
+// segment is then summed left to right by one thread; the fixed-point sums
+// the sweeps use are carried beside the FP64 sums.  In-lists are the
+// transpose: the super arcs stably sorted by head (ascending source within a
+// head), with the fixed-point weights carried through the sort.
 //
 // The reference always contracts the original arcs (source node ascending,
 // then stored out-link order: the order its std::map sums them), so from
@@ -53,34 +55,35 @@ __global__ void k_node_sp(const uint32_t *__restrict__ mod, const uint32_t *__re
   if (i < n) node_sp[i] = mod2sp[mod[i]];
 }
 
-// flag arcs whose ends lie in different super nodes
+// sort key of every arc: (src_sp, dst_sp), or kIntra when both ends lie in
+// one super node (bits leaves the all-ones pattern unused by valid keys)
+constexpr uint64_t kIntra = ~0ull;
+
 template <int G>
-__global__ void k_inter_flags(const uint32_t *__restrict__ optr, const uint32_t *__restrict__ oidx, int32_t n,
-                              const uint32_t *__restrict__ node_sp, uint32_t *__restrict__ flag) {
+__global__ void k_arc_keys(const uint32_t *__restrict__ optr, const uint32_t *__restrict__ oidx, int32_t n,
+                           const uint32_t *__restrict__ node_sp, int bits, uint64_t *__restrict__ keys,
+                           uint32_t *__restrict__ vals) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t a = t / G;
   const int lane = (int)(t % G);
   if (a >= n) return;
   const uint32_t sa = node_sp[a];
-  for (uint32_t k = optr[a] + lane; k < optr[a + 1]; k += G) flag[k] = node_sp[oidx[k]] != sa ? 1u : 0u;
+  for (uint32_t k = optr[a] + lane; k < optr[a + 1]; k += G) {
+    const uint32_t sb = node_sp[oidx[k]];
+    keys[k] = sb != sa ? (((uint64_t)sa << bits) | (uint64_t)sb) : kIntra;
+    vals[k] = k;
+  }
 }
 
-template <int G>
-__global__ void k_inter_keys(const uint32_t *__restrict__ optr, const uint32_t *__restrict__ oidx, int32_t n,
-                             const uint32_t *__restrict__ node_sp, const uint32_t *__restrict__ flag,
-                             const uint32_t *__restrict__ pos, int bits, uint64_t *__restrict__ keys,
-                             uint32_t *__restrict__ vals) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t a = t / G;
-  const int lane = (int)(t % G);
-  if (a >= n) return;
-  const uint64_t sa = node_sp[a];
-  for (uint32_t k = optr[a] + lane; k < optr[a + 1]; k += G) {
-    if (!flag[k]) continue;
-    const uint32_t j = pos[k];
-    keys[j] = (sa << bits) | (uint64_t)node_sp[oidx[k]];
-    vals[j] = k;
+// number of keys before the first kIntra (keys sorted ascending)
+__global__ void k_count_inter(const uint64_t *__restrict__ keys, int64_t m, uint32_t *__restrict__ out) {
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] == kIntra) hi = mid;
+    else lo = mid + 1;
   }
+  *out = (uint32_t)lo;
 }
 
 __global__ void k_key_heads(const uint64_t *__restrict__ keys, int64_t m, uint32_t *__restrict__ flag) {
@@ -95,13 +98,15 @@ __global__ void k_key_starts(const uint32_t *__restrict__ flag, const uint32_t *
 }
 
 // one thread per (src, dst) super arc: sequential FP64 sum in arc order (in
-// mixed precision, from the node level: of the FP32 arc flows)
+// mixed precision, from the node level: of the FP32 arc flows); also the
+// transpose's key (dst, src) and the fixed-point weight it carries
 __global__ void k_reduce_segments(const uint32_t *__restrict__ start, int64_t n_seg, int64_t m,
                                   const uint64_t *__restrict__ keys, const uint32_t *__restrict__ arc,
                                   const double *__restrict__ w, const float *__restrict__ wf,
                                   const long long *__restrict__ wx, int mixed, int bits,
                                   uint32_t *__restrict__ out_idx, double *__restrict__ out_w,
-                                  long long *__restrict__ out_wx) {
+                                  long long *__restrict__ out_wx, uint64_t *__restrict__ tkey,
+                                  uint64_t *__restrict__ twx) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_seg) return;
   const uint32_t b = start[s];
@@ -118,9 +123,34 @@ __global__ void k_reduce_segments(const uint32_t *__restrict__ start, int64_t n_
     sx += wx ? wx[k] : fx_of(f);
   }
   const uint64_t key = keys[b];
-  out_idx[s] = (uint32_t)(key & ((1ull << bits) - 1ull));
+  const uint64_t mask = (1ull << bits) - 1ull;
+  out_idx[s] = (uint32_t)(key & mask);
   out_w[s] = sum;
   out_wx[s] = sx;
+  tkey[s] = ((key & mask) << bits) | (key >> bits);
+  twx[s] = (uint64_t)sx;
+}
+
+// in-CSR of the new level from the transposed keys (head, source) sorted by
+// head: sources and fixed-point weights
+__global__ void k_in_from_keys(const uint64_t *__restrict__ key, const uint64_t *__restrict__ wx, int64_t m,
+                               int bits, uint32_t *__restrict__ in_idx, long long *__restrict__ in_wx) {
+  const uint64_t mask = (1ull << bits) - 1ull;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x) {
+    in_idx[k] = (uint32_t)(key[k] & mask);
+    in_wx[k] = (long long)wx[k];
+  }
+}
+
+// in_ptr from the heads (key >> bits, ascending): position i fills every
+// head in (head[i-1], head[i]], and position m the tail up to n
+__global__ void k_in_ptr(const uint64_t *__restrict__ key, int bits, int64_t m, int32_t n,
+                         uint32_t *__restrict__ ptr) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= m; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t lo = i == 0 ? 0 : (int64_t)(key[i - 1] >> bits) + 1;
+    const int64_t hi = i == m ? (int64_t)n : (int64_t)(key[i] >> bits);
+    for (int64_t d = lo; d <= hi; ++d) ptr[d] = (uint32_t)i;
+  }
 }
 
 // out_ptr of the super level from the segments' sources (ascending): segment
@@ -222,34 +252,26 @@ void contract(Ctx &c) {
   c.sync();
   mod2sp.free();
 
-  // inter-super-node arcs in source arc order
-  DevBuf<uint32_t> aflag(c.cache), apos(c.cache), atot(c.cache);
-  aflag.alloc(m);
-  apos.alloc(m);
+  // every arc by key: the inter-super-node arcs in key order (source arc
+  // order inside a key), the intra ones after them
+  DevBuf<uint32_t> atot(c.cache);
   atot.alloc(1);
-  uint32_t m_inter = 0;
-  if (m > 0) {
-    k_inter_flags<16><<<grid_for((int64_t)ns_src * 16, 256), 256, 0, s>>>(S.out_ptr.p, S.out_idx.p, ns_src, usp,
-                                                                           aflag.p);
-    RMX_LAUNCH_CHECK();
-    exclusive_scan_u32(aflag.p, apos.p, m, atot.p, c.scratch, s);
-    RMX_HIP(hipMemcpyAsync(&m_inter, atot.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    c.sync();
-  }
-  const int bits = bits_for(ns);
+  const int bits = bits_for((int64_t)ns + 1);   // the all-ones key stays free for kIntra
   DevBuf<uint64_t> keys(c.cache);
   DevBuf<uint32_t> arcs(c.cache);
-  keys.alloc(m_inter);
-  arcs.alloc(m_inter);
-  if (m_inter > 0) {
-    k_inter_keys<16><<<grid_for((int64_t)ns_src * 16, 256), 256, 0, s>>>(S.out_ptr.p, S.out_idx.p, ns_src, usp,
-                                                                          aflag.p, apos.p, bits, keys.p, arcs.p);
+  keys.alloc(m);
+  arcs.alloc(m);
+  uint32_t m_inter = 0;
+  if (m > 0) {
+    k_arc_keys<16><<<grid_for((int64_t)ns_src * 16, 256), 256, 0, s>>>(S.out_ptr.p, S.out_idx.p, ns_src, usp, bits,
+                                                                        keys.p, arcs.p);
     RMX_LAUNCH_CHECK();
-    radix_sort_pairs_u64(keys.p, arcs.p, m_inter, 0, 2 * bits, c.scratch, s);
+    radix_sort_pairs_u64(keys.p, arcs.p, m, 0, 2 * bits, c.scratch, s);
+    k_count_inter<<<1, 1, 0, s>>>(keys.p, m, atot.p);
+    RMX_LAUNCH_CHECK();
+    RMX_HIP(hipMemcpyAsync(&m_inter, atot.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   }
   c.sync();
-  aflag.free();
-  apos.free();
 
   // one super arc per distinct key
   uint32_t m_super = 0;
@@ -273,11 +295,14 @@ void contract(Ctx &c) {
   D.out_idx.alloc(m_super);
   D.out_w.alloc(m_super);
   D.out_wx.alloc(m_super);
+  DevBuf<uint64_t> tkey(c.cache), twx(c.cache);   // transposed keys and their weights
+  tkey.alloc(m_super);
+  twx.alloc(m_super);
 
   if (m_super > 0) {
     k_reduce_segments<<<grid_for(m_super, 256), 256, 0, s>>>(
         kstart.p, m_super, m_inter, keys.p, arcs.p, S.out_w.p, S.out_wf.p, from_super ? S.out_wx.p : nullptr,
-        c.precision == 1 ? 1 : 0, bits, D.out_idx.p, D.out_w.p, D.out_wx.p);
+        c.precision == 1 ? 1 : 0, bits, D.out_idx.p, D.out_w.p, D.out_wx.p, tkey.p, twx.p);
     RMX_LAUNCH_CHECK();
   }
   D.out_ptr.alloc((int64_t)ns + 1);
@@ -294,7 +319,25 @@ void contract(Ctx &c) {
   kstart.free();
   usp_buf.free();
 
-  build_in_csr_weighted(c, D);   // transpose
+  // transpose: stable sort by head (the high half of the transposed key)
+  D.in_ptr.alloc((int64_t)ns + 1);
+  D.in_idx.alloc(m_super);
+  D.in_wx.alloc(m_super);
+  if (m_super > 0) {
+    uint64_t *sk = nullptr, *sw = nullptr;
+    radix_sort_pairs_u64_u64(tkey.p, twx.p, m_super, bits, 2 * bits, c.scratch, s, &sk, &sw);
+    k_in_from_keys<<<(unsigned)std::min<int64_t>(16384, (m_super + 255) / 256), 256, 0, s>>>(sk, sw, m_super, bits,
+                                                                                            D.in_idx.p, D.in_wx.p);
+    RMX_LAUNCH_CHECK();
+    k_in_ptr<<<(unsigned)std::min<int64_t>(16384, (m_super + 256) / 256), 256, 0, s>>>(sk, bits, m_super, (int32_t)ns,
+                                                                                      D.in_ptr.p);
+    RMX_LAUNCH_CHECK();
+  } else {
+    fill_u32(D.in_ptr.p, 0u, (int64_t)ns + 1, s);
+  }
+  c.sync();
+  tkey.free();
+  twx.free();
   build_buckets(c, D);
   c.sync();
   c.L1.swap(c.L2);

@@ -47,22 +47,6 @@ __global__ void k_arc_src(const uint32_t *__restrict__ ptr, int32_t n, uint32_t 
   for (uint32_t e = ptr[wave] + lane; e < ptr[wave + 1]; e += kWave) src[e] = (uint32_t)wave;
 }
 
-// after the stable sort, with the arc weights (contracted levels): every
-// in-arc gathers its source and its exact fixed-point weight from the out-arc
-// it mirrors (the sweeps read only those on a contracted level; the FP64
-// in-weights are derived on request, see super_in_weights)
-__global__ void k_in_gather(const uint32_t *__restrict__ sorted_e, const uint32_t *__restrict__ src, int64_t m,
-                            const long long *__restrict__ owx, uint32_t *__restrict__ in_idx,
-                            long long *__restrict__ iwx) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; k < m; k += stride) {
-    const uint32_t e = sorted_e[k];
-    in_idx[k] = src[e];
-    iwx[k] = owx[e];
-  }
-}
-
 // FP64 weight of every in-arc (v <- u): the weight of u's out-arc to v,
 // found by binary search in u's out-list (sorted by head)
 __global__ void k_in_w_lookup(const uint32_t *__restrict__ iptr, const uint32_t *__restrict__ iidx, int32_t n,
@@ -195,16 +179,14 @@ void upload_graph(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const in
 
 namespace {
 
-// in_of_out: the in-CSR position of every out-arc (level 0); weighted: the
-// in-arc weights are gathered from the out-arc weights (contracted levels)
-void transpose(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out, bool weighted) {
+// in_of_out: the in-CSR position of every out-arc
+void transpose(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) {
   hipStream_t s = c.stream;
   const int64_t m = L.m;
   const int32_t n = L.n;
   L.in_ptr.alloc(n + 1);
   L.in_idx.alloc(m);
   if (in_of_out) in_of_out->alloc(m);
-  if (weighted) L.in_wx.alloc(m);
   if (m == 0) {
     fill_u32(L.in_ptr.p, 0u, n + 1, s);
     c.sync();
@@ -222,11 +204,7 @@ void transpose(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out, bool weighted) {
     int bits = 1;
     while (bits < 32 && ((int64_t)1 << bits) < (int64_t)n) ++bits;
     radix_sort_pairs_u32(keys.p, vals.p, m, 0, bits, c.scratch, s);
-    if (weighted)
-      k_in_gather<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, m, L.out_wx.p, L.in_idx.p, L.in_wx.p);
-    else
-      k_in_from_sorted<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, m, L.in_idx.p,
-                                                  in_of_out ? in_of_out->p : nullptr);
+    k_in_from_sorted<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, m, L.in_idx.p, in_of_out ? in_of_out->p : nullptr);
     RMX_LAUNCH_CHECK();
     k_ptr_from_sorted<<<gs_grid(m + 1), 256, 0, s>>>(keys.p, m, n, L.in_ptr.p);
     RMX_LAUNCH_CHECK();
@@ -236,9 +214,7 @@ void transpose(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out, bool weighted) {
 
 }  // namespace
 
-void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) { transpose(c, L, in_of_out, false); }
-
-void build_in_csr_weighted(Ctx &c, Level &L) { transpose(c, L, nullptr, true); }
+void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) { transpose(c, L, in_of_out); }
 
 void super_in_weights(Ctx &c, const Level &L, double *in_w) {
   if (L.m == 0) return;

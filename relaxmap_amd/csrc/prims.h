@@ -26,6 +26,10 @@ void radix_sort_pairs_u32(uint32_t *keys, uint32_t *vals, int64_t n, int begin_b
                           Scratch &scratch, hipStream_t s);
 void radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit,
                           Scratch &scratch, hipStream_t s);
+// Leaves the result where the last pass wrote it (the inputs or the scratch,
+// valid until the scratch is used again): *out_keys / *out_vals.
+void radix_sort_pairs_u64_u64(uint64_t *keys, uint64_t *vals, int64_t n, int begin_bit, int end_bit,
+                              Scratch &scratch, hipStream_t s, uint64_t **out_keys, uint64_t **out_vals);
 
 void fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
 void fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s);

@@ -5,8 +5,7 @@
 // Radix sort: 8-bit digits; per pass an upsweep (per-tile digit histogram,
 // digit-major so that one exclusive scan yields every tile's scatter base)
 // and a downsweep that ranks each tile's elements stably with wave ballots
-// (equal-digit peers within a wave) and a per-digit running count across the
-// tile's 16 striped steps.
+// (equal-digit peers within a wave) and wave-private digit counts.
 #include "prims.h"
 #include "context.h"
 
@@ -166,46 +165,77 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_upsweep(const K *__restrict
   counts[(int64_t)threadIdx.x * n_tiles + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Stable scatter of one tile: every element's rank within its digit is found
-// with wavefront ballots (tile order), the tile is reordered by digit in LDS,
-// and each digit's run is then written to its global range with consecutive
-// lanes on consecutive addresses.
-template <typename K>
+// Stable scatter of one tile.  Wave w owns the tile's elements
+// [w * 1024, (w + 1) * 1024), taken 64 at a time in index order; it ranks
+// each element among the wave's earlier elements of the same digit with
+// ballots (equal-digit peers of a step) and a wave-private digit count in LDS,
+// so no workgroup barrier is needed while ranking.  One digit-major scan
+// over the per-wave counts then gives every (wave, digit) its first position,
+// the tile is reordered by digit in LDS, and each digit's run is written to
+// its global range with consecutive lanes on consecutive addresses.
+template <typename K, typename V>
 __global__ __launch_bounds__(kSortBlock) void k_sort_downsweep(const K *__restrict__ keys_in,
-                                                               const uint32_t *__restrict__ vals_in,
+                                                               const V *__restrict__ vals_in,
                                                                K *__restrict__ keys_out,
-                                                               uint32_t *__restrict__ vals_out, int64_t n,
+                                                               V *__restrict__ vals_out, int64_t n,
                                                                int shift, int64_t n_tiles,
                                                                const uint32_t *__restrict__ offsets) {
   constexpr int W = kSortBlock / kWave;
+  constexpr int PW = kSortTile / W;   // elements per wave
   __shared__ K s_key[kSortTile];
-  __shared__ uint32_t s_val[kSortTile];
+  __shared__ V s_val[kSortTile];
+  __shared__ uint32_t hist[W][kRadix];   // per-wave digit counts, then per-wave first positions
   __shared__ uint32_t tile_off[kRadix];
   __shared__ uint32_t digit_start[kRadix];
-  __shared__ uint32_t wave_cnt[W][kRadix];
-  __shared__ uint32_t wave_off[W][kRadix];
   __shared__ uint32_t wsum[W];
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
   const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  const int64_t wbase = base + (int64_t)w * PW;
   tile_off[t] = offsets[(int64_t)t * n_tiles + blockIdx.x];
-  digit_start[t] = 0;
 #pragma unroll
-  for (int i = 0; i < W; ++i) wave_cnt[i][t] = 0;
-  __syncthreads();
+  for (int q = 0; q < W; ++q) hist[q][t] = 0;
   K key[kSortItems];
-  uint32_t val[kSortItems];
+  V val[kSortItems];
 #pragma unroll
   for (int i = 0; i < kSortItems; ++i) {
-    const int64_t j = base + (int64_t)i * kSortBlock + t;
+    const int64_t j = wbase + (int64_t)i * kWave + lane;
     const bool valid = j < n;
     key[i] = valid ? keys_in[j] : K(0);
-    val[i] = valid ? vals_in[j] : 0u;
-    if (valid) atomicAdd(&digit_start[(uint32_t)(key[i] >> shift) & 0xffu], 1u);
+    val[i] = valid ? vals_in[j] : V(0);
   }
   __syncthreads();
-  {   // exclusive scan of the tile's digit counts (thread t holds digit t)
-    const uint32_t c = digit_start[t];
-    uint32_t x = c;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t rank[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const int64_t j = wbase + (int64_t)i * kWave + lane;
+    const bool valid = j < n;
+    const uint32_t d = (uint32_t)(key[i] >> shift) & 0xffu;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bal = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    const uint32_t r = (uint32_t)__popcll(peers & lt_mask);
+    const uint32_t before = hist[w][d];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();   // every lane has read its count before the leaders update it
+    if (valid && r == 0) hist[w][d] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    rank[i] = before + r;
+  }
+  __syncthreads();
+  {   // thread t holds digit t: its first position in the tile, then per wave
+    uint32_t c[W];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      c[q] = hist[q][t];
+      tot += c[q];
+    }
+    uint32_t x = tot;
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
       const uint32_t y = __shfl_up(x, off, kWave);
@@ -215,39 +245,21 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_downsweep(const K *__restri
     __syncthreads();
     uint32_t pre = 0;
     for (int q = 0; q < w; ++q) pre += wsum[q];
-    digit_start[t] = pre + x - c;
+    uint32_t run = pre + x - tot;
+    digit_start[t] = run;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      hist[q][t] = run;
+      run += c[q];
+    }
   }
   __syncthreads();
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t running = 0;   // digit t's count in earlier steps
 #pragma unroll
   for (int i = 0; i < kSortItems; ++i) {
-    const int64_t j = base + (int64_t)i * kSortBlock + t;
-    const bool valid = j < n;
-    const uint32_t d = (uint32_t)(key[i] >> shift) & 0xffu;
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint64_t bal = __ballot((d >> b) & 1u);
-      peers &= ((d >> b) & 1u) ? bal : ~bal;
-    }
-    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-    if (valid && rank == 0) wave_cnt[w][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    {
-      uint32_t run = running;
-#pragma unroll
-      for (int q = 0; q < W; ++q) {
-        const uint32_t cq = wave_cnt[q][t];
-        wave_off[q][t] = run;
-        run += cq;
-        wave_cnt[q][t] = 0;
-      }
-      running = run;
-    }
-    __syncthreads();
-    if (valid) {
-      const uint32_t p = digit_start[d] + wave_off[w][d] + rank;
+    const int64_t j = wbase + (int64_t)i * kWave + lane;
+    if (j < n) {
+      const uint32_t d = (uint32_t)(key[i] >> shift) & 0xffu;
+      const uint32_t p = hist[w][d] + rank[i];
       s_key[p] = key[i];
       s_val[p] = val[i];
     }
@@ -267,9 +279,14 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_downsweep(const K *__restri
   }
 }
 
-template <typename K>
-void radix_sort_impl(K *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit, Scratch &scratch,
-                     hipStream_t s) {
+// With out_k/out_v the result is left where the last pass wrote it (the
+// inputs or the scratch) and the pointers say where; otherwise it is copied
+// back into keys/vals.
+template <typename K, typename V>
+void radix_sort_impl(K *keys, V *vals, int64_t n, int begin_bit, int end_bit, Scratch &scratch, hipStream_t s,
+                     K **out_k = nullptr, V **out_v = nullptr) {
+  if (out_k) *out_k = keys;
+  if (out_v) *out_v = vals;
   if (n <= 1 || begin_bit >= end_bit) return;
   if (n > (int64_t)0xffffffffLL) throw HipError("radix sort: n >= 2^32");
   const int64_t n_tiles = (n + kSortTile - 1) / kSortTile;
@@ -278,30 +295,33 @@ void radix_sort_impl(K *keys, uint32_t *vals, int64_t n, int begin_bit, int end_
   auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t off_k = 0;
   size_t off_v = off_k + align(sizeof(K) * n);
-  size_t off_c = off_v + align(sizeof(uint32_t) * n);
+  size_t off_c = off_v + align(sizeof(V) * n);
   size_t off_t = off_c + align(sizeof(uint32_t) * n_counts);
   size_t total = off_t + align(sizeof(uint32_t) * scan_tmp);
   char *base = (char *)scratch.get(total);
   K *tk = (K *)(base + off_k);
-  uint32_t *tv = (uint32_t *)(base + off_v);
+  V *tv = (V *)(base + off_v);
   uint32_t *counts = (uint32_t *)(base + off_c);
   uint32_t *stmp = (uint32_t *)(base + off_t);
   K *src_k = keys, *dst_k = tk;
-  uint32_t *src_v = vals, *dst_v = tv;
+  V *src_v = vals, *dst_v = tv;
   int passes = 0;
   for (int shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
     k_sort_upsweep<K><<<(unsigned)n_tiles, kSortBlock, 0, s>>>(src_k, n, shift, n_tiles, counts);
     RMX_LAUNCH_CHECK();
     scan_impl(counts, counts, n_counts, nullptr, stmp, s);
-    k_sort_downsweep<K><<<(unsigned)n_tiles, kSortBlock, 0, s>>>(src_k, src_v, dst_k, dst_v, n, shift, n_tiles,
-                                                                 counts);
+    k_sort_downsweep<K, V><<<(unsigned)n_tiles, kSortBlock, 0, s>>>(src_k, src_v, dst_k, dst_v, n, shift, n_tiles,
+                                                                    counts);
     RMX_LAUNCH_CHECK();
     std::swap(src_k, dst_k);
     std::swap(src_v, dst_v);
   }
-  if (passes & 1) {
+  if (out_k) {
+    *out_k = src_k;
+    *out_v = src_v;
+  } else if (passes & 1) {
     RMX_HIP(hipMemcpyAsync(keys, src_k, sizeof(K) * n, hipMemcpyDeviceToDevice, s));
-    RMX_HIP(hipMemcpyAsync(vals, src_v, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+    RMX_HIP(hipMemcpyAsync(vals, src_v, sizeof(V) * n, hipMemcpyDeviceToDevice, s));
   }
 }
 
@@ -335,12 +355,17 @@ void exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *
 
 void radix_sort_pairs_u32(uint32_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit,
                           Scratch &scratch, hipStream_t s) {
-  radix_sort_impl<uint32_t>(keys, vals, n, begin_bit, end_bit, scratch, s);
+  radix_sort_impl<uint32_t, uint32_t>(keys, vals, n, begin_bit, end_bit, scratch, s);
 }
 
 void radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit,
                           Scratch &scratch, hipStream_t s) {
-  radix_sort_impl<uint64_t>(keys, vals, n, begin_bit, end_bit, scratch, s);
+  radix_sort_impl<uint64_t, uint32_t>(keys, vals, n, begin_bit, end_bit, scratch, s);
+}
+
+void radix_sort_pairs_u64_u64(uint64_t *keys, uint64_t *vals, int64_t n, int begin_bit, int end_bit,
+                              Scratch &scratch, hipStream_t s, uint64_t **out_keys, uint64_t **out_vals) {
+  radix_sort_impl<uint64_t, uint64_t>(keys, vals, n, begin_bit, end_bit, scratch, s, out_keys, out_vals);
 }
 
 void fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s) {
