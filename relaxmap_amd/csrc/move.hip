@@ -192,27 +192,51 @@ __device__ inline double delta_empty(const UnitCtx &U) {
   return plogp(U.S + U.d1 + q2) - U.phiS - 2.0 * plogp(q2) + plogp(q2 + fx_to(U.p)) + U.Kr;
 }
 
-__device__ inline void unit_ctx(const EvalArgs &a, uint32_t u, long long fo_o, long long fi_o, UnitCtx &U) {
-  U.o = a.mod[u];
-  const Stat4 su = a.us[u];
+// The unit's own reads, issued ahead of its arcs so that their latency
+// overlaps the arc gathers.
+struct UnitLoads {
+  uint32_t o;
+  Stat4 su, so;
+  int32_t units_o, stale_o;
+  double S;
+};
+
+__device__ inline UnitLoads load_unit(const EvalArgs &a, uint32_t u) {
+  UnitLoads l;
+  l.o = a.mod[u];
+  l.su = a.us[u];
+  l.S = a.S_inst ? a.S_inst[a.inst[u]] : a.scal[kS];
+  l.so = a.ms[l.o];
+  l.units_o = a.mUnits[l.o];
+  l.stale_o = a.node_level ? a.mStale[l.o] : 0;
+  return l;
+}
+
+__device__ inline void unit_ctx(const EvalArgs &a, const UnitLoads &l, long long fo_o, long long fi_o,
+                                UnitCtx &U) {
+  U.o = l.o;
+  const Stat4 su = l.su;
   U.p = su.p;
   const double pd = fx_to(su.p), dd = fx_to(su.d);
   U.td = fx_to(su.t);
   U.tele_out = kAlpha * pd + kBeta * dd;
   U.qu = module_exit(su.p, su.t, su.d, su.x);
-  const Stat4 so = a.ms[U.o];
+  const Stat4 so = l.so;
   const double qo = module_exit(so.p, so.t, so.d, so.x);
   const double outo = kBeta * fx_to(fo_o) + U.tele_out * fx_to(so.t - su.t);
   const double ino = kBeta * fx_to(fi_o) + (kAlpha * fx_to(so.p - su.p) + kBeta * fx_to(so.d - su.d)) * U.td;
   const double q1 = qo - U.qu + outo + ino;
-  const double S = a.S_inst ? a.S_inst[a.inst[u]] : a.scal[kS];
+  const double S = l.S;
   U.Kr = -2.0 * plogp(q1) + 2.0 * plogp(qo) + plogp(q1 + fx_to(so.p - su.p)) - plogp(qo + fx_to(so.p));
   U.S = S;
   U.phiS = plogp(S);
   U.d1 = q1 - qo;
-  const int32_t units_o = a.mUnits[U.o];
-  U.single = units_o == 1;
-  U.offer_empty = a.empty_avail && (a.node_level ? a.mStale[U.o] > 1 : units_o > 1);
+  U.single = l.units_o == 1;
+  U.offer_empty = a.empty_avail && (a.node_level ? l.stale_o > 1 : l.units_o > 1);
+}
+
+__device__ inline void unit_ctx(const EvalArgs &a, uint32_t u, long long fo_o, long long fi_o, UnitCtx &U) {
+  unit_ctx(a, load_unit(a, u), fo_o, fi_o, U);
 }
 
 __device__ inline bool allowed(const EvalArgs &a, const UnitCtx &U, uint32_t m) {
@@ -323,6 +347,86 @@ __device__ inline int lds_find(const uint32_t *keys, int H, uint32_t m) {
   }
 }
 
+// Fixed-point weight of out-arc (out) or in-arc k; both directions share the
+// weight mode of the level.
+__device__ inline long long arc_fxw(const EvalArgs &a, bool out, uint32_t k) {
+  if (a.aout.mode == 2) return *((out ? a.aout.wx : a.ain.wx) + k);
+  if (a.aout.mode == 1) return fx_of((double)*((out ? a.aout.wf : a.ain.wf) + k));
+  return fx_of(*((out ? a.aout.w : a.ain.w) + k));
+}
+
+// Aggregates arcs j = first, first + STEP, ... of a unit's out-list followed
+// by its in-list (ntot arcs, nout of them out-arcs) into the table.  Each
+// lane loads U arcs (their heads, modules and weights) before inserting any,
+// so that U gathers per lane are in flight at once.
+template <int U, int STEP>
+__device__ inline void gather_arcs(const EvalArgs &a, uint32_t ob, uint32_t nout, uint32_t ib, uint32_t ntot,
+                                   uint32_t first, uint32_t *keys, int H, long long *fo, long long *fi) {
+  for (uint32_t j0 = first; j0 < ntot; j0 += (uint32_t)(U * STEP)) {
+    uint32_t v[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const uint32_t j = j0 + (uint32_t)(q * STEP);
+      const uint32_t *p = j < nout ? a.oidx + ob + j : a.iidx + ib + (j - nout);
+      v[q] = j < ntot ? *p : 0u;
+    }
+    uint32_t m[U];
+    long long w[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const uint32_t j = j0 + (uint32_t)(q * STEP);
+      const bool out = j < nout;
+      m[q] = j < ntot ? a.mod[v[q]] : 0u;
+      w[q] = j < ntot ? arc_fxw(a, out, out ? ob + j : ib + (j - nout)) : 0ll;
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const uint32_t j = j0 + (uint32_t)(q * STEP);
+      if (j < ntot) {
+        const uint32_t sl = lds_insert(keys, H, m[q]);
+        atomicAdd((unsigned long long *)(j < nout ? &fo[sl] : &fi[sl]), (unsigned long long)w[q]);
+      }
+    }
+  }
+}
+
+// Best candidate among table slots slot(first), slot(first + STEP), ...
+// below n; UC candidates per lane have their module statistics loaded
+// together.
+template <int UC, int STEP, bool F32, typename Slot>
+__device__ inline void eval_cands(const EvalArgs &a, const UnitCtx &U, const uint32_t *keys, const long long *fo,
+                                  const long long *fi, int first, int n, Slot slot, Best &b) {
+  for (int j0 = first; j0 < n; j0 += UC * STEP) {
+    uint32_t mm[UC];
+    int sl[UC];
+#pragma unroll
+    for (int q = 0; q < UC; ++q) {
+      const int j = j0 + q * STEP;
+      sl[q] = slot(j < n ? j : first);
+      mm[q] = j < n ? keys[sl[q]] : U.o;
+    }
+    Stat4 M[UC];
+    int32_t mu[UC];
+#pragma unroll
+    for (int q = 0; q < UC; ++q) {
+      const bool c = mm[q] != U.o;
+      M[q] = c ? a.ms[mm[q]] : Stat4{0, 0, 0, 0};
+      mu[q] = (c && U.single && mm[q] > U.o) ? a.mUnits[mm[q]] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < UC; ++q) {
+      // a singleton may join another singleton only if that module's id is smaller
+      if (mm[q] == U.o || mu[q] == 1) continue;
+      const double dl = cand_delta<F32>(U, M[q], fo[sl[q]], fi[sl[q]]);
+      if (better(dl, mm[q], b) && dl < 0.0) {
+        b.d = dl;
+        b.m = mm[q];
+        b.i = sl[q];
+      }
+    }
+  }
+}
+
 // G lanes per unit (G <= 64: the group lives inside one wavefront), H keys.
 // After the flows are aggregated, the non-empty slots are compacted in place
 // to the front of the table (a slot only ever moves down, and every lane
@@ -344,12 +448,15 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
   bool work = false;
   uint32_t u = 0;
   if (g < a.list_n) {
-    u = a.bucket[a.list[g]];
+    u = a.list[g];   // the batch list holds the units themselves below the hub bucket
     work = true;
   }
   Best b{0.0, kNone, -1};
   long long fo_o = 0, fi_o = 0, fo_n = 0, fi_n = 0;
   if (work) {
+    const uint32_t ob = a.optr[u], oe = a.optr[u + 1];
+    const uint32_t ib = a.iptr[u], ie = a.iptr[u + 1];
+    const UnitLoads ul = load_unit(a, u);
     for (int s = lane; s < H; s += G) {
       keys[s] = kEmptyKey;
       fo[s] = 0;
@@ -357,22 +464,11 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    const uint32_t ob = a.optr[u], oe = a.optr[u + 1];
-    const uint32_t ib = a.iptr[u], ie = a.iptr[u + 1];
-    for (uint32_t k = ob + lane; k < oe; k += G) {
-      uint32_t m = a.mod[a.oidx[k]];
-      uint32_t s = lds_insert(keys, H, m);
-      atomicAdd((unsigned long long *)&fo[s], (unsigned long long)a.aout.fxw(k));
-    }
-    for (uint32_t k = ib + lane; k < ie; k += G) {
-      uint32_t m = a.mod[a.iidx[k]];
-      uint32_t s = lds_insert(keys, H, m);
-      atomicAdd((unsigned long long *)&fi[s], (unsigned long long)a.ain.fxw(k));
-    }
+    gather_arcs<(G >= 32 ? 4 : 2), G>(a, ob, oe - ob, ib, (oe - ob) + (ie - ib), (uint32_t)lane, keys, H, fo, fi);
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) visits = (oe - ob) + (ie - ib);
-    const uint32_t o = a.mod[u];
+    const uint32_t o = ul.o;
     const int so = lds_find(keys, H, o);
     if (so >= 0) {
       fo_o = fo[so];
@@ -402,14 +498,8 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
     }
     if (lane == 0) ncand = (unsigned long long)nc;
     UnitCtx U;
-    unit_ctx(a, u, fo_o, fi_o, U);
-    for (int i = lane; i < nc; i += G) {
-      const uint32_t m = keys[i];
-      if (m == o) continue;
-      if (!allowed(a, U, m)) continue;
-      const double dl = cand_delta<F32>(U, a.ms[m], fo[i], fi[i]);
-      if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; b.i = i; }
-    }
+    unit_ctx(a, ul, fo_o, fi_o, U);
+    eval_cands<(G >= 32 ? 2 : 1), G, F32>(a, U, keys, fo, fi, lane, nc, [](int j) { return j; }, b);
     if (lane == 0 && U.offer_empty) {
       const double dl = delta_empty(U);
       if (better(dl, a.empty_key, b) && dl < 0.0) { b.d = dl; b.m = a.empty_key; b.i = -1; }
@@ -470,10 +560,10 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
   const int wl = threadIdx.x & (kWave - 1);
   unsigned long long vis_acc = 0, cand_acc = 0;   // thread 0
   for (int64_t g = blockIdx.x; g < a.list_n; g += gridDim.x) {
-    const uint32_t pos = a.list[g];
-    const uint32_t u = a.bucket[pos];
+    const uint32_t u = a.list[g];   // the batch list holds the units themselves below the hub bucket
     const uint32_t ob = a.optr[u], oe = a.optr[u + 1];
     const uint32_t ib = a.iptr[u], ie = a.iptr[u + 1];
+    const UnitLoads ul = load_unit(a, u);
     int cap = 256;
     while (cap < 2 * (int)((oe - ob) + (ie - ib)) + 2) cap <<= 1;
     if (cap > H) cap = H;
@@ -485,22 +575,13 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
       fi[s] = 0;
     }
     __syncthreads();
-    for (uint32_t k = ob + threadIdx.x; k < oe; k += 256) {
-      uint32_t m = a.mod[a.oidx[k]];
-      uint32_t s = lds_insert(keys, cap, m);
-      atomicAdd((unsigned long long *)&fo[s], (unsigned long long)a.aout.fxw(k));
-    }
-    for (uint32_t k = ib + threadIdx.x; k < ie; k += 256) {
-      uint32_t m = a.mod[a.iidx[k]];
-      uint32_t s = lds_insert(keys, cap, m);
-      atomicAdd((unsigned long long *)&fi[s], (unsigned long long)a.ain.fxw(k));
-    }
+    gather_arcs<4, 256>(a, ob, oe - ob, ib, (oe - ob) + (ie - ib), threadIdx.x, keys, cap, fo, fi);
     __syncthreads();
     if (threadIdx.x == 0) {
       vis_acc += (oe - ob) + (ie - ib);
       add_units(a, 1u);
     }
-    const uint32_t o = a.mod[u];
+    const uint32_t o = ul.o;
     long long fo_o = 0, fi_o = 0;
     {
       int so = lds_find(keys, cap, o);
@@ -530,16 +611,9 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
       __builtin_amdgcn_wave_barrier();
     }
     UnitCtx U;
-    unit_ctx(a, u, fo_o, fi_o, U);
+    unit_ctx(a, ul, fo_o, fi_o, U);
     Best b{0.0, kNone, -1};
-    for (int j = wl; j < nw; j += kWave) {
-      const int i = wave_slot(j, w);
-      const uint32_t m = keys[i];
-      if (m == o) continue;
-      if (!allowed(a, U, m)) continue;
-      const double dl = cand_delta<F32>(U, a.ms[m], fo[i], fi[i]);
-      if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; b.i = i; }
-    }
+    eval_cands<2, kWave, F32>(a, U, keys, fo, fi, wl, nw, [w](int j) { return wave_slot(j, w); }, b);
     if (threadIdx.x == 0 && U.offer_empty) {
       const double dl = delta_empty(U);
       if (better(dl, a.empty_key, b) && dl < 0.0) { b.d = dl; b.m = a.empty_key; b.i = -1; }
@@ -1401,7 +1475,7 @@ __global__ void k_batch_keys(const uint32_t *__restrict__ bucket, int32_t n, int
   const uint32_t u = bucket[i];
   const uint32_t k = (all_active || act[u]) ? batch_of(seed, batches, u) : (uint32_t)batches;
   keys[i] = (uint32_t)b * (uint32_t)(batches + 1) + k;
-  vals[i] = (uint32_t)i;
+  vals[i] = b + 1 < kNumBuckets ? u : (uint32_t)i;   // hubs keep their bucket position (their table region)
 }
 
 __global__ void k_hist256(const uint32_t *__restrict__ key, int64_t n, uint32_t *__restrict__ cnt) {
