@@ -13,9 +13,13 @@ namespace rmx {
 // Degree buckets of the move sweep: units are evaluated by groups of G
 // lanes whose LDS hash table holds H keys; the last bucket (hubs) uses one
 // workgroup per unit with its table in global memory.
-constexpr int kNumBuckets = 6;
-// upper bound of (out + in) degree per bucket
-constexpr int kBucketMaxDeg[kNumBuckets] = {16, 32, 128, 512, 2048, 0x7fffffff};
+constexpr int kNumBuckets = 8;
+// upper bound of (out + in) degree per bucket; a table holds at least twice
+// the degree, so the cut-offs follow the table sizes (and with them the LDS
+// per workgroup, which sets how many workgroups share a CU)
+constexpr int kBucketMaxDeg[kNumBuckets] = {16, 32, 128, 256, 512, 1023, 2048, 0x7fffffff};
+// batches per sweep: (bucket, batch) sort keys must fit one 8-bit digit
+constexpr int kMaxBatches = 256 / kNumBuckets - 1;
 // Hubs are evaluated in partitions of about kHubPart arcs (split by a hash of
 // the neighbour's module), each aggregated in an LDS table.
 constexpr int kHubPart = 1024;

@@ -26,8 +26,10 @@
 // Neighbour-module flows are aggregated in an LDS hash table per unit with
 // 64-bit integer atomics (exact, order independent).  Units are bucketed by
 // degree: groups of 8..64 lanes per unit with LDS tables of 32..1024 keys,
-// a 256-lane workgroup with a 4096-key LDS table, and for hubs a workgroup
-// with its table in global memory.
+// a 256-lane workgroup with a 2048- or 4096-key LDS table, and hubs split by
+// neighbour module into partitions of about kHubPart arcs, each aggregated
+// in an LDS table (hubs whose partition overflows fall back to global-memory
+// tables).
 #include "devutil.h"
 
 #include <algorithm>
@@ -1588,9 +1590,10 @@ void fold_counters(Ctx &c, hipStream_t s) {
 static void profile_eval(Ctx &c, int bucket, hipStream_t s) {
   // entries carry the kernels' own names (both levels together); the hub
   // bucket is a sequence of kernels and is marked as such
-  static const char *names[kNumBuckets] = {"k_eval_group<8, 32>", "k_eval_group<16, 64>", "k_eval_group<32, 256>",
-                                           "k_eval_group<64, 1024>", "k_eval_block<4096>",
-                                           "hub sequence: k_hubp_plan..k_hubp_finish"};
+  static const char *names[kNumBuckets] = {"k_eval_group<8, 32>",    "k_eval_group<16, 64>",
+                                           "k_eval_group<32, 256>",  "k_eval_group<64, 512>",
+                                           "k_eval_group<64, 1024>", "k_eval_block<2048>",
+                                           "k_eval_block<4096>",     "hub sequence: k_hubp_plan..k_hubp_finish"};
   const float ms = c.prof.stop(s);   // end event right after the launch
   uint32_t before[16];
   std::memcpy(before, c.h_cnt, sizeof(before));
@@ -1715,7 +1718,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   {
     const int *o = L.bucket_off;
     k_batch_keys<<<grid_for(n, 256), 256, 0, s>>>(L.bucket.p, n, make_int4(o[0], o[1], o[2], o[3]),
-                                                  make_int4(o[4], o[5], o[6], o[6]), c.act.p, prioritized ? 0 : 1,
+                                                  make_int4(o[4], o[5], o[6], o[7]), c.act.p, prioritized ? 0 : 1,
                                                   sweep_seed, B, c.batch_keys.p, c.batch_list.p);
     RMX_LAUNCH_CHECK();
     radix_sort_pairs_u32(c.batch_keys.p, c.batch_list.p, n, 0, 8, c.scratch2, s);
@@ -1795,10 +1798,18 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
           else k_eval_group<32, 256, false><<<(unsigned)((cnt + 7) / 8), 256, 0, s>>>(a);
           break;
         case 3:
+          if (f32) k_eval_group<64, 512, true><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
+          else k_eval_group<64, 512, false><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
+          break;
+        case 4:
           if (f32) k_eval_group<64, 1024, true><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
           else k_eval_group<64, 1024, false><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
           break;
-        case 4:
+        case 5:
+          if (f32) k_eval_block<2048, true><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
+          else k_eval_block<2048, false><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
+          break;
+        case 6:
           if (f32) k_eval_block<4096, true><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
           else k_eval_block<4096, false><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
           break;
