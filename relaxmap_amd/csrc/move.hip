@@ -1590,10 +1590,14 @@ void fold_counters(Ctx &c, hipStream_t s) {
 static void profile_eval(Ctx &c, int bucket, hipStream_t s) {
   // entries carry the kernels' own names (both levels together); the hub
   // bucket is a sequence of kernels and is marked as such
-  static const char *names[kNumBuckets] = {"k_eval_group<8, 32>",    "k_eval_group<16, 64>",
-                                           "k_eval_group<32, 256>",  "k_eval_group<64, 512>",
-                                           "k_eval_group<64, 1024>", "k_eval_block<2048>",
-                                           "k_eval_block<4096>",     "hub sequence: k_hubp_plan..k_hubp_finish"};
+  // the kernels' own names, with the precision template argument
+  static const char *names[2][kNumBuckets] = {
+      {"k_eval_group<8, 32, false>", "k_eval_group<16, 64, false>", "k_eval_group<32, 256, false>",
+       "k_eval_group<64, 512, false>", "k_eval_group<64, 1024, false>", "k_eval_block<2048, false>",
+       "k_eval_block<4096, false>", "hub sequence: k_hubp_plan..k_hubp_finish"},
+      {"k_eval_group<8, 32, true>", "k_eval_group<16, 64, true>", "k_eval_group<32, 256, true>",
+       "k_eval_group<64, 512, true>", "k_eval_group<64, 1024, true>", "k_eval_block<2048, true>",
+       "k_eval_block<4096, true>", "hub sequence: k_hubp_plan..k_hubp_finish"}};
   const float ms = c.prof.stop(s);   // end event right after the launch
   uint32_t before[16];
   std::memcpy(before, c.h_cnt, sizeof(before));
@@ -1605,7 +1609,7 @@ static void profile_eval(Ctx &c, int bucket, hipStream_t s) {
   const double cands = u64(c.h_cnt, cCandsLo) - u64(before, cCandsLo);
   const double units = (double)c.h_cnt[cUnits] - (double)before[cUnits];
   const double wbytes = (c.level == 0 && c.precision == 1) ? 4.0 : 8.0;
-  KernelStat &k = c.prof.get(names[bucket]);
+  KernelStat &k = c.prof.get(names[c.precision == 1 ? 1 : 0][bucket]);
   k.launches += 1;
   k.ms += ms;
   k.bytes += visits * (8.0 + wbytes) + units * 48.0 + cands * 32.0;
