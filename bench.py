@@ -172,24 +172,36 @@ def measured_traffic(kernel):
 
 def config4(args):
     """64 directed SBMs (500-50k nodes, 10-200 blocks, Poisson(12), mu in
-    [0.1, 0.5]) x 8 seeds = 512 independent instances, partitioned by
-    concurrent contexts: instances/s and the per-network codelength spread
-    over seeds."""
+    [0.1, 0.5]) x 8 seeds = 512 independent instances: instances/s and the
+    per-network codelength spread over seeds.  Primary: one batch of all 512
+    instances (rmx_batch_upload); beside it, concurrent contexts taking the
+    networks one by one."""
     from relaxmap_amd import synth
-    from relaxmap_amd.batch import run_instances
+    from relaxmap_amd.batch import run_batched, run_instances
     graphs = synth.config4_graphs()
     seeds = list(range(1, 9))
-    run_instances(graphs[:4], seeds[:1], workers=args.workers)   # warm-up
-    res, dt = run_instances(graphs, seeds, workers=args.workers)
-    spread = []
-    for gi in range(len(graphs)):
-        L = np.array([res[(gi, s)][0] for s in seeds])
-        spread.append(float((L.max() - L.min()) / L.min()))
-    out = {"instances": len(res), "networks": len(graphs), "seeds": len(seeds), "workers": args.workers,
+
+    def spread_of(res):
+        spread = []
+        for gi in range(len(graphs)):
+            L = np.array([res[(gi, s)][0] for s in seeds])
+            spread.append(float((L.max() - L.min()) / L.min()))
+        return {"median": float(np.median(spread)), "max": float(np.max(spread))}
+
+    run_batched(graphs[:4], seeds[:2])   # warm-up
+    res, dt = run_batched(graphs, seeds)
+    out = {"instances": len(res), "networks": len(graphs), "seeds": len(seeds), "mode": "one batch",
            "nodes": int(sum(g.n for g in graphs)), "arcs": int(sum(g.m for g in graphs)), "seconds": dt,
-           "instances_per_s": len(res) / dt, "codelength_rel_spread_over_seeds": {
-               "median": float(np.median(spread)), "max": float(np.max(spread))}}
-    log(f"config 4: {len(res)} instances in {dt:.2f} s ({len(res) / dt:.1f} instances/s)")
+           "instances_per_s": len(res) / dt, "codelength_rel_spread_over_seeds": spread_of(res)}
+    log(f"config 4: {len(res)} instances as one batch in {dt:.2f} s ({len(res) / dt:.1f} instances/s)")
+    run_instances(graphs[:4], seeds[:1], workers=args.workers)   # warm-up
+    res_c, dt_c = run_instances(graphs, seeds, workers=args.workers)
+    out["contexts"] = {"workers": args.workers, "seconds": dt_c, "instances_per_s": len(res_c) / dt_c,
+                       "codelength_rel_spread_over_seeds": spread_of(res_c)}
+    rel = [abs(res[k][0] - res_c[k][0]) / res_c[k][0] for k in res]
+    out["codelength_rel_batch_vs_contexts"] = {"median": float(np.median(rel)), "max": float(np.max(rel))}
+    log(f"config 4: {len(res_c)} instances by {args.workers} contexts in {dt_c:.2f} s "
+        f"({len(res_c) / dt_c:.1f} instances/s)")
     return out
 
 

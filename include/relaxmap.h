@@ -67,6 +67,23 @@ int rmx_graph_upload(rmx_ctx *ctx, int32_t n, int64_t m, const int64_t *out_ptr,
 int rmx_load_network(rmx_ctx *ctx, const char *path, int32_t *n, int64_t *m, int32_t *n_self_links,
                      int32_t *n_double_links);
 
+/* A batch of independent networks (many graphs, or one graph under many
+ * seeds) partitioned together: the union of the networks' out-CSRs, where
+ * instance I owns the node ids [node_off[I], node_off[I+1]) and no arc joins
+ * two instances.  seeds[I] drives the sweeps of instance I.  After
+ * rmx_initiate (steady state of every instance on its own, with the stop
+ * rule applied per instance) rmx_partition runs main()'s schedule
+ * (OmpRelaxmap.cpp:181-219) for every instance at once, each with its own
+ * codelength and stop decisions; modules never leave their instance.  No
+ * reference counterpart: the reference runs one network per process.
+ * attempts > 1 and rmx_calculate_codelength are refused on a batch. */
+int rmx_batch_upload(rmx_ctx *ctx, int32_t n_inst, const int32_t *node_off, const uint64_t *seeds, int32_t n,
+                     int64_t m, const int64_t *out_ptr, const int32_t *out_idx, const double *w,
+                     const double *node_w);
+/* Per instance: codelength (nats) and non-empty modules of the current
+ * partition, and the steady-state iterations. */
+int rmx_batch_results(rmx_ctx *ctx, double *codelength, int32_t *n_module, int32_t *iterations);
+
 /* Host-side network files, no GPU needed.  rmx_net_read parses a .net or
  * .txt file with the reference's rules (FileIO.cpp:32-265, then the
  * self-loop and w <= 0 removal of OmpRelaxmap.cpp:117-135) on `threads` host

@@ -84,6 +84,9 @@ def load_library():
         "rmx_get_sub_modules": (C.c_int, [vp, _I32P]),
         "rmx_contract_sub_modules": (C.c_int, [vp, _I32P, _I64P]),
         "rmx_graph_upload": (C.c_int, [vp, C.c_int32, C.c_int64, _I64P, _I32P, _F64P, _F64P]),
+        "rmx_batch_upload": (C.c_int, [vp, C.c_int32, _I32P, C.POINTER(C.c_uint64), C.c_int32, C.c_int64, _I64P,
+                                       _I32P, _F64P, _F64P]),
+        "rmx_batch_results": (C.c_int, [vp, _F64P, _I32P, _I32P]),
         "rmx_load_network": (C.c_int, [vp, C.c_char_p, _I32P, _I64P, _I32P, _I32P]),
         "rmx_initiate": (C.c_int, [vp, _I32P, _F64P, _I64P]),
         "rmx_get_flows": (C.c_int, [vp, _F64P]),
@@ -221,6 +224,35 @@ class Network:
         self.n, self.m = n.value, m.value
         self.n_self_links, self.n_double_links = ns.value, nd.value
         return self
+
+    def batch_upload(self, node_off, seeds, out_ptr, out_idx, w, node_w=None):
+        """A batch of independent networks: the union out-CSR, instance I owning
+        nodes [node_off[I], node_off[I+1]); seeds[I] drives its sweeps."""
+        node_off = np.ascontiguousarray(node_off, dtype=np.int32)
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        out_ptr = np.ascontiguousarray(out_ptr, dtype=np.int64)
+        out_idx = np.ascontiguousarray(out_idx, dtype=np.int32)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        nwp = None
+        if node_w is not None:
+            node_w = np.ascontiguousarray(node_w, dtype=np.float64)
+            nwp = _p(node_w, _F64P)
+        n = len(out_ptr) - 1
+        self._check(self._L.rmx_batch_upload(self._h, len(node_off) - 1, _p(node_off, _I32P),
+                                             _p(seeds, C.POINTER(C.c_uint64)), int(n), len(out_idx),
+                                             _p(out_ptr, _I64P), _p(out_idx, _I32P), _p(w, _F64P), nwp))
+        self.n, self.m = int(n), len(out_idx)
+        self.n_inst = len(node_off) - 1
+        return self
+
+    def batch_results(self):
+        """Per instance: codelength (nats), non-empty modules, steady-state iterations."""
+        k = self.n_inst
+        L = np.empty(k, dtype=np.float64)
+        nm = np.empty(k, dtype=np.int32)
+        it = np.empty(k, dtype=np.int32)
+        self._check(self._L.rmx_batch_results(self._h, _p(L, _F64P), _p(nm, _I32P), _p(it, _I32P)))
+        return L, nm, it
 
     def upload(self, n, out_ptr, out_idx, w, node_w=None):
         out_ptr = np.ascontiguousarray(out_ptr, dtype=np.int64)

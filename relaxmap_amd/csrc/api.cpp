@@ -19,6 +19,10 @@
 
 using namespace rmx;
 
+namespace rmx {
+void batch_partition(Ctx &c, const rmx_partition_opts &o, rmx_partition_result &r);   // batch.hip
+}
+
 struct rmx_ctx {
   Ctx c;
   std::vector<std::string> names;
@@ -57,7 +61,10 @@ void free_ctx_device(Ctx &c) {
   c.Lsub.free_all();
   c.node_w.free(); c.raw_w.free(); c.dangling.free(); c.in_of_out.free(); c.node_sp.free();
   c.ms.free(); c.mNodes.free(); c.mUnits.free(); c.mStale.free();
-  c.mInst.free();
+  c.b_off.free(); c.b_node.free(); c.b_seed.free(); c.b_S.free(); c.b_L.free(); c.b_C.free();
+  c.b_Lold.free(); c.b_Lstep.free(); c.b_Louter.free(); c.b_nmod.free(); c.b_D.free(); c.b_R.free();
+  c.b_iters.free(); c.b_step.free(); c.b_outer.free(); c.b_inner.free();
+  c.b_ebeg.free(); c.b_ecnt.free(); c.b_ecur.free(); c.b_count.free();
   c.act.free(); c.act_next.free(); c.target.free(); c.newmod.free(); c.stamp.free(); c.movers.free(); c.movers_st.free();
   c.empty_list.free(); c.want_empty.free(); c.rank_tmp.free(); c.agg_buf.free();
   c.batch_keys.free(); c.batch_list.free();
@@ -143,28 +150,108 @@ int rmx_set_hub_path(rmx_ctx *ctx, int global_tables) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// the out-CSR rules of rmx_graph_upload; "" when they hold
+const char *check_csr(int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx, const double *w) {
+  if (n <= 0 || m < 0 || !out_ptr || (m > 0 && (!out_idx || !w))) return "empty network or null arrays";
+  if (out_ptr[0] != 0 || out_ptr[n] != m) return "out_ptr does not span the arcs";
+  for (int32_t i = 0; i < n; ++i) {
+    if (out_ptr[i + 1] < out_ptr[i]) return "out_ptr not monotone";
+    for (int64_t k = out_ptr[i]; k < out_ptr[i + 1]; ++k) {
+      int32_t d = out_idx[k];
+      if (d < 0 || d >= n) return "arc head out of range";
+      if (d == i) return "self-loop in arc list";
+      if (k > out_ptr[i] && out_idx[k - 1] >= d) return "arcs not sorted or duplicated";
+      if (!(w[k] > 0.0) || !std::isfinite(w[k])) return "arc weight not positive";
+    }
+  }
+  return "";
+}
+
+void set_names(rmx_ctx *ctx, int32_t n) {
+  if ((int32_t)ctx->names.size() != n) {
+    ctx->names.resize(n);
+    for (int32_t i = 0; i < n; ++i) ctx->names[i] = std::to_string(i + 1);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
 int rmx_graph_upload(rmx_ctx *ctx, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx,
                      const double *w, const double *node_w) {
   return guarded(ctx, [&] {
-    if (n <= 0 || m < 0 || !out_ptr || (m > 0 && (!out_idx || !w)))
-      return fail(ctx, RMX_EINVAL, "empty network or null arrays");
-    if (out_ptr[0] != 0 || out_ptr[n] != m) return fail(ctx, RMX_EINVAL, "out_ptr does not span the arcs");
-    for (int32_t i = 0; i < n; ++i) {
-      if (out_ptr[i + 1] < out_ptr[i]) return fail(ctx, RMX_EINVAL, "out_ptr not monotone");
-      for (int64_t k = out_ptr[i]; k < out_ptr[i + 1]; ++k) {
-        int32_t d = out_idx[k];
-        if (d < 0 || d >= n) return fail(ctx, RMX_EINVAL, "arc head out of range");
-        if (d == i) return fail(ctx, RMX_EINVAL, "self-loop in arc list");
-        if (k > out_ptr[i] && out_idx[k - 1] >= d) return fail(ctx, RMX_EINVAL, "arcs not sorted or duplicated");
-        if (!(w[k] > 0.0) || !std::isfinite(w[k])) return fail(ctx, RMX_EINVAL, "arc weight not positive");
+    const char *bad = check_csr(n, m, out_ptr, out_idx, w);
+    if (*bad) return fail(ctx, RMX_EINVAL, bad);
+    RMX_HIP(hipSetDevice(ctx->c.device));
+    ctx->c.nb = 0;
+    upload_graph(ctx->c, n, m, out_ptr, out_idx, w, node_w);
+    set_names(ctx, n);
+    return RMX_OK;
+  });
+}
+
+int rmx_batch_upload(rmx_ctx *ctx, int32_t n_inst, const int32_t *node_off, const uint64_t *seeds, int32_t n,
+                     int64_t m, const int64_t *out_ptr, const int32_t *out_idx, const double *w,
+                     const double *node_w) {
+  return guarded(ctx, [&] {
+    if (n_inst < 1 || !node_off || !seeds) return fail(ctx, RMX_EINVAL, "no instances or null arrays");
+    const char *bad = check_csr(n, m, out_ptr, out_idx, w);
+    if (*bad) return fail(ctx, RMX_EINVAL, bad);
+    if (node_off[0] != 0 || node_off[n_inst] != n) return fail(ctx, RMX_EINVAL, "node_off does not span the nodes");
+    std::vector<uint32_t> inst((size_t)n);
+    for (int32_t I = 0; I < n_inst; ++I) {
+      if (node_off[I + 1] < node_off[I]) return fail(ctx, RMX_EINVAL, "node_off not monotone");
+      for (int32_t i = node_off[I]; i < node_off[I + 1]; ++i) {
+        inst[i] = (uint32_t)I;
+        for (int64_t k = out_ptr[i]; k < out_ptr[i + 1]; ++k)
+          if (out_idx[k] < node_off[I] || out_idx[k] >= node_off[I + 1])
+            return fail(ctx, RMX_EINVAL, "an arc joins two instances");
       }
     }
-    RMX_HIP(hipSetDevice(ctx->c.device));
-    upload_graph(ctx->c, n, m, out_ptr, out_idx, w, node_w);
-    if ((int32_t)ctx->names.size() != n) {
-      ctx->names.resize(n);
-      for (int32_t i = 0; i < n; ++i) ctx->names[i] = std::to_string(i + 1);
-    }
+    Ctx &c = ctx->c;
+    RMX_HIP(hipSetDevice(c.device));
+    c.nb = 0;
+    upload_graph(c, n, m, out_ptr, out_idx, w, node_w);
+    c.nb = n_inst;
+    c.h_boff.assign(node_off, node_off + n_inst + 1);
+    std::vector<uint64_t> sd(n_inst);
+    for (int32_t I = 0; I < n_inst; ++I) sd[I] = mix64(seeds[I] ^ 0x6a09e667f3bcc909ull);
+    c.b_off.alloc((int64_t)n_inst + 1);
+    c.b_node.alloc(n);
+    c.b_seed.alloc(n_inst);
+    c.L0.uoff.alloc((int64_t)n_inst + 1);
+    c.L0.uinst.alloc(n);
+    RMX_HIP(hipMemcpyAsync(c.b_off.p, c.h_boff.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice,
+                           c.stream));
+    RMX_HIP(hipMemcpyAsync(c.L0.uoff.p, c.h_boff.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice,
+                           c.stream));
+    RMX_HIP(hipMemcpyAsync(c.b_node.p, inst.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
+    RMX_HIP(hipMemcpyAsync(c.L0.uinst.p, inst.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
+    RMX_HIP(hipMemcpyAsync(c.b_seed.p, sd.data(), sizeof(uint64_t) * n_inst, hipMemcpyHostToDevice, c.stream));
+    c.sync();   // host staging vectors go out of scope
+    set_names(ctx, n);
+    return RMX_OK;
+  });
+}
+
+int rmx_batch_results(rmx_ctx *ctx, double *codelength, int32_t *n_module, int32_t *iterations) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (c.nb <= 0) return fail(ctx, RMX_ESTATE, "no batch uploaded");
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    module_totals(c, &c.sum_exit, &c.codelength);   // per instance, current partition
+    if (codelength)
+      RMX_HIP(hipMemcpyAsync(codelength, c.b_L.p, sizeof(double) * c.nb, hipMemcpyDeviceToHost, c.stream));
+    if (n_module)
+      RMX_HIP(hipMemcpyAsync(n_module, c.b_nmod.p, sizeof(int32_t) * c.nb, hipMemcpyDeviceToHost, c.stream));
+    if (iterations)
+      RMX_HIP(hipMemcpyAsync(iterations, c.b_iters.p, sizeof(int32_t) * c.nb, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
     return RMX_OK;
   });
 }
@@ -273,6 +360,7 @@ int rmx_initiate(rmx_ctx *ctx, int32_t *iterations, double *codelength, int64_t 
     steady_state(c, 200, 50, 1.0e-15);
     init_modules_singletons(c);
     module_totals(c, &c.sum_exit, &c.codelength);
+    if (c.nb > 0) c.initial_codelength = c.codelength;   // a batch: the sum over its instances
     ensure_sweep_buffers(c);
     reset_active(c);
     c.initiated = true;
@@ -317,6 +405,7 @@ int rmx_codelength(rmx_ctx *ctx, double *codelength, double *sum_exit, int32_t *
 int rmx_calculate_codelength(rmx_ctx *ctx, double *codelength) {
   return guarded(ctx, [&] {
     Ctx &c = ctx->c;
+    if (c.nb > 0) return fail(ctx, RMX_ESTATE, "a batch has one codelength per instance: rmx_batch_results");
     if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
     propagate_labels(c);
     *codelength = calculate_codelength(c);
@@ -330,6 +419,10 @@ int rmx_set_partition(rmx_ctx *ctx, const int32_t *part) {
     if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
     for (int32_t i = 0; i < c.N; ++i)
       if (part[i] < 0 || part[i] >= c.N) return fail(ctx, RMX_EINVAL, "module index out of range");
+    for (int32_t I = 0; I < c.nb; ++I)   // a batch: every module inside its instance's node ids
+      for (uint32_t i = c.h_boff[I]; i < c.h_boff[I + 1]; ++i)
+        if ((uint32_t)part[i] < c.h_boff[I] || (uint32_t)part[i] >= c.h_boff[I + 1])
+          return fail(ctx, RMX_EINVAL, "a module joins two instances");
     c.level = 0;
     c.sweep_counter = 0;   // runs from an installed partition repeat exactly
     DevBuf<uint32_t> d(c.cache);
@@ -635,6 +728,11 @@ extern "C" int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_p
     if (opts->max_iter < 0) return fail(ctx, RMX_EINVAL, "max_iter < 0");
     std::memset(res, 0, sizeof(*res));
     RMX_HIP(hipSetDevice(c.device));
+    if (c.nb > 0) {   // a batch of instances: its seeds are the attempts
+      if (opts->attempts > 1) return fail(ctx, RMX_EINVAL, "attempts > 1 on a batch: add instances instead");
+      batch_partition(c, *opts, *res);
+      return RMX_OK;
+    }
     const double ln2 = std::log(2.0);
     const double t_start = now_s();
     const int attempts = opts->attempts > 1 ? opts->attempts : 1;

@@ -154,14 +154,17 @@ __global__ void k_inst_totals(const uint32_t *__restrict__ start, int32_t n_seg,
   }
 }
 
-// instances with more than one member start active
+// instances with more than one member start active (in a batch of
+// networks, only those of networks still in the super-step loop)
 __global__ void k_inst_init(const uint32_t *__restrict__ start, int32_t n_seg, int32_t n,
                             const uint32_t *__restrict__ nodes, const uint32_t *__restrict__ par,
+                            const uint8_t *__restrict__ bact, const uint32_t *__restrict__ bnode,
                             uint8_t *__restrict__ act) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_seg) return;
   const uint32_t b = start[s], e = s + 1 < n_seg ? start[s + 1] : (uint32_t)n;
-  act[par[nodes[b]]] = e - b > 1 ? 1 : 0;
+  const uint32_t I = par[nodes[b]];
+  act[I] = (e - b > 1 && (!bact || bact[bnode[I]])) ? 1 : 0;
 }
 
 // the stop rule of partition_module_network, per instance
@@ -363,7 +366,8 @@ int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t se
   k_inst_const<<<gw, 256, 0, s>>>(c.inst_start.p, c.n_inst, n, c.inst_nodes.p, c.sub_par.p, S.us.p, c.inst_C.p);
   RMX_LAUNCH_CHECK();
   fill_u8(c.inst_act.p, 0, n, s);
-  k_inst_init<<<gs, 256, 0, s>>>(c.inst_start.p, c.n_inst, n, c.inst_nodes.p, c.sub_par.p, c.inst_act.p);
+  k_inst_init<<<gs, 256, 0, s>>>(c.inst_start.p, c.n_inst, n, c.inst_nodes.p, c.sub_par.p,
+                                 c.nb > 0 ? c.b_step.p : nullptr, c.b_node.p, c.inst_act.p);
   RMX_LAUNCH_CHECK();
 
   // partition_module_network on every instance (OmpRelaxmap.cpp:436-498,

@@ -72,6 +72,61 @@ __global__ __launch_bounds__(kRedBlock) void k_totals_final(const double *__rest
   }
 }
 
+// A batch: one workgroup per instance over its module ids (modules never
+// leave their instance).  S = sum of exits, L = phi(S) - 2 sum phi(q) +
+// sum phi(q + P) - sum phi(p) (Module.cpp:474-501 per instance), and the
+// number of non-empty modules.
+__global__ __launch_bounds__(kRedBlock) void k_batch_totals(const uint32_t *__restrict__ off,
+                                                            const Stat4 *__restrict__ ms,
+                                                            const int32_t *__restrict__ mNodes,
+                                                            const double *__restrict__ C, double *__restrict__ S_out,
+                                                            double *__restrict__ L_out, int32_t *__restrict__ nmod) {
+  __shared__ double lds[kRedBlock / kWave];
+  const uint32_t b = off[blockIdx.x], e = off[blockIdx.x + 1];
+  double sq = 0, pq = 0, pqp = 0, cnt = 0;
+  for (uint32_t m = b + threadIdx.x; m < e; m += kRedBlock) {
+    if (mNodes[m] > 0) {
+      const Stat4 st = ms[m];
+      const double q = module_exit(st.p, st.t, st.d, st.x);
+      sq += q;
+      pq += plogp(q);
+      pqp += plogp(q + fx_to(st.p));
+      cnt += 1.0;
+    }
+  }
+  sq = block_sum<double, kRedBlock>(sq, lds);
+  pq = block_sum<double, kRedBlock>(pq, lds);
+  pqp = block_sum<double, kRedBlock>(pqp, lds);
+  cnt = block_sum<double, kRedBlock>(cnt, lds);
+  if (threadIdx.x == 0) {
+    S_out[blockIdx.x] = sq;
+    L_out[blockIdx.x] = plogp(sq) - 2.0 * pq + pqp - C[blockIdx.x];
+    nmod[blockIdx.x] = (int32_t)cnt;
+  }
+}
+
+// sums over the instances (the host's view of a batch: total codelength,
+// total exit, modules)
+__global__ __launch_bounds__(kRedBlock) void k_batch_sum(int32_t nb, const double *__restrict__ S,
+                                                         const double *__restrict__ L,
+                                                         const int32_t *__restrict__ nmod, double *__restrict__ scal) {
+  __shared__ double lds[kRedBlock / kWave];
+  double s = 0, l = 0, k = 0;
+  for (int32_t I = threadIdx.x; I < nb; I += kRedBlock) {
+    s += S[I];
+    l += L[I];
+    k += (double)nmod[I];
+  }
+  s = block_sum<double, kRedBlock>(s, lds);
+  l = block_sum<double, kRedBlock>(l, lds);
+  k = block_sum<double, kRedBlock>(k, lds);
+  if (threadIdx.x == 0) {
+    scal[kS] = s;
+    scal[kL] = l;
+    scal[kNM] = k;
+  }
+}
+
 __global__ void k_zero_modules(int32_t n, Stat4 *ms, int32_t *mNodes, int32_t *mUnits) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -225,6 +280,17 @@ ArcW arcw_in(const Ctx &c, const Level &L) {
 }
 
 void module_totals_async(Ctx &c) {
+  if (c.nb > 0) {   // per instance: S for the sweeps, L for the driver
+    c.b_S.alloc(c.nb);
+    c.b_L.alloc(c.nb);
+    c.b_nmod.alloc(c.nb);
+    k_batch_totals<<<c.nb, kRedBlock, 0, c.stream>>>(c.b_off.p, c.ms.p, c.mNodes.p, c.b_C.p, c.b_S.p, c.b_L.p,
+                                                      c.b_nmod.p);
+    RMX_LAUNCH_CHECK();
+    k_batch_sum<<<1, kRedBlock, 0, c.stream>>>(c.nb, c.b_S.p, c.b_L.p, c.b_nmod.p, c.scal.p);
+    RMX_LAUNCH_CHECK();
+    return;
+  }
   k_module_totals<<<kRedGrid, kRedBlock, 0, c.stream>>>(c.ms.p, c.mNodes.p, c.N,
                                                          c.partial.p);
   RMX_LAUNCH_CHECK();

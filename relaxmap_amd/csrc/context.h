@@ -136,6 +136,9 @@ struct Level {
   int64_t hub_arcs = 0;    // sum of the hubs' degrees
   int64_t hub_parts = 0;   // partitions of all hubs (partitioned hub evaluation)
   int32_t max_deg = 0;
+  // batch of instances: instance of every unit, and the first unit of every
+  // instance (units are grouped by instance, in instance order)
+  DevBuf<uint32_t> uinst, uoff;
   void free_all();
   void swap(Level &o);
 };
@@ -196,7 +199,24 @@ struct Ctx {
   // module SoA, N slots (module ids are node ids, as in the reference)
   DevBuf<Stat4> ms;                     // fixed-point sums: flow, teleport, dangling, exit arcs
   DevBuf<int32_t> mNodes, mUnits, mStale;  // original nodes, units at the current level, stale count
-  DevBuf<uint32_t> mInst;                 // instance (graph) of each module (batched mode)
+
+  // A batch of independent instances (rmx_batch_upload): instance I owns the
+  // node ids [b_off[I], b_off[I + 1]) and no arc joins two instances.
+  // Modules never leave their instance (an empty module is drawn from the
+  // instance's own ids), so every per-instance reduction runs over an id
+  // range, and units of every level stay grouped by instance.
+  int32_t nb = 0;                          // instances; 0: one network
+  std::vector<uint32_t> h_boff;            // node offsets, host copy
+  DevBuf<uint32_t> b_off, b_node;          // node offsets (nb + 1); instance of every node
+  DevBuf<uint64_t> b_seed;                 // per-instance seed of the batch draw
+  DevBuf<double> b_S, b_L, b_C;            // per instance: sum of exits, codelength, sum phi(p)
+  DevBuf<double> b_Lold, b_Lstep, b_Louter;  // codelength before a sweep / a super step / an outer round
+  DevBuf<int32_t> b_nmod;                  // non-empty modules per instance
+  DevBuf<double> b_D, b_R;                 // steady state: dangling mass and L1 residual per instance
+  DevBuf<uint32_t> b_iters;                // steady-state iterations per instance
+  DevBuf<uint8_t> b_step, b_outer, b_inner;   // still in the super-step / outer / sweep loop
+  DevBuf<uint32_t> b_ebeg, b_ecnt, b_ecur; // empty modules per instance: list segment, count, taken
+  DevBuf<uint32_t> b_count;                // counters read by the host driver
 
   // sweep work arrays (sized for level 0, reused by level 1)
   DevBuf<uint8_t> act, act_next;

@@ -184,6 +184,27 @@ __global__ void k_copy_i32(const int32_t *__restrict__ a, int32_t *__restrict__ 
   if (i < n) b[i] = a[i];
 }
 
+// a batch: the instance of every super node (that of its module id) and the
+// first super node of every instance (super nodes follow module-id order,
+// and an instance's modules lie in its id range)
+__global__ void k_unit_inst(const uint32_t *__restrict__ mod, int32_t ns, const uint32_t *__restrict__ b_node,
+                            uint32_t *__restrict__ uinst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ns) uinst[i] = b_node[mod[i]];
+}
+
+__global__ void k_unit_off(const uint32_t *__restrict__ uinst, int32_t ns, int32_t nb, uint32_t *__restrict__ uoff) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I > nb) return;
+  uint32_t lo = 0, hi = (uint32_t)ns;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (uinst[mid] < (uint32_t)I) lo = mid + 1;
+    else hi = mid;
+  }
+  uoff[I] = lo;
+}
+
 int bits_for(int64_t n) {
   int b = 1;
   while (b < 32 && ((int64_t)1 << b) < n) ++b;
@@ -338,6 +359,14 @@ void contract(Ctx &c) {
   c.sync();
   tkey.free();
   twx.free();
+  if (c.nb > 0) {
+    D.uinst.alloc(ns);
+    D.uoff.alloc((int64_t)c.nb + 1);
+    k_unit_inst<<<grid_for(ns, 256), 256, 0, s>>>(D.mod.p, (int32_t)ns, c.b_node.p, D.uinst.p);
+    RMX_LAUNCH_CHECK();
+    k_unit_off<<<grid_for((int64_t)c.nb + 1, 256), 256, 0, s>>>(D.uinst.p, (int32_t)ns, c.nb, D.uoff.p);
+    RMX_LAUNCH_CHECK();
+  }
   build_buckets(c, D);
   c.sync();
   c.L1.swap(c.L2);

@@ -105,8 +105,8 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rows,
                                               const uint32_t *__restrict__ iptr, const uint32_t *__restrict__ iidx,
                                               const double *__restrict__ iw, const float *__restrict__ iwf,
                                               const double *__restrict__ x, const float *__restrict__ xf,
-                                              const double *__restrict__ tau, const double *__restrict__ scal,
-                                              double *__restrict__ y) {
+                                              const double *__restrict__ tau, const double *__restrict__ Dv,
+                                              const uint32_t *__restrict__ rinst, double *__restrict__ y) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t g = t / G;
   const int lane = (int)(t % G);
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rows,
     for (uint32_t k = b + lane; k < e; k += G) acc = fma(iw[k], x[iidx[k]], acc);
   }
   acc = group_sum<G>(acc);
-  if (lane == 0) y[r] = (kAlpha + kBeta * scal[kD]) * tau[r] + kBeta * acc;
+  if (lane == 0) y[r] = (kAlpha + kBeta * Dv[rinst ? rinst[r] : 0]) * tau[r] + kBeta * acc;
 }
 
 template <bool MIXED>
@@ -129,7 +129,8 @@ __global__ __launch_bounds__(256) void k_spmv_block(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ iidx, const double *__restrict__ iw,
                                                     const float *__restrict__ iwf, const double *__restrict__ x,
                                                     const float *__restrict__ xf, const double *__restrict__ tau,
-                                                    const double *__restrict__ scal, double *__restrict__ y) {
+                                                    const double *__restrict__ Dv,
+                                                    const uint32_t *__restrict__ rinst, double *__restrict__ y) {
   __shared__ double lds[4];
   for (int64_t g = blockIdx.x; g < nrows; g += gridDim.x) {
     const uint32_t r = rows[g];
@@ -141,7 +142,7 @@ __global__ __launch_bounds__(256) void k_spmv_block(const uint32_t *__restrict__
       for (uint32_t k = b + threadIdx.x; k < e; k += 256) acc = fma(iw[k], x[iidx[k]], acc);
     }
     acc = block_sum<double, 256>(acc, lds);
-    if (threadIdx.x == 0) y[r] = (kAlpha + kBeta * scal[kD]) * tau[r] + kBeta * acc;
+    if (threadIdx.x == 0) y[r] = (kAlpha + kBeta * Dv[rinst ? rinst[r] : 0]) * tau[r] + kBeta * acc;
   }
 }
 
@@ -215,6 +216,110 @@ __global__ __launch_bounds__(kRedBlock) void k_init_x(double *__restrict__ x, fl
     part_d[blockIdx.x] = dm;
     part_r[blockIdx.x] = 0.0;
   }
+}
+
+// --- a batch of instances: one workgroup per instance over its node range --
+__device__ inline double block_total(double v, double *lds, double *out) {
+  v = block_sum<double, kRedBlock>(v, lds);
+  if (threadIdx.x == 0) *out = v;
+  __syncthreads();
+  return *out;
+}
+
+// teleport weights: tau = w / (sum of w over the node's instance)
+__global__ __launch_bounds__(kRedBlock) void k_btau(const uint32_t *__restrict__ off, const double *__restrict__ nw,
+                                                    double *__restrict__ tau) {
+  __shared__ double lds[kRedBlock / kWave];
+  __shared__ double tot;
+  const uint32_t b = off[blockIdx.x], e = off[blockIdx.x + 1];
+  double s = 0.0;
+  for (uint32_t i = b + threadIdx.x; i < e; i += kRedBlock) s += nw[i];
+  s = block_total(s, lds, &tot);
+  for (uint32_t i = b + threadIdx.x; i < e; i += kRedBlock) tau[i] = nw[i] / s;
+}
+
+// p = 1/n on every instance, its dangling mass; nothing converged yet
+__global__ __launch_bounds__(kRedBlock) void k_binit(const uint32_t *__restrict__ off,
+                                                     const uint8_t *__restrict__ dang, double *__restrict__ x,
+                                                     float *__restrict__ xf, double *__restrict__ D,
+                                                     double *__restrict__ R, uint8_t *__restrict__ done) {
+  __shared__ double lds[kRedBlock / kWave];
+  __shared__ double tot;
+  const uint32_t b = off[blockIdx.x], e = off[blockIdx.x + 1];
+  const double v = e > b ? 1.0 / (double)(e - b) : 0.0;
+  double dm = 0.0;
+  for (uint32_t i = b + threadIdx.x; i < e; i += kRedBlock) {
+    x[i] = v;
+    xf[i] = (float)v;
+    if (dang[i]) dm += v;
+  }
+  dm = block_total(dm, lds, &tot);
+  if (threadIdx.x == 0) {
+    D[blockIdx.x] = dm;
+    R[blockIdx.x] = 0.0;
+    done[blockIdx.x] = e > b ? 0 : 1;
+  }
+}
+
+// p <- y / sum(y) over the instance, its L1 change and dangling mass; an
+// instance that has stopped keeps its p
+__global__ __launch_bounds__(kRedBlock) void k_bnorm(const uint32_t *__restrict__ off, const double *__restrict__ y,
+                                                     const uint8_t *__restrict__ dang,
+                                                     const uint8_t *__restrict__ done, double *__restrict__ x,
+                                                     float *__restrict__ xf, double *__restrict__ D,
+                                                     double *__restrict__ R) {
+  __shared__ double lds[kRedBlock / kWave];
+  __shared__ double tot;
+  if (done[blockIdx.x]) return;
+  const uint32_t b = off[blockIdx.x], e = off[blockIdx.x + 1];
+  double s = 0.0;
+  for (uint32_t i = b + threadIdx.x; i < e; i += kRedBlock) s += y[i];
+  s = block_total(s, lds, &tot);
+  double r = 0.0, dm = 0.0;
+  for (uint32_t i = b + threadIdx.x; i < e; i += kRedBlock) {
+    const double pn = y[i] / s;
+    r += fabs(pn - x[i]);
+    if (dang[i]) dm += pn;
+    x[i] = pn;
+    xf[i] = (float)pn;
+  }
+  __syncthreads();
+  r = block_sum<double, kRedBlock>(r, lds);
+  if (threadIdx.x == 0) R[blockIdx.x] = r;
+  dm = block_sum<double, kRedBlock>(dm, lds);
+  if (threadIdx.x == 0) D[blockIdx.x] = dm;
+}
+
+// the stop rule after iteration `it`, per instance; *running counts the
+// instances that go on
+__global__ __launch_bounds__(kRedBlock) void k_bcheck(int32_t nb, int it, int min_iter, int max_iter, double tol,
+                                                      const double *__restrict__ R, uint8_t *__restrict__ done,
+                                                      uint32_t *__restrict__ iters, uint32_t *__restrict__ running) {
+  __shared__ uint32_t lds[kRedBlock / kWave];
+  uint32_t k = 0;
+  for (int32_t I = threadIdx.x; I < nb; I += kRedBlock) {
+    if (done[I]) continue;
+    if (it >= max_iter || (it >= min_iter && !(R[I] > tol))) {
+      done[I] = 1;
+      iters[I] = (uint32_t)it;
+    } else {
+      ++k;
+    }
+  }
+  k = block_sum<uint32_t, kRedBlock>(k, lds);
+  if (threadIdx.x == 0) *running = k;
+}
+
+// sum of phi(p) over every instance's nodes (constant during the partition)
+__global__ __launch_bounds__(kRedBlock) void k_bconst(const uint32_t *__restrict__ off, const double *__restrict__ p,
+                                                      double *__restrict__ C) {
+  __shared__ double lds[kRedBlock / kWave];
+  __shared__ double tot;
+  const uint32_t b = off[blockIdx.x], e = off[blockIdx.x + 1];
+  double a = 0.0;
+  for (uint32_t i = b + threadIdx.x; i < e; i += kRedBlock) a += plogp(p[i]);
+  a = block_total(a, lds, &tot);
+  if (threadIdx.x == 0) C[blockIdx.x] = a;
 }
 
 // --- flows on arcs and node statistics (Module.cpp:267-325) ---------------
@@ -326,6 +431,8 @@ struct PullCSR {
   const float *wf = nullptr;
   const double *tau = nullptr;
   const uint8_t *dangling = nullptr;
+  const double *Dv = nullptr;        // dangling mass: one value, or one per instance (batch)
+  const uint32_t *rinst = nullptr;   // instance of every row (batch), else null
 };
 
 __global__ void k_out_degree_key(const uint32_t *__restrict__ optr, int32_t n, uint32_t *__restrict__ key) {
@@ -400,11 +507,11 @@ void launch_spmv_bin(int bin, const RowBins &rb, const PullCSR &L, Ctx &c, doubl
   if (G > 0) {
     constexpr int GG = G > 0 ? G : 1;
     k_spmv<GG, MIXED><<<grid_for((int64_t)cnt * GG, 256), 256, 0, c.stream>>>(
-        rows, cnt, L.ptr, L.idx, L.w, L.wf, c.x.p, c.xf.p, L.tau, c.scal.p, y);
+        rows, cnt, L.ptr, L.idx, L.w, L.wf, c.x.p, c.xf.p, L.tau, L.Dv, L.rinst, y);
   } else {
     unsigned g = cnt < 4096 ? (unsigned)cnt : 4096u;
-    k_spmv_block<MIXED><<<g, 256, 0, c.stream>>>(rows, cnt, L.ptr, L.idx, L.w, L.wf, c.x.p, c.xf.p, L.tau,
-                                                 c.scal.p, y);
+    k_spmv_block<MIXED><<<g, 256, 0, c.stream>>>(rows, cnt, L.ptr, L.idx, L.w, L.wf, c.x.p, c.xf.p, L.tau, L.Dv,
+                                                 L.rinst, y);
   }
   RMX_LAUNCH_CHECK();
   if (c.prof.on) {
@@ -472,13 +579,25 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   c.partial.alloc(8 * kRedGrid);
   double *part = c.partial.p;
 
-  // teleport weights tau = w / sum w (OmpRelaxmap.cpp:113-115)
-  k_sum_partials<<<kRedGrid, kRedBlock, 0, s>>>(c.node_w.p, n, part);
-  RMX_LAUNCH_CHECK();
-  k_final_rd<<<1, kRedBlock, 0, s>>>(part, part, c.scal.p);   // scal[kR] = scal[kD] = sum w
-  RMX_LAUNCH_CHECK();
-  k_tau<<<grid_for(n, 256), 256, 0, s>>>(c.node_w.p, n, c.scal.p + kR, L.tau.p);
-  RMX_LAUNCH_CHECK();
+  const bool batch = c.nb > 0;
+  if (batch) {
+    c.b_D.alloc(c.nb);
+    c.b_R.alloc(c.nb);
+    c.b_iters.alloc(c.nb);
+    c.b_count.alloc(4);
+  }
+  // teleport weights tau = w / sum w (OmpRelaxmap.cpp:113-115), per instance in a batch
+  if (batch) {
+    k_btau<<<c.nb, kRedBlock, 0, s>>>(c.b_off.p, c.node_w.p, L.tau.p);
+    RMX_LAUNCH_CHECK();
+  } else {
+    k_sum_partials<<<kRedGrid, kRedBlock, 0, s>>>(c.node_w.p, n, part);
+    RMX_LAUNCH_CHECK();
+    k_final_rd<<<1, kRedBlock, 0, s>>>(part, part, c.scal.p);   // scal[kR] = scal[kD] = sum w
+    RMX_LAUNCH_CHECK();
+    k_tau<<<grid_for(n, 256), 256, 0, s>>>(c.node_w.p, n, c.scal.p + kR, L.tau.p);
+    RMX_LAUNCH_CHECK();
+  }
 
   k_normalize<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, c.raw_w.p, c.in_of_out.p, L.out_w.p,
                                                                   L.in_w.p, L.in_wf.p, c.dangling.p);
@@ -490,7 +609,17 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   DevBuf<uint8_t> pdang(c.cache);
   PullCSR F;
   F.n = n;
-  {
+  if (batch) {   // instances keep their node ranges: no relabelling
+    F.ptr = L.in_ptr.p;
+    F.idx = L.in_idx.p;
+    F.w = L.in_w.p;
+    F.wf = L.in_wf.p;
+    F.tau = L.tau.p;
+    F.dangling = c.dangling.p;
+    F.Dv = c.b_D.p;
+    F.rinst = c.b_node.p;
+  } else {
+    F.Dv = c.scal.p + kD;
     inv.alloc(n);
     perm.alloc(n);
     pptr.alloc((int64_t)n + 1);
@@ -536,12 +665,40 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   RMX_HIP(hipEventCreate(&e0));
   RMX_HIP(hipEventCreate(&e1));
   RMX_HIP(hipEventRecord(e0, s));
+  int it = 0;
+  double r = 1.0;
+  if (batch) {
+    // every instance iterates until its own stop rule holds, then keeps its p
+    const int32_t nb = c.nb;
+    DevBuf<uint8_t> done(c.cache);
+    done.alloc(nb);
+    fill_u32(c.b_iters.p, 0u, nb, s);
+    k_binit<<<nb, kRedBlock, 0, s>>>(c.b_off.p, c.dangling.p, c.x.p, c.xf.p, c.b_D.p, c.b_R.p, done.p);
+    RMX_LAUNCH_CHECK();
+    const int b_min = mixed ? 1 : min_iter;
+    const double b_tol = mixed ? kMixedTol : tol;
+    uint32_t running = 1;
+    while (running > 0) {
+      if (mixed) spmv_all<true>(c, F, rb, c.y.p);
+      else spmv_all<false>(c, F, rb, c.y.p);
+      k_bnorm<<<nb, kRedBlock, 0, s>>>(c.b_off.p, c.y.p, c.dangling.p, done.p, c.x.p, c.xf.p, c.b_D.p, c.b_R.p);
+      RMX_LAUNCH_CHECK();
+      ++it;
+      k_bcheck<<<1, kRedBlock, 0, s>>>(nb, it, b_min, max_iter, b_tol, c.b_R.p, done.p, c.b_iters.p, c.b_count.p);
+      RMX_LAUNCH_CHECK();
+      RMX_HIP(hipMemcpyAsync(&running, c.b_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      c.sync();
+    }
+    std::vector<double> hr(nb);
+    RMX_HIP(hipMemcpyAsync(hr.data(), c.b_R.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    c.sync();
+    r = 0.0;
+    for (double v : hr) r = v > r ? v : r;   // the largest residual of the batch
+  } else {
   k_init_x<<<kRedGrid, kRedBlock, 0, s>>>(c.x.p, c.xf.p, F.dangling, n, part + 2 * kRedGrid, part + kRedGrid);
   RMX_LAUNCH_CHECK();
   k_final_rd<<<1, kRedBlock, 0, s>>>(part + kRedGrid, part + 2 * kRedGrid, c.scal.p);
   RMX_LAUNCH_CHECK();
-  int it = 0;
-  double r = 1.0;
   std::vector<double> hist;
   const int first_check = mixed ? 1 : min_iter;
   while (it < max_iter) {
@@ -567,6 +724,7 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
       if (mixed && hist.size() > 10 && r > 0.5 * hist[hist.size() - 11]) break;
     }
   }
+  }
   RMX_HIP(hipEventRecord(e1, s));
   RMX_HIP(hipEventSynchronize(e1));
   float ms = 0;
@@ -576,10 +734,11 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   RMX_HIP(hipEventDestroy(e1));
   c.iterations = it;
   c.resid = r;
-  // p back on the original labels
-  k_unpermute<<<grid_for(n, 256), 256, 0, s>>>(c.x.p, perm.p, n, c.y.p);
-  RMX_LAUNCH_CHECK();
-  c.x.swap(c.y);
+  if (!batch) {   // p back on the original labels
+    k_unpermute<<<grid_for(n, 256), 256, 0, s>>>(c.x.p, perm.p, n, c.y.p);
+    RMX_LAUNCH_CHECK();
+    c.x.swap(c.y);
+  }
   c.sync();   // the relabelled copy goes back to the cache
 
   // flows, node statistics and the literal node exits
@@ -596,6 +755,11 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   c.initial_codelength = c.h_scal[kL];
   c.initial_sum_exit = c.h_scal[kS];
   c.all_log_all = c.h_scal[kSP];
+  if (batch) {   // sum of phi(p) per instance; the codelengths come from the singleton modules
+    c.b_C.alloc(c.nb);
+    k_bconst<<<c.nb, kRedBlock, 0, s>>>(c.b_off.p, L.p.p, c.b_C.p);
+    RMX_LAUNCH_CHECK();
+  }
   {
     // number of dangling nodes
     std::vector<uint8_t> dg(n);

@@ -16,6 +16,7 @@ void Level::free_all() {
   out_w.free(); in_w.free(); out_wf.free(); in_wf.free(); out_wx.free(); in_wx.free();
   p.free(); tau.free(); us.free();
   nmem.free(); mod.free(); bucket.free(); hub_off.free();
+  uinst.free(); uoff.free();
   hub_pool = 0;
   hub_arcs = 0;
   hub_parts = 0;
@@ -35,6 +36,8 @@ void Level::swap(Level &o) {
   std::swap(hub_arcs, o.hub_arcs);
   std::swap(hub_parts, o.hub_parts);
   std::swap(max_deg, o.max_deg);
+  uinst.swap(o.uinst);
+  uoff.swap(o.uoff);
 }
 
 namespace {

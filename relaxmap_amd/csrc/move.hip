@@ -86,8 +86,9 @@ struct EvalArgs {
   int empty_avail;
   uint32_t empty_key;
   const double *scal;
-  const double *S_inst;      // per-instance S (sub-module partition), indexed by inst[u]; null: scal[kS]
+  const double *S_inst;      // per-instance S (sub-module partition or batch), indexed by inst[u]; null: scal[kS]
   const uint32_t *inst;
+  const uint32_t *e_cnt, *e_cur;   // batch: empty modules of each instance and those taken (else null)
   double vthresh;
   uint32_t *target;
   long long *agg;        // per unit: flows to/from the old and the new module
@@ -201,13 +202,16 @@ struct UnitLoads {
   Stat4 su, so;
   int32_t units_o, stale_o;
   double S;
+  bool empty_left;   // an empty module is still available to the unit's instance
 };
 
 __device__ inline UnitLoads load_unit(const EvalArgs &a, uint32_t u) {
   UnitLoads l;
   l.o = a.mod[u];
   l.su = a.us[u];
-  l.S = a.S_inst ? a.S_inst[a.inst[u]] : a.scal[kS];
+  const uint32_t I = a.inst ? a.inst[u] : 0u;
+  l.S = a.S_inst ? a.S_inst[I] : a.scal[kS];
+  l.empty_left = a.e_cnt ? a.e_cur[I] < a.e_cnt[I] : a.empty_avail != 0;
   l.so = a.ms[l.o];
   l.units_o = a.mUnits[l.o];
   l.stale_o = a.node_level ? a.mStale[l.o] : 0;
@@ -234,7 +238,7 @@ __device__ inline void unit_ctx(const EvalArgs &a, const UnitLoads &l, long long
   U.phiS = plogp(S);
   U.d1 = q1 - qo;
   U.single = l.units_o == 1;
-  U.offer_empty = a.empty_avail && (a.node_level ? l.stale_o > 1 : l.units_o > 1);
+  U.offer_empty = l.empty_left && (a.node_level ? l.stale_o > 1 : l.units_o > 1);
 }
 
 __device__ inline void unit_ctx(const EvalArgs &a, uint32_t u, long long fo_o, long long fi_o, UnitCtx &U) {
@@ -1294,6 +1298,70 @@ __global__ void k_resolve(const uint32_t *__restrict__ movers, const uint32_t *_
   }
 }
 
+// The same for a batch: an empty-module request is served from the
+// requester's instance, by its rank among that instance's requests.  rank is
+// the exclusive scan of the requests over the level's units (grouped by
+// instance), *total its sum.
+__global__ void k_resolve_batch(const uint32_t *__restrict__ movers, const uint32_t *__restrict__ counters,
+                                const uint32_t *__restrict__ target, const uint32_t *__restrict__ rank,
+                                const uint32_t *__restrict__ total, int32_t n_units,
+                                const uint32_t *__restrict__ uinst, const uint32_t *__restrict__ uoff,
+                                const uint32_t *__restrict__ empty_list, const uint32_t *__restrict__ e_beg,
+                                const uint32_t *__restrict__ e_cnt, const uint32_t *__restrict__ e_cur,
+                                uint32_t empty_key, uint32_t *__restrict__ newmod, uint32_t *__restrict__ stamp,
+                                uint32_t *__restrict__ want, uint32_t cur) {
+  const uint32_t nm = counters[cMovers];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nm; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = movers[i];
+    uint32_t t = target[u];
+    if (t == empty_key) {
+      const uint32_t I = uinst[u];
+      const uint32_t first = uoff[I];
+      const uint32_t before = (int32_t)first < n_units ? rank[first] : *total;
+      const uint32_t r = e_cur[I] + (rank[u] - before);
+      t = r < e_cnt[I] ? empty_list[e_beg[I] + r] : kInvalid;
+      want[u] = 0u;
+    }
+    newmod[u] = t;
+    if (t != kInvalid) stamp[u] = cur;
+  }
+}
+
+// after k_resolve_batch: each instance's taken empty modules
+__global__ void k_bempty_advance(int32_t nb, const uint32_t *__restrict__ uoff, const uint32_t *__restrict__ rank,
+                                 const uint32_t *__restrict__ total, int32_t n_units,
+                                 const uint32_t *__restrict__ e_cnt, uint32_t *__restrict__ e_cur) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nb) return;
+  const uint32_t f = uoff[I], l = uoff[I + 1];
+  const uint32_t a = (int32_t)f < n_units ? rank[f] : *total;
+  const uint32_t b = (int32_t)l < n_units ? rank[l] : *total;
+  const uint32_t req = b - a, left = e_cnt[I] - e_cur[I];
+  e_cur[I] += req < left ? req : left;
+}
+
+// every instance's segment of the (ascending) empty-module list: the ids in
+// [off[I], off[I + 1])
+__global__ void k_bempty_seg(int32_t nb, const uint32_t *__restrict__ off, const uint32_t *__restrict__ empty_list,
+                             uint32_t n_empty, uint32_t *__restrict__ e_beg, uint32_t *__restrict__ e_cnt,
+                             uint32_t *__restrict__ e_cur) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nb) return;
+  auto lower = [&](uint32_t v) {
+    uint32_t lo = 0, hi = n_empty;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (empty_list[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  const uint32_t b = lower(off[I]), e = lower(off[I + 1]);
+  e_beg[I] = b;
+  e_cnt[I] = e - b;
+  e_cur[I] = 0;
+}
+
 // G lanes per mover: module statistics, exit flows and activity marks.
 // X changes of a move of u from o to n with every neighbour fixed:
 //   X_o += -(W_u - A_o) + B_o,  X_n += (W_u - A_n) - B_n;
@@ -1467,7 +1535,8 @@ __global__ void k_apply(const uint32_t *__restrict__ movers, const uint32_t *__r
 
 // sort key of every bucket position: (bucket, batch) or (bucket, inactive)
 __global__ void k_batch_keys(const uint32_t *__restrict__ bucket, int32_t n, int4 off_lo, int4 off_hi,
-                             const uint8_t *__restrict__ act, int all_active, uint64_t seed, int batches,
+                             const uint8_t *__restrict__ act, int all_active, uint64_t seed,
+                             const uint32_t *__restrict__ uinst, const uint64_t *__restrict__ iseed, int batches,
                              uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1475,7 +1544,8 @@ __global__ void k_batch_keys(const uint32_t *__restrict__ bucket, int32_t n, int
   int b = 0;
   while (b + 1 < kNumBuckets && i >= offs[b + 1]) ++b;
   const uint32_t u = bucket[i];
-  const uint32_t k = (all_active || act[u]) ? batch_of(seed, batches, u) : (uint32_t)batches;
+  const uint64_t sd = uinst ? seed ^ iseed[uinst[u]] : seed;   // a batch: every instance its own draw
+  const uint32_t k = (all_active || act[u]) ? batch_of(sd, batches, u) : (uint32_t)batches;
   keys[i] = (uint32_t)b * (uint32_t)(batches + 1) + k;
   vals[i] = b + 1 < kNumBuckets ? u : (uint32_t)i;   // hubs keep their bucket position (their table region)
 }
@@ -1652,6 +1722,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   hipStream_t s = c.stream;
   const int32_t n = L.n;
   const uint64_t sweep_seed = mix64(seed ^ mix64(++c.sweep_counter));
+  const bool batch = c.nb > 0 && !c.sub_mode;   // independent instances (not a sub-module partition)
 
   // empty modules available to this sweep, ascending id (none in a
   // sub-module partition: its stale member lists never exceed one member,
@@ -1672,6 +1743,14 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   c.sync();
   n_empty = c.sub_mode ? 0u : c.h_cnt[8];
   uint32_t empty_cursor = 0;
+  if (batch) {   // each instance's empty modules
+    c.b_ebeg.alloc(c.nb);
+    c.b_ecnt.alloc(c.nb);
+    c.b_ecur.alloc(c.nb);
+    k_bempty_seg<<<grid_for(c.nb, 256), 256, 0, s>>>(c.nb, c.b_off.p, c.empty_list.p, n_empty, c.b_ebeg.p,
+                                                      c.b_ecnt.p, c.b_ecur.p);
+    RMX_LAUNCH_CHECK();
+  }
   if (c.prof.on) {   // h_cnt mirrors the counters from here on (profiling deltas)
     RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
     c.sync();
@@ -1723,7 +1802,8 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     const int *o = L.bucket_off;
     k_batch_keys<<<grid_for(n, 256), 256, 0, s>>>(L.bucket.p, n, make_int4(o[0], o[1], o[2], o[3]),
                                                   make_int4(o[4], o[5], o[6], o[7]), c.act.p, prioritized ? 0 : 1,
-                                                  sweep_seed, B, c.batch_keys.p, c.batch_list.p);
+                                                  sweep_seed, batch ? L.uinst.p : nullptr, c.b_seed.p, B,
+                                                  c.batch_keys.p, c.batch_list.p);
     RMX_LAUNCH_CHECK();
     radix_sort_pairs_u32(c.batch_keys.p, c.batch_list.p, n, 0, 8, c.scratch2, s);
     fill_u32(c.counters.p + 16, 0u, 256, s);
@@ -1757,8 +1837,10 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   a.node_level = c.level == 0 ? 1 : 0;
   a.empty_key = (uint32_t)c.N + 1u;
   a.scal = c.scal.p;
-  a.S_inst = c.sub_mode ? c.inst_S.p : nullptr;
-  a.inst = c.sub_mode ? c.sub_par.p : nullptr;
+  a.S_inst = c.sub_mode ? c.inst_S.p : (batch ? c.b_S.p : nullptr);
+  a.inst = c.sub_mode ? c.sub_par.p : (batch ? L.uinst.p : nullptr);
+  a.e_cnt = batch ? c.b_ecnt.p : nullptr;
+  a.e_cur = batch ? c.b_ecur.p : nullptr;
   a.vthresh = prioritized ? vthresh : 0.0;
   a.target = c.target.p;
   a.agg = c.agg_buf.p;
@@ -1891,14 +1973,28 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     const uint32_t n_movers = c.h_cnt[cMovers], n_want = c.h_cnt[cWant];
     if (n_movers == 0) continue;
     ++c.stamp_now;
-    if (n_want > 0) exclusive_scan_u32(c.want_empty.p, c.rank_tmp.p, n, nullptr, c.scratch, s);
-    k_resolve<<<stride_grid(n_movers, 256), 256, 0, s>>>(c.movers.p, c.counters.p, c.target.p, c.rank_tmp.p,
-                                                          c.empty_list.p, empty_cursor, n_empty, a.empty_key,
-                                                          c.newmod.p, c.stamp.p, c.want_empty.p, c.stamp_now);
-    RMX_LAUNCH_CHECK();
-    if (n_want > 0) {
-      uint32_t left = n_empty - empty_cursor;
-      empty_cursor += n_want < left ? n_want : left;
+    if (batch) {
+      if (n_want > 0) exclusive_scan_u32(c.want_empty.p, c.rank_tmp.p, n, c.counters.p + 9, c.scratch, s);
+      k_resolve_batch<<<stride_grid(n_movers, 256), 256, 0, s>>>(
+          c.movers.p, c.counters.p, c.target.p, c.rank_tmp.p, c.counters.p + 9, n, L.uinst.p, L.uoff.p,
+          c.empty_list.p, c.b_ebeg.p, c.b_ecnt.p, c.b_ecur.p, a.empty_key, c.newmod.p, c.stamp.p, c.want_empty.p,
+          c.stamp_now);
+      RMX_LAUNCH_CHECK();
+      if (n_want > 0) {
+        k_bempty_advance<<<grid_for(c.nb, 256), 256, 0, s>>>(c.nb, L.uoff.p, c.rank_tmp.p, c.counters.p + 9, n,
+                                                              c.b_ecnt.p, c.b_ecur.p);
+        RMX_LAUNCH_CHECK();
+      }
+    } else {
+      if (n_want > 0) exclusive_scan_u32(c.want_empty.p, c.rank_tmp.p, n, nullptr, c.scratch, s);
+      k_resolve<<<stride_grid(n_movers, 256), 256, 0, s>>>(c.movers.p, c.counters.p, c.target.p, c.rank_tmp.p,
+                                                            c.empty_list.p, empty_cursor, n_empty, a.empty_key,
+                                                            c.newmod.p, c.stamp.p, c.want_empty.p, c.stamp_now);
+      RMX_LAUNCH_CHECK();
+      if (n_want > 0) {
+        uint32_t left = n_empty - empty_cursor;
+        empty_cursor += n_want < left ? n_want : left;
+      }
     }
     const uint32_t big_deg = n_hubs > 0 ? (uint32_t)kBucketMaxDeg[kNumBuckets - 2] : 0xffffffffu;
     k_commit<16><<<stride_grid((int64_t)n_movers * 16, 256), 256, 0, s>>>(

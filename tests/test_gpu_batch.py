@@ -59,3 +59,81 @@ def test_attempts_keep_the_best():
         assert r.best_attempt == best
         assert r.computed_codelength == single[best]
         assert abs(net.codelength - r.computed_codelength) <= 1e-9 * abs(r.computed_codelength)
+
+
+# ---------------------------------------------------------------- one batch
+def _oracle_codelength(g, part):
+    from oracle.pyoracle import OracleNet
+    src = np.repeat(np.arange(g.n), np.diff(np.asarray(g.ptr)))
+    o = OracleNet.from_arcs(g.n, src, np.asarray(g.idx), np.asarray(g.w)).initiate()
+    o.set_partition(np.asarray(part, dtype=np.int32))
+    o.update_members()
+    return o.calculate_codelength()
+
+
+def test_batch_partition_valid_and_exact():
+    """Every instance's modules stay inside it, and its codelength equals the
+    oracle's codelength of its own partition (FP64 flows)."""
+    from relaxmap_amd.batch import run_batched, union_of
+    graphs = _graphs()[:4]
+    seeds = [1, 2]
+    res, dt, net = run_batched(graphs, seeds, precision="fp64", return_net=True)
+    with net:
+        part = net.partition()
+    pairs, node_off, _, _, _, _ = union_of(graphs, seeds)
+    print(f"[gpu] batch of {len(pairs)} instances in {dt:.2f} s", flush=True)
+    for k, (gi, s) in enumerate(pairs):
+        a, b = int(node_off[k]), int(node_off[k + 1])
+        p = part[a:b]
+        assert p.min() >= a and p.max() < b, (gi, s)
+        ref = _oracle_codelength(graphs[gi], p - a)
+        assert abs(res[(gi, s)][2] - ref) <= 1e-9 * abs(ref), (gi, s)
+        assert res[(gi, s)][1] == len(np.unique(p))
+
+
+def test_batch_flows_match_single_networks():
+    """The steady state of every instance is that of its network alone."""
+    from relaxmap_amd import Network
+    from relaxmap_amd.batch import union_of
+    graphs = _graphs()[:3]
+    pairs, node_off, iseeds, ptr, idx, w = union_of(graphs, [1])
+    # FP64: the reference's rule per instance (flows 1e-12 L1); mixed: its
+    # 1e-8 residual stop (flows 1e-6 L1 of FP64)
+    for prec, tol, tol_l in (("fp64", 1e-12, 1e-9), ("mixed", 1e-6, 1e-6)):
+        with Network(precision=prec) as net:
+            net.batch_upload(node_off, iseeds, ptr, idx, w)
+            net.initiate()
+            p = net.flows()
+            L0 = net.batch_results()[0]
+        for k, (gi, _) in enumerate(pairs):
+            a, b = int(node_off[k]), int(node_off[k + 1])
+            with Network(precision="fp64") as solo:
+                graphs[gi].upload_to(solo)
+                _, l0 = solo.initiate()
+                q = solo.flows()
+            assert abs(p[a:b].sum() - 1.0) <= 1e-12
+            assert np.abs(p[a:b] - q).sum() <= tol, (prec, gi)
+            assert abs(L0[k] - l0) <= tol_l * abs(l0), (prec, gi)   # singleton modules
+
+
+def test_batch_close_to_single_runs():
+    """Same networks and seeds, one batch vs one context each: the batch
+    draws its own random sweeps, so codelengths agree to within 1 %."""
+    from relaxmap_amd.batch import run_batched, run_instances
+    graphs = _graphs()
+    seeds = [1, 2]
+    rb, tb = run_batched(graphs, seeds)
+    ri, ti = run_instances(graphs, seeds, workers=4)
+    print(f"[gpu] {len(rb)} instances: batch {tb:.2f} s, contexts {ti:.2f} s", flush=True)
+    assert set(rb) == set(ri)
+    rel = [abs(rb[k][0] - ri[k][0]) / ri[k][0] for k in rb]
+    print(f"[gpu] codelength difference: median {np.median(rel):.2e}, max {max(rel):.2e}", flush=True)
+    assert max(rel) <= 0.01
+
+
+def test_batch_deterministic():
+    from relaxmap_amd.batch import run_batched
+    graphs = _graphs()[3:5]
+    r1, _ = run_batched(graphs, [3, 4])
+    r2, _ = run_batched(graphs, [3, 4])
+    assert r1 == r2
