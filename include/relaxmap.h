@@ -67,6 +67,21 @@ int rmx_graph_upload(rmx_ctx *ctx, int32_t n, int64_t m, const int64_t *out_ptr,
 int rmx_load_network(rmx_ctx *ctx, const char *path, int32_t *n, int64_t *m, int32_t *n_self_links,
                      int32_t *n_double_links);
 
+/* Degree buckets of the sweeps (no reference counterpart): units with
+ * degree (out + in) in (max_deg[k-1], max_deg[k]] are evaluated by kernel k
+ * (k = 0..6: 8..64-lane groups with 32..1024-key LDS tables, then 2048- and
+ * 4096-key workgroup tables); above max_deg[6], by the hub partitions.
+ * max_deg[k] may not exceed its default (16, 32, 128, 256, 512, 1023,
+ * 2048), the largest degree the kernel's table holds at load 1/2.  The
+ * choice changes speed only: every kernel finds the same best move.
+ * rmx_autotune times the eligible kernels on the current level's units of
+ * each degree class (evaluation only, nothing moves) and installs the
+ * fastest cut-offs that keep the kernel order. */
+#define RMX_BUCKET_CUTS 7
+int rmx_set_bucket_cutoffs(rmx_ctx *ctx, const int32_t *max_deg);
+int rmx_get_bucket_cutoffs(rmx_ctx *ctx, int32_t *max_deg);
+int rmx_autotune(rmx_ctx *ctx, int32_t *max_deg);
+
 /* A batch of independent networks (many graphs, or one graph under many
  * seeds) partitioned together: the union of the networks' out-CSRs, where
  * instance I owns the node ids [node_off[I], node_off[I+1]) and no arc joins

@@ -87,6 +87,9 @@ def load_library():
         "rmx_batch_upload": (C.c_int, [vp, C.c_int32, _I32P, C.POINTER(C.c_uint64), C.c_int32, C.c_int64, _I64P,
                                        _I32P, _F64P, _F64P]),
         "rmx_batch_results": (C.c_int, [vp, _F64P, _I32P, _I32P]),
+        "rmx_set_bucket_cutoffs": (C.c_int, [vp, _I32P]),
+        "rmx_get_bucket_cutoffs": (C.c_int, [vp, _I32P]),
+        "rmx_autotune": (C.c_int, [vp, _I32P]),
         "rmx_load_network": (C.c_int, [vp, C.c_char_p, _I32P, _I64P, _I32P, _I32P]),
         "rmx_initiate": (C.c_int, [vp, _I32P, _F64P, _I64P]),
         "rmx_get_flows": (C.c_int, [vp, _F64P]),
@@ -244,6 +247,25 @@ class Network:
         self.n, self.m = int(n), len(out_idx)
         self.n_inst = len(node_off) - 1
         return self
+
+    def set_bucket_cutoffs(self, max_deg):
+        """Upper degree bounds of the 7 evaluation buckets below the hubs."""
+        a = np.ascontiguousarray(max_deg, dtype=np.int32)
+        if a.shape != (7,):
+            raise ValueError("7 cut-offs")
+        self._check(self._L.rmx_set_bucket_cutoffs(self._h, _p(a, _I32P)))
+
+    def bucket_cutoffs(self):
+        a = np.empty(7, dtype=np.int32)
+        self._check(self._L.rmx_get_bucket_cutoffs(self._h, _p(a, _I32P)))
+        return a
+
+    def autotune(self):
+        """Times the evaluation kernels on the current level and installs the
+        fastest cut-offs; returns them."""
+        a = np.empty(7, dtype=np.int32)
+        self._check(self._L.rmx_autotune(self._h, _p(a, _I32P)))
+        return a
 
     def batch_results(self):
         """Per instance: codelength (nats), non-empty modules, steady-state iterations."""

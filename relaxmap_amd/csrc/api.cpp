@@ -22,6 +22,7 @@ using namespace rmx;
 namespace rmx {
 void batch_partition(Ctx &c, const rmx_partition_opts &o, rmx_partition_result &r);   // batch.hip
 }
+static_assert(RMX_BUCKET_CUTS == kNumBuckets - 1, "one cut-off per bucket below the hubs");
 
 struct rmx_ctx {
   Ctx c;
@@ -235,6 +236,58 @@ int rmx_batch_upload(rmx_ctx *ctx, int32_t n_inst, const int32_t *node_off, cons
     RMX_HIP(hipMemcpyAsync(c.b_seed.p, sd.data(), sizeof(uint64_t) * n_inst, hipMemcpyHostToDevice, c.stream));
     c.sync();   // host staging vectors go out of scope
     set_names(ctx, n);
+    return RMX_OK;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+void install_cuts(Ctx &c, const BucketCuts &cuts) {
+  c.cuts = cuts;
+  if (c.L0.n > 0) build_buckets(c, c.L0);
+  if (c.level == 1 && c.L1.n > 0) build_buckets(c, c.L1);
+  c.sync();
+}
+
+}  // namespace
+
+extern "C" {
+
+int rmx_set_bucket_cutoffs(rmx_ctx *ctx, const int32_t *max_deg) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!max_deg) return fail(ctx, RMX_EINVAL, "null cut-offs");
+    BucketCuts cuts = c.cuts;
+    for (int k = 0; k < kNumBuckets - 1; ++k) {
+      if (max_deg[k] < 0 || max_deg[k] > kBucketMaxDeg[k])
+        return fail(ctx, RMX_EINVAL, "cut-off " + std::to_string(k) + " outside 0.." + std::to_string(kBucketMaxDeg[k]));
+      if (k > 0 && max_deg[k] < max_deg[k - 1]) return fail(ctx, RMX_EINVAL, "cut-offs not ascending");
+      cuts.v[k] = max_deg[k];
+    }
+    RMX_HIP(hipSetDevice(c.device));
+    install_cuts(c, cuts);
+    return RMX_OK;
+  });
+}
+
+int rmx_get_bucket_cutoffs(rmx_ctx *ctx, int32_t *max_deg) {
+  return guarded(ctx, [&] {
+    if (!max_deg) return fail(ctx, RMX_EINVAL, "null cut-offs");
+    for (int k = 0; k < kNumBuckets - 1; ++k) max_deg[k] = ctx->c.cuts.v[k];
+    return RMX_OK;
+  });
+}
+
+int rmx_autotune(rmx_ctx *ctx, int32_t *max_deg) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    RMX_HIP(hipSetDevice(c.device));
+    install_cuts(c, autotune_buckets(c));
+    if (max_deg)
+      for (int k = 0; k < kNumBuckets - 1; ++k) max_deg[k] = c.cuts.v[k];
     return RMX_OK;
   });
 }

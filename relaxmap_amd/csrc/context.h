@@ -18,6 +18,12 @@ constexpr int kNumBuckets = 8;
 // the degree, so the cut-offs follow the table sizes (and with them the LDS
 // per workgroup, which sets how many workgroups share a CU)
 constexpr int kBucketMaxDeg[kNumBuckets] = {16, 32, 128, 256, 512, 1023, 2048, 0x7fffffff};
+// Upper degree bounds of the buckets in use (the defaults above, or the
+// autotuner's): bucket b takes the units with degree in (v[b-1], v[b]]; a
+// bound never exceeds the default, the table size of the bucket's kernel.
+struct BucketCuts {
+  int32_t v[kNumBuckets];
+};
 // batches per sweep: (bucket, batch) sort keys must fit one 8-bit digit
 constexpr int kMaxBatches = 256 / kNumBuckets - 1;
 // Hubs are evaluated in partitions of about kHubPart arcs (split by a hash of
@@ -186,6 +192,8 @@ struct Ctx {
   int precision = 0;   // 0 FP64, 1 mixed (FP32 bulk, FP64 accumulation)
   int batches = 8;     // sub-sweeps per sweep
   int hub_global = 0;  // 1: hubs with global-memory tables instead of LDS partitions
+  BucketCuts cuts = {{kBucketMaxDeg[0], kBucketMaxDeg[1], kBucketMaxDeg[2], kBucketMaxDeg[3], kBucketMaxDeg[4],
+                      kBucketMaxDeg[5], kBucketMaxDeg[6], kBucketMaxDeg[7]}};
   bool initiated = false;
 
   Level L0, L1;
@@ -320,6 +328,10 @@ void instance_totals_async(Ctx &c);   // per-instance S and L of the sub-module 
 // ----- contract.hip
 void update_members(Ctx &c);
 void contract(Ctx &c);
+// Times every eligible evaluation kernel on the current level's units of
+// each degree class and returns the fastest cut-offs that keep buckets in
+// kernel order (move.hip).  Evaluation only: the state is left unchanged.
+BucketCuts autotune_buckets(Ctx &c);
 void propagate_labels(Ctx &c);   // level 1 labels back to nodes
 void to_node_level(Ctx &c);      // api.cpp
 

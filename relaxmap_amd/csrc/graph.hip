@@ -96,7 +96,7 @@ __global__ void k_in_from_sorted(const uint32_t *__restrict__ sorted_e, const ui
 }
 
 __global__ void k_bucket_key(const uint32_t *__restrict__ optr, const uint32_t *__restrict__ iptr, int32_t n,
-                             uint32_t *__restrict__ key, uint32_t *__restrict__ maxdeg) {
+                             BucketCuts cuts, uint32_t *__restrict__ key, uint32_t *__restrict__ maxdeg) {
   // grid-stride with a bounded grid and one atomic per workgroup: atomics on
   // a single address serialise across the device
   __shared__ uint32_t s_max[256 / kWave];
@@ -104,7 +104,7 @@ __global__ void k_bucket_key(const uint32_t *__restrict__ optr, const uint32_t *
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t deg = (optr[i + 1] - optr[i]) + (iptr[i + 1] - iptr[i]);
     uint32_t b = 0;
-    while (b + 1 < (uint32_t)kNumBuckets && deg > (uint32_t)kBucketMaxDeg[b]) ++b;
+    while (b + 1 < (uint32_t)kNumBuckets && deg > (uint32_t)cuts.v[b]) ++b;
     key[i] = b;
     mx = deg > mx ? deg : mx;
   }
@@ -234,8 +234,8 @@ void build_buckets(Ctx &c, Level &L) {
   cnt.alloc(16 + 1);
   fill_u32(cnt.p, 0u, 17, s);
   L.bucket.alloc(n);
-  k_bucket_key<<<std::min<unsigned>(grid_for(n, 256), 2048u), 256, 0, s>>>(L.out_ptr.p, L.in_ptr.p, n, keys.p,
-                                                                          cnt.p + 16);
+  k_bucket_key<<<std::min<unsigned>(grid_for(n, 256), 2048u), 256, 0, s>>>(L.out_ptr.p, L.in_ptr.p, n, c.cuts,
+                                                                          keys.p, cnt.p + 16);
   RMX_LAUNCH_CHECK();
   k_small_hist<<<gs_grid(n), 256, 0, s>>>(keys.p, n, cnt.p);
   RMX_LAUNCH_CHECK();

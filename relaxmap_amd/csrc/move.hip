@@ -1637,6 +1637,99 @@ __global__ void k_compact(const uint32_t *__restrict__ flag, const uint32_t *__r
   if (i < n && flag[i]) out[pos[i]] = (uint32_t)i;
 }
 
+// The evaluation kernel of degree bucket bk (below the hubs) over a.list.
+void launch_bucket(int bk, const EvalArgs &a, int32_t cnt, bool f32, hipStream_t s) {
+  switch (bk) {
+    case 0:
+      if (f32) k_eval_group<8, 32, true><<<(unsigned)((cnt + 31) / 32), 256, 0, s>>>(a);
+      else k_eval_group<8, 32, false><<<(unsigned)((cnt + 31) / 32), 256, 0, s>>>(a);
+      break;
+    case 1:
+      if (f32) k_eval_group<16, 64, true><<<(unsigned)((cnt + 15) / 16), 256, 0, s>>>(a);
+      else k_eval_group<16, 64, false><<<(unsigned)((cnt + 15) / 16), 256, 0, s>>>(a);
+      break;
+    case 2:
+      if (f32) k_eval_group<32, 256, true><<<(unsigned)((cnt + 7) / 8), 256, 0, s>>>(a);
+      else k_eval_group<32, 256, false><<<(unsigned)((cnt + 7) / 8), 256, 0, s>>>(a);
+      break;
+    case 3:
+      if (f32) k_eval_group<64, 512, true><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
+      else k_eval_group<64, 512, false><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
+      break;
+    case 4:
+      if (f32) k_eval_group<64, 1024, true><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
+      else k_eval_group<64, 1024, false><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
+      break;
+    case 5:
+      if (f32) k_eval_block<2048, true><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
+      else k_eval_block<2048, false><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
+      break;
+    case 6:
+      if (f32) k_eval_block<4096, true><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
+      else k_eval_block<4096, false><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
+      break;
+    default:
+      break;
+  }
+}
+
+// Arguments of the evaluation kernels for one sweep of level L.
+EvalArgs sweep_args(Ctx &c, Level &L, bool prioritized, double vthresh, uint64_t sweep_seed, bool batch) {
+  EvalArgs a{};
+  a.bucket = L.bucket.p;
+  a.hub_base = L.bucket_off[kNumBuckets - 1];
+  a.optr = L.out_ptr.p;
+  a.oidx = L.out_idx.p;
+  a.iptr = L.in_ptr.p;
+  a.iidx = L.in_idx.p;
+  a.aout = arcw_out(c, L);
+  a.ain = arcw_in(c, L);
+  a.mod = L.mod.p;
+  a.us = L.us.p;
+  a.ms = c.ms.p;
+  a.mUnits = c.mUnits.p;
+  a.mStale = c.mStale.p;
+  a.act = c.act.p;
+  a.all_active = prioritized ? 0 : 1;
+  a.seed = sweep_seed;
+  a.batches = c.batches;
+  a.node_level = c.level == 0 ? 1 : 0;
+  a.empty_key = (uint32_t)c.N + 1u;
+  a.scal = c.scal.p;
+  a.S_inst = c.sub_mode ? c.inst_S.p : (batch ? c.b_S.p : nullptr);
+  a.inst = c.sub_mode ? c.sub_par.p : (batch ? L.uinst.p : nullptr);
+  a.e_cnt = batch ? c.b_ecnt.p : nullptr;
+  a.e_cur = batch ? c.b_ecur.p : nullptr;
+  a.vthresh = prioritized ? vthresh : 0.0;
+  a.target = c.target.p;
+  a.agg = c.agg_buf.p;
+  a.movers_st = c.movers_st.p;
+  a.mv_cap = c.mv_cap;
+  a.counters = c.counters.p;
+  a.stripes = c.counters.p + kStripeBase;
+  a.want = c.want_empty.p;
+  a.hkeys = c.hub_keys.p;
+  a.hout = c.hub_out.p;
+  a.hin = c.hub_in.p;
+  a.hub_off = L.hub_off.p;
+  return a;
+}
+
+// degree classes of the autotuner (upper bounds; above the last: hubs)
+constexpr int kNumClasses = 9;
+constexpr int kClassMax[kNumClasses] = {8, 16, 32, 64, 128, 256, 512, 1023, 2048};
+
+__global__ void k_class_key(const uint32_t *__restrict__ optr, const uint32_t *__restrict__ iptr, int32_t n,
+                            uint32_t *__restrict__ key, uint32_t *__restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t deg = (optr[i + 1] - optr[i]) + (iptr[i + 1] - iptr[i]);
+  uint32_t k = 0;
+  while (k < (uint32_t)kNumClasses && deg > (uint32_t)kClassMax[k]) ++k;
+  key[i] = k;
+  vals[i] = (uint32_t)i;
+}
+
 unsigned stride_grid(int64_t items, int per_block) {
   int64_t g = (items + per_block - 1) / per_block;
   if (g > 8192) g = 8192;
@@ -1816,43 +1909,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     for (int k = 0; k < NK; ++k) seg[k + 1] = seg[k] + h[k];
   }
 
-  EvalArgs a;
-  a.bucket = L.bucket.p;
-  a.hub_base = L.bucket_off[kNumBuckets - 1];
-  a.optr = L.out_ptr.p;
-  a.oidx = L.out_idx.p;
-  a.iptr = L.in_ptr.p;
-  a.iidx = L.in_idx.p;
-  a.aout = arcw_out(c, L);
-  a.ain = arcw_in(c, L);
-  a.mod = L.mod.p;
-  a.us = L.us.p;
-  a.ms = c.ms.p;
-  a.mUnits = c.mUnits.p;
-  a.mStale = c.mStale.p;
-  a.act = c.act.p;
-  a.all_active = prioritized ? 0 : 1;
-  a.seed = sweep_seed;
-  a.batches = c.batches;
-  a.node_level = c.level == 0 ? 1 : 0;
-  a.empty_key = (uint32_t)c.N + 1u;
-  a.scal = c.scal.p;
-  a.S_inst = c.sub_mode ? c.inst_S.p : (batch ? c.b_S.p : nullptr);
-  a.inst = c.sub_mode ? c.sub_par.p : (batch ? L.uinst.p : nullptr);
-  a.e_cnt = batch ? c.b_ecnt.p : nullptr;
-  a.e_cur = batch ? c.b_ecur.p : nullptr;
-  a.vthresh = prioritized ? vthresh : 0.0;
-  a.target = c.target.p;
-  a.agg = c.agg_buf.p;
-  a.movers_st = c.movers_st.p;
-  a.mv_cap = c.mv_cap;
-  a.counters = c.counters.p;
-  a.stripes = c.counters.p + kStripeBase;
-  a.want = c.want_empty.p;
-  a.hkeys = c.hub_keys.p;
-  a.hout = c.hub_out.p;
-  a.hin = c.hub_in.p;
-  a.hub_off = L.hub_off.p;
+  EvalArgs a = sweep_args(c, L, prioritized, vthresh, sweep_seed, batch);
 
   ArcW aout = arcw_out(c, L);
   const bool f32 = c.precision == 1;   // mixed precision: candidates in FP32
@@ -1871,33 +1928,8 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       a.list_n = cnt;
       if (c.prof.on) c.prof.begin(s);
       switch (bk) {
-        case 0:
-          if (f32) k_eval_group<8, 32, true><<<(unsigned)((cnt + 31) / 32), 256, 0, s>>>(a);
-          else k_eval_group<8, 32, false><<<(unsigned)((cnt + 31) / 32), 256, 0, s>>>(a);
-          break;
-        case 1:
-          if (f32) k_eval_group<16, 64, true><<<(unsigned)((cnt + 15) / 16), 256, 0, s>>>(a);
-          else k_eval_group<16, 64, false><<<(unsigned)((cnt + 15) / 16), 256, 0, s>>>(a);
-          break;
-        case 2:
-          if (f32) k_eval_group<32, 256, true><<<(unsigned)((cnt + 7) / 8), 256, 0, s>>>(a);
-          else k_eval_group<32, 256, false><<<(unsigned)((cnt + 7) / 8), 256, 0, s>>>(a);
-          break;
-        case 3:
-          if (f32) k_eval_group<64, 512, true><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
-          else k_eval_group<64, 512, false><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
-          break;
-        case 4:
-          if (f32) k_eval_group<64, 1024, true><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
-          else k_eval_group<64, 1024, false><<<(unsigned)((cnt + 3) / 4), 256, 0, s>>>(a);
-          break;
-        case 5:
-          if (f32) k_eval_block<2048, true><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
-          else k_eval_block<2048, false><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
-          break;
-        case 6:
-          if (f32) k_eval_block<4096, true><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
-          else k_eval_block<4096, false><<<(unsigned)(cnt < 8192 ? cnt : 8192), 256, 0, s>>>(a);
+        case 0: case 1: case 2: case 3: case 4: case 5: case 6:
+          launch_bucket(bk, a, cnt, f32, s);
           break;
         default: {
           if (c.hub_global == 1) {
@@ -1996,7 +2028,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
         empty_cursor += n_want < left ? n_want : left;
       }
     }
-    const uint32_t big_deg = n_hubs > 0 ? (uint32_t)kBucketMaxDeg[kNumBuckets - 2] : 0xffffffffu;
+    const uint32_t big_deg = n_hubs > 0 ? (uint32_t)c.cuts.v[kNumBuckets - 2] : 0xffffffffu;
     k_commit<16><<<stride_grid((int64_t)n_movers * 16, 256), 256, 0, s>>>(
         c.movers.p, c.counters.p, L.out_ptr.p, L.out_idx.p, L.in_ptr.p, L.in_idx.p, aout, L.mod.p, c.newmod.p,
         c.stamp.p, c.stamp_now, c.agg_buf.p, L.us.p, L.nmem.p, c.ms.p, c.mNodes.p, c.mUnits.p, c.act_next.p,
@@ -2044,6 +2076,117 @@ int64_t n_active(Ctx &c) {
   int64_t k = 0;
   for (int32_t i = 0; i < c.cur().n; ++i) k += h[i];
   return k;
+}
+
+BucketCuts autotune_buckets(Ctx &c) {
+  ensure_sweep_buffers(c);
+  Level &L = c.cur();
+  hipStream_t s = c.stream;
+  const int32_t n = L.n;
+  BucketCuts cuts = c.cuts;
+  if (n <= 0) return cuts;
+  // the level's units grouped by degree class
+  DevBuf<uint32_t> key(c.cache), list(c.cache), cnt(c.cache);
+  key.alloc(n);
+  list.alloc(n);
+  cnt.alloc(256);
+  k_class_key<<<grid_for(n, 256), 256, 0, s>>>(L.out_ptr.p, L.in_ptr.p, n, key.p, list.p);
+  RMX_LAUNCH_CHECK();
+  radix_sort_pairs_u32(key.p, list.p, n, 0, 8, c.scratch, s);
+  fill_u32(cnt.p, 0u, 256, s);
+  k_hist256<<<(unsigned)std::min<int64_t>(1024, (n + 255) / 256), 256, 0, s>>>(key.p, n, cnt.p);
+  RMX_LAUNCH_CHECK();
+  std::vector<uint32_t> h(256);
+  RMX_HIP(hipMemcpyAsync(h.data(), cnt.p, sizeof(uint32_t) * 256, hipMemcpyDeviceToHost, s));
+  c.sync();
+  std::vector<uint32_t> off(kNumClasses + 1, 0);
+  for (int k = 0; k < kNumClasses; ++k) off[k + 1] = off[k] + h[k];
+
+  // evaluation only (no empty-module option, nothing committed)
+  const bool batch = c.nb > 0 && !c.sub_mode;
+  EvalArgs a = sweep_args(c, L, false, 0.0, 0, batch);
+  a.empty_avail = 0;
+  a.e_cnt = nullptr;
+  a.e_cur = nullptr;
+  a.batch = 0;
+  const bool f32 = c.precision == 1;
+  constexpr int NK = kNumBuckets - 1;   // evaluation kernels below the hubs
+  constexpr double kInf = 1e300;
+  constexpr uint32_t kMinUnits = 4096;  // smaller classes keep any eligible kernel
+  double cost[kNumClasses][NK];
+  hipEvent_t e0, e1;
+  RMX_HIP(hipEventCreate(&e0));
+  RMX_HIP(hipEventCreate(&e1));
+  for (int cl = 0; cl < kNumClasses; ++cl) {
+    const uint32_t units = h[cl];
+    int natural = 0;   // the smallest kernel whose table fits the class
+    while (natural < NK && kBucketMaxDeg[natural] < kClassMax[cl]) ++natural;
+    for (int k = 0; k < NK; ++k) {
+      cost[cl][k] = kInf;
+      if (kBucketMaxDeg[k] < kClassMax[cl]) continue;        // table too small
+      if (units < kMinUnits) { cost[cl][k] = 0.0; continue; }
+      if (k > natural + 2) continue;                         // far too large tables
+      a.list = list.p + off[cl];
+      a.list_n = (int32_t)units;
+      float best = 1e30f;
+      for (int rep = 0; rep < 2; ++rep) {
+        k_reset_counters<<<1, 64, 0, s>>>(c.counters.p);
+        fill_u32(c.counters.p + kStripeBase, 0u, kStripes * kStripeWords, s);
+        RMX_HIP(hipEventRecord(e0, s));
+        launch_bucket(k, a, (int32_t)units, f32, s);
+        RMX_LAUNCH_CHECK();
+        RMX_HIP(hipEventRecord(e1, s));
+        RMX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        RMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        best = ms < best ? ms : best;
+      }
+      cost[cl][k] = best;
+    }
+  }
+  RMX_HIP(hipEventDestroy(e0));
+  RMX_HIP(hipEventDestroy(e1));
+  k_reset_counters<<<1, 64, 0, s>>>(c.counters.p);
+  fill_u32(c.counters.p + kStripeBase, 0u, kStripes * kStripeWords, s);
+  fill_u32(c.counters.p, 0u, 16, s);
+  c.sync();
+
+  // cheapest assignment of classes to kernels that keeps kernel order
+  // non-decreasing with degree (so that it is a set of cut-offs)
+  double best[kNumClasses][NK];
+  int from[kNumClasses][NK];
+  for (int cl = 0; cl < kNumClasses; ++cl)
+    for (int k = 0; k < NK; ++k) {
+      best[cl][k] = kInf;
+      from[cl][k] = -1;
+      if (cost[cl][k] >= kInf) continue;
+      if (cl == 0) {
+        best[cl][k] = cost[cl][k];
+        continue;
+      }
+      for (int j = 0; j <= k; ++j)
+        if (best[cl - 1][j] < kInf && best[cl - 1][j] + cost[cl][k] < best[cl][k]) {
+          best[cl][k] = best[cl - 1][j] + cost[cl][k];
+          from[cl][k] = j;
+        }
+    }
+  int k_last = 0;
+  for (int k = 1; k < NK; ++k)
+    if (best[kNumClasses - 1][k] < best[kNumClasses - 1][k_last]) k_last = k;
+  if (best[kNumClasses - 1][k_last] >= kInf) return cuts;   // cannot happen: the default is feasible
+  int asg[kNumClasses];
+  for (int cl = kNumClasses - 1, k = k_last; cl >= 0; --cl) {
+    asg[cl] = k;
+    k = from[cl][k];
+  }
+  for (int k = 0; k < NK; ++k) {
+    int32_t v = k == 0 ? 0 : cuts.v[k - 1];
+    for (int cl = 0; cl < kNumClasses; ++cl)
+      if (asg[cl] == k) v = kClassMax[cl];
+    cuts.v[k] = v;
+  }
+  cuts.v[NK] = kBucketMaxDeg[NK];
+  return cuts;
 }
 
 }  // namespace rmx
