@@ -242,7 +242,7 @@ struct Ctx {
   DevBuf<uint32_t> hp_bcnt, hp_boff, hp_bm;
   DevBuf<double> hp_bd;
   DevBuf<long long> hp_bfo, hp_bfi;
-  DevBuf<uint32_t> hp_plan, hp_ovf;
+  DevBuf<uint32_t> hp_plan, hp_ovf, hp_hubu, hp_chub, hp_phub;
   DevBuf<long long> hp_own;
   DevBuf<uint32_t> counters;         // small device counters, then kStripes copies of the launch counters
   DevBuf<uint32_t> movers_st;        // movers per counter copy, mv_cap each

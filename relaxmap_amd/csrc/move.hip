@@ -943,6 +943,9 @@ struct HubParts {
   double *bd;       // per partition: best dL, module, and its flows
   uint32_t *bm;
   long long *bfo, *bfi;
+  uint32_t *hubu;   // per hub: its unit
+  uint32_t *chub;   // per arc chunk: its hub
+  uint32_t *phub;   // per partition: its hub
   int force_ovf;    // treat every partition as overflowed (tests the fallback)
 };
 
@@ -977,6 +980,7 @@ __global__ __launch_bounds__(256) void k_hubp_plan(EvalArgs a, HubParts H) {
     uint32_t ac = 0, pc = 0, dc = 0;
     if (i < a.list_n) {
       const uint32_t u = a.bucket[a.list[i]];
+      H.hubu[i] = u;
       const uint32_t deg = (a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u]);
       ac = (deg + kHubArcChunk - 1) / kHubArcChunk;
       pc = (deg + kHubPart - 1) / kHubPart;
@@ -1002,6 +1006,16 @@ __global__ __launch_bounds__(256) void k_hubp_plan(EvalArgs a, HubParts H) {
   }
 }
 
+// the hub of every arc chunk and of every partition (a wavefront per hub),
+// so that chunk and partition workgroups find their hub with one read
+__global__ __launch_bounds__(256) void k_hubp_owners(EvalArgs a, HubParts H) {
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (i >= a.list_n) return;
+  for (uint32_t c = H.A[i] + lane; c < H.A[i + 1]; c += kWave) H.chub[c] = (uint32_t)i;
+  for (uint32_t p = H.PB[i] + lane; p < H.PB[i + 1]; p += kWave) H.phub[p] = (uint32_t)i;
+}
+
 struct HubChunk {
   int i;              // hub (position in the batch's hub list)
   uint32_t u, o;      // unit and its module
@@ -1010,8 +1024,8 @@ struct HubChunk {
 
 __device__ inline HubChunk hub_chunk(const EvalArgs &a, const HubParts &H, uint32_t c) {
   HubChunk h;
-  h.i = chunk_owner(H.A, a.list_n, c);
-  h.u = a.bucket[a.list[h.i]];
+  h.i = (int)H.chub[c];
+  h.u = H.hubu[h.i];
   h.o = a.mod[h.u];
   h.ob = a.optr[h.u];
   h.od = a.optr[h.u + 1] - h.ob;
@@ -1143,8 +1157,9 @@ __global__ __launch_bounds__(256) void k_hubp_eval(EvalArgs a, HubParts H) {
   unsigned long long cand_acc = 0;   // thread 0
   const uint32_t total = H.tot[1];
   for (uint32_t p = blockIdx.x; p < total; p += gridDim.x) {
-    const int i = chunk_owner(H.PB, a.list_n, p);
-    const uint32_t u = a.bucket[a.list[i]];
+    const int i = (int)H.phub[p];
+    const uint32_t u = H.hubu[i];
+    const UnitLoads ul = load_unit(a, u);   // in flight while the partition aggregates
     const uint32_t e0 = H.boff[p], e1 = H.boff[p + 1];
     for (int sl = threadIdx.x; sl < kHubTable; sl += 256) {
       s_keys[sl] = kEmptyKey;
@@ -1198,7 +1213,7 @@ __global__ __launch_bounds__(256) void k_hubp_eval(EvalArgs a, HubParts H) {
       __builtin_amdgcn_wave_barrier();
     }
     UnitCtx U;
-    unit_ctx(a, u, H.own[2 * i], H.own[2 * i + 1], U);
+    unit_ctx(a, ul, H.own[2 * i], H.own[2 * i + 1], U);
     Best b{0.0, kNone, -1};
     for (int j = wl; j < nw; j += kWave) {
       const int sl = wave_slot(j, w);
@@ -1880,9 +1895,12 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       c.hp_bfi.alloc(L.hub_parts);
     }
     if (c.hp_plan.n < 3 * ((int64_t)n_hubs + 1) + 4) c.hp_plan.alloc(3 * ((int64_t)n_hubs + 1) + 4);
+    if (c.hp_phub.n < L.hub_parts) c.hp_phub.alloc(L.hub_parts);
+    if (c.hp_chub.n < L.hub_arcs / kHubArcChunk + n_hubs + 1) c.hp_chub.alloc(L.hub_arcs / kHubArcChunk + n_hubs + 1);
     if (c.hp_ovf.n < n_hubs) {
       c.hp_ovf.alloc(n_hubs);
       c.hp_own.alloc(2 * (int64_t)n_hubs);
+      c.hp_hubu.alloc(n_hubs);
     }
   }
 
@@ -1961,7 +1979,11 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
           H.bfo = c.hp_bfo.p;
           H.bfi = c.hp_bfi.p;
           H.force_ovf = c.hub_global == 2 ? 1 : 0;
+          H.hubu = c.hp_hubu.p;
+          H.chub = c.hp_chub.p;
+          H.phub = c.hp_phub.p;
           k_hubp_plan<<<1, 256, 0, s>>>(a, H);
+          k_hubp_owners<<<grid_for((int64_t)cnt * kWave, 256), 256, 0, s>>>(a, H);
           fill_u32(H.bcnt, 0u, NB + 1, s);
           fill_i64(H.own, 0, 2 * (int64_t)cnt, s);
           fill_u32(H.ovf, 0u, cnt, s);
