@@ -927,6 +927,7 @@ __global__ __launch_bounds__(256) void k_hub_finish(EvalArgs a, HubPlan P) {
 constexpr int kHubTable = 2 * kHubPart;   // LDS slots of a partition
 constexpr int kHubHist = 4096;            // LDS bins of a chunk's histogram (more: global atomics)
 constexpr uint32_t kInBit = 0x80000000u;  // entry direction: in-arc
+constexpr uint32_t kOwnArc = 0xffffffffu; // an arc into the hub's own module (module ids stay below 2^31 - 1)
 
 struct HubParts {
   uint32_t *A;      // arc-chunk prefix per hub (n_h + 1)
@@ -1057,11 +1058,13 @@ __global__ __launch_bounds__(256) void k_hubp_count(EvalArgs a, HubParts H) {
         const bool out = j < h.od;
         const uint32_t k = out ? h.ob + j : h.ib + (j - h.od);
         const uint32_t m = a.mod[out ? a.oidx[k] : a.iidx[k]];
-        H.mtmp[H.AB[h.i] + j] = out ? m : (m | kInBit);
-        if (m == h.o) {
+        if (m == h.o) {   // own module: only its sums are needed, the arc is not partitioned
+          H.mtmp[H.AB[h.i] + j] = kOwnArc;
           if (out) so += a.aout.fxw(k);
           else si += a.ain.fxw(k);
+          continue;
         }
+        H.mtmp[H.AB[h.i] + j] = out ? m : (m | kInBit);
         q = part_of(m, h.np);
       }
       if (q != run_q) {
@@ -1120,8 +1123,10 @@ __global__ __launch_bounds__(256) void k_hubp_scatter(EvalArgs a, HubParts H) {
       key[q] = 0;
       if (j < h.j1) {
         key[q] = H.mtmp[H.AB[h.i] + j];
-        part[q] = part_of(key[q] & ~kInBit, h.np);
-        rank[q] = local ? atomicAdd(&hcnt[part[q]], 1u) : atomicAdd(&H.bcnt[H.PB[h.i] + part[q]], 1u);
+        if (key[q] != kOwnArc) {
+          part[q] = part_of(key[q] & ~kInBit, h.np);
+          rank[q] = local ? atomicAdd(&hcnt[part[q]], 1u) : atomicAdd(&H.bcnt[H.PB[h.i] + part[q]], 1u);
+        }
       }
     }
     __syncthreads();
@@ -1132,7 +1137,7 @@ __global__ __launch_bounds__(256) void k_hubp_scatter(EvalArgs a, HubParts H) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const uint32_t j = h.j0 + (uint32_t)(q * 256) + threadIdx.x;
-      if (j < h.j1) {
+      if (j < h.j1 && key[q] != kOwnArc) {
         const bool out = j < h.od;
         const uint32_t k = out ? h.ob + j : h.ib + (j - h.od);
         const uint32_t pos = H.boff[H.PB[h.i] + part[q]] + (local ? hbase[part[q]] : 0u) + rank[q];
