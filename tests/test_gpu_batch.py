@@ -137,3 +137,36 @@ def test_batch_deterministic():
     r1, _ = run_batched(graphs, [3, 4])
     r2, _ = run_batched(graphs, [3, 4])
     assert r1 == r2
+
+
+class _Net:
+    """A network given by its out-CSR (the attributes union_of reads)."""
+
+    def __init__(self, n, ptr, idx, w):
+        self.n, self.ptr, self.idx, self.w = n, np.asarray(ptr), np.asarray(idx), np.asarray(w, dtype=np.float64)
+        self.m = len(self.idx)
+
+
+def test_batch_degenerate_instances():
+    """One node, one arc, a 3-cycle and a dangling pair beside an SBM: every
+    instance is partitioned on its own and matches the oracle."""
+    from relaxmap_amd import synth
+    from relaxmap_amd.batch import run_batched, union_of
+    graphs = [_Net(1, [0, 0], [], []), _Net(2, [0, 1, 1], [1], [1.0]), _Net(3, [0, 1, 2, 3], [1, 2, 0], [1, 1, 1]),
+              _Net(4, [0, 1, 1, 2, 2], [1, 3], [2.0, 0.5]), synth.sbm(300, 6, 8.0, 0.2, 5)]
+    for prec in ("fp64", "mixed"):
+        res, _, net = run_batched(graphs, [1, 2], precision=prec, return_net=True)
+        with net:
+            part = net.partition()
+            p = net.flows()
+        pairs, node_off, _, _, _, _ = union_of(graphs, [1, 2])
+        for k, (gi, s) in enumerate(pairs):
+            a, b = int(node_off[k]), int(node_off[k + 1])
+            assert abs(p[a:b].sum() - 1.0) <= 1e-12, (prec, gi)
+            assert part[a:b].min() >= a and part[a:b].max() < b, (prec, gi)
+            if graphs[gi].m == 0:
+                assert res[(gi, s)][2] == 0.0 and res[(gi, s)][1] == 1
+                continue
+            ref = _oracle_codelength(graphs[gi], part[a:b] - a)
+            tol = 1e-9 if prec == "fp64" else 1e-6
+            assert abs(res[(gi, s)][2] - ref) <= tol * max(abs(ref), 1e-12), (prec, gi, res[(gi, s)][2], ref)
