@@ -27,7 +27,7 @@ extern "C" {
 #define RMX_EIO (-5)       /* file could not be read or written */
 
 #define RMX_FP64 0         /* FP64 throughout */
-#define RMX_MIXED 1        /* FP32 arc weights / SpMV bulk, FP64 accumulation */
+#define RMX_MIXED 1        /* FP32 arc weights and flows in bulk; exact fixed-point or FP64 sums */
 
 typedef struct rmx_ctx rmx_ctx;
 

@@ -10,17 +10,18 @@
 // L1, so its iterate is then within beta / (1 - beta) r < 6e-8 of the fixed
 // point (or when ten iterations no longer halve r, at the FP32 noise floor).
 //
-// The pull is bucketed by in-degree: a group of G in {4..64} lanes per row
-// streams the row's in-arcs (coalesced in_idx / in_w) and gathers x; rows
-// with more than 256 in-arcs get a 256-lane workgroup.  Mixed precision
-// streams FP32 weights and an FP32 copy of x, accumulating in FP64; the
-// residual, the normalisation and D stay FP64.
+// FP64: the pull is bucketed by in-degree: a group of G in {4..64} lanes per
+// row streams the row's in-arcs (coalesced in_idx / in_w) and gathers x;
+// rows with more than 256 in-arcs get a 256-lane workgroup.  It runs on a
+// relabelled copy of the in-CSR: nodes in order of descending out-degree, so
+// the sources read most often share a compact, cache-resident part of x
+// (the gathers of x dominate the traffic).  Every row keeps its arcs in
+// their original order, so each row sum is the same as on the original
+// labels; p is mapped back after the last iteration.
 //
-// The iteration runs on a relabelled copy of the in-CSR: nodes in order of
-// descending out-degree, so the sources read most often share a compact,
-// cache-resident part of x (the gathers of x dominate the traffic).  Every
-// row keeps its arcs in their original order, so each row sum is the same
-// as on the original labels; p is mapped back after the last iteration.
+// Mixed precision: a blocked push / pull on the original labels, FP32
+// weights and an FP32 copy of x, exact fixed-point row sums (below); the
+// residual, the normalisation and D stay FP64.
 #include "context.h"
 
 #include <algorithm>
@@ -52,10 +53,12 @@ __device__ inline long long group_sum_ll(long long v) {
 }
 
 // --- row-normalised out weights (Module.cpp:237-247) --------------------
-template <int G>
+// FP64: also into the in-CSR (the pull); mixed: an FP32 copy in out-CSR
+// order (the push).
+template <int G, bool MIXED>
 __global__ void k_normalize(const uint32_t *__restrict__ optr, int32_t n, const double *__restrict__ raw,
                             const uint32_t *__restrict__ in_of_out, double *__restrict__ out_w,
-                            double *__restrict__ in_w, float *__restrict__ in_wf, uint8_t *__restrict__ dangling) {
+                            double *__restrict__ in_w, float *__restrict__ out_wf, uint8_t *__restrict__ dangling) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t r = t / G;
   const int lane = (int)(t % G);
@@ -68,9 +71,8 @@ __global__ void k_normalize(const uint32_t *__restrict__ optr, int32_t n, const 
   for (uint32_t k = b + lane; k < e; k += G) {
     double v = raw[k] / s;
     out_w[k] = v;
-    uint32_t ik = in_of_out[k];
-    in_w[ik] = v;
-    in_wf[ik] = (float)v;
+    if (MIXED) out_wf[k] = (float)v;
+    else in_w[in_of_out[k]] = v;
   }
 }
 
@@ -99,12 +101,11 @@ __global__ __launch_bounds__(256) void k_row_bin(const uint32_t *__restrict__ ip
   }
 }
 
-// --- pull SpMV ----------------------------------------------------------
-template <int G, bool MIXED>
+// --- pull SpMV (FP64) ---------------------------------------------------
+template <int G>
 __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rows, int32_t nrows,
                                               const uint32_t *__restrict__ iptr, const uint32_t *__restrict__ iidx,
-                                              const double *__restrict__ iw, const float *__restrict__ iwf,
-                                              const double *__restrict__ x, const float *__restrict__ xf,
+                                              const double *__restrict__ iw, const double *__restrict__ x,
                                               const double *__restrict__ tau, const double *__restrict__ Dv,
                                               const uint32_t *__restrict__ rinst, double *__restrict__ y) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -114,21 +115,15 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rows,
   const uint32_t r = rows[g];
   const uint32_t b = iptr[r], e = iptr[r + 1];
   double acc = 0.0;
-  if (MIXED) {
-    for (uint32_t k = b + lane; k < e; k += G) acc += (double)(iwf[k] * xf[iidx[k]]);
-  } else {
-    for (uint32_t k = b + lane; k < e; k += G) acc = fma(iw[k], x[iidx[k]], acc);
-  }
+  for (uint32_t k = b + lane; k < e; k += G) acc = fma(iw[k], x[iidx[k]], acc);
   acc = group_sum<G>(acc);
   if (lane == 0) y[r] = (kAlpha + kBeta * Dv[rinst ? rinst[r] : 0]) * tau[r] + kBeta * acc;
 }
 
-template <bool MIXED>
 __global__ __launch_bounds__(256) void k_spmv_block(const uint32_t *__restrict__ rows, int32_t nrows,
                                                     const uint32_t *__restrict__ iptr,
                                                     const uint32_t *__restrict__ iidx, const double *__restrict__ iw,
-                                                    const float *__restrict__ iwf, const double *__restrict__ x,
-                                                    const float *__restrict__ xf, const double *__restrict__ tau,
+                                                    const double *__restrict__ x, const double *__restrict__ tau,
                                                     const double *__restrict__ Dv,
                                                     const uint32_t *__restrict__ rinst, double *__restrict__ y) {
   __shared__ double lds[4];
@@ -136,13 +131,172 @@ __global__ __launch_bounds__(256) void k_spmv_block(const uint32_t *__restrict__
     const uint32_t r = rows[g];
     const uint32_t b = iptr[r], e = iptr[r + 1];
     double acc = 0.0;
-    if (MIXED) {
-      for (uint32_t k = b + threadIdx.x; k < e; k += 256) acc += (double)(iwf[k] * xf[iidx[k]]);
-    } else {
-      for (uint32_t k = b + threadIdx.x; k < e; k += 256) acc = fma(iw[k], x[iidx[k]], acc);
-    }
+    for (uint32_t k = b + threadIdx.x; k < e; k += 256) acc = fma(iw[k], x[iidx[k]], acc);
     acc = block_sum<double, 256>(acc, lds);
     if (threadIdx.x == 0) y[r] = (kAlpha + kBeta * Dv[rinst ? rinst[r] : 0]) * tau[r] + kBeta * acc;
+  }
+}
+
+// --- mixed precision: blocked push / pull ---------------------------------
+// The heads are cut into bins of R consecutive rows.  Every out-arc has a
+// fixed slot in a bin-major layout whose arcs keep their out-CSR (tail)
+// order inside each bin.  Per iteration the push streams the out-CSR and
+// writes x_tail * w into its slot: successive tails fill every bin's slots
+// in increasing order, so the writes land in a few lines per bin.  The pull
+// sums each bin's slots into its R rows in LDS.  The sums are exact
+// fixed-point (2^62) sums of the FP32 products, independent of their order.
+constexpr int kPushTile = 2048;             // arcs per push workgroup: 256 lanes x 8 consecutive arcs
+// out_ptr entries the push keeps in LDS.  The 40 KB also hold the push to 4
+// workgroups per CU: fewer tiles in flight narrow the range of slots each
+// bin is written at, so L2 completes its lines before evicting them
+// (3.4 ms instead of 4.5 ms at 8 workgroups per CU on 529M arcs).
+constexpr int kPushRows = 10240;
+constexpr int kPullRowsMax = 8192;          // rows per bin, at most (64 KB of LDS accumulators)
+constexpr int kPullBlock = 512;
+constexpr double kPullInv = 1.0 / 4611686018427387904.0;   // 2^-62
+constexpr float kPullScale = 4611686018427387904.0f;       // 2^62
+
+// tile t starts at arc t * kPushTile; trow[t] = the row that holds arc
+// min(t * kPushTile, m - 1), the largest r with optr[r] <= that arc
+__global__ void k_pb_tile_rows(const uint32_t *__restrict__ optr, int32_t n, int64_t m, int64_t ntiles,
+                               uint32_t *__restrict__ trow) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  const int64_t k = std::min<int64_t>(t * kPushTile, m - 1);
+  int64_t lo = 0, hi = n;   // optr[lo] <= k < optr[hi]
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)optr[mid] <= k) lo = mid;
+    else hi = mid;
+  }
+  trow[t] = (uint32_t)lo;
+}
+
+__global__ void k_pb_keys(const uint32_t *__restrict__ oidx, int64_t m, int rshift, uint32_t *__restrict__ key) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) key[k] = oidx[k] >> rshift;
+}
+
+// bptr[b] = first position of bin b in the sorted bin keys
+__global__ void k_pb_bounds(const uint32_t *__restrict__ skey, int64_t m, int32_t nbins,
+                            uint32_t *__restrict__ bptr) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbins) return;
+  int64_t lo = 0, hi = m;   // first position with key >= b
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)skey[mid] < b) lo = mid + 1;
+    else hi = mid;
+  }
+  bptr[b] = (uint32_t)lo;
+}
+
+// slot of every out-arc, and the head of every slot relative to its bin
+__global__ void k_pb_layout(const uint32_t *__restrict__ sarc, const uint32_t *__restrict__ oidx, int64_t m,
+                            int rshift, uint32_t *__restrict__ slot, uint16_t *__restrict__ bdst) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const uint32_t a = sarc[j];
+  slot[a] = (uint32_t)j;
+  bdst[j] = (uint16_t)(oidx[a] & ((1u << rshift) - 1u));
+}
+
+// Push: workgroups are spread so that each of the 8 XCDs (workgroup i runs
+// on XCD i mod 8) takes one contiguous eighth of the tiles; every bin then
+// receives each XCD's writes in one contiguous range.  The streamed slots
+// and weights are read non-temporally, so that L2 keeps the partly written
+// lines of the bins until they are complete.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ inline u32x4 stream_u4(const uint32_t *p) { return __builtin_nontemporal_load((const u32x4 *)p); }
+__device__ inline f32x4 stream_f4(const float *p) { return __builtin_nontemporal_load((const f32x4 *)p); }
+
+__global__ __launch_bounds__(256) void k_pb_push(const uint32_t *__restrict__ optr, const uint32_t *__restrict__ trow,
+                                                 int64_t ntiles, int64_t per_xcd, int64_t m,
+                                                 const uint32_t *__restrict__ slot, const float *__restrict__ wf,
+                                                 const float *__restrict__ xf, float *__restrict__ val) {
+  __shared__ uint32_t sp[kPushRows];
+  const int64_t tile = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (tile >= ntiles) return;
+  const int64_t a0 = tile * kPushTile;
+  const int64_t a1 = std::min<int64_t>(m, a0 + kPushTile);
+  const uint32_t r0 = trow[tile], r1 = trow[tile + 1];
+  const int64_t nr = (int64_t)r1 - r0 + 1;   // rows r0..r1 cover the tile; their bounds are optr[r0..r1+1]
+  const bool in_lds = nr + 1 <= kPushRows;
+  if (in_lds)
+    for (int64_t i = threadIdx.x; i <= nr; i += 256) sp[i] = optr[r0 + i];
+  __syncthreads();
+  const int64_t k0 = a0 + 8 * (int64_t)threadIdx.x;
+  if (k0 >= a1) return;
+  const uint32_t *bounds = in_lds ? sp : optr + r0;
+  // the row of k0: the largest i < nr with bounds[i] <= k0 (bounds[0] <= a0)
+  int64_t lo = 0, hi = nr;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)bounds[mid] <= k0) lo = mid;
+    else hi = mid;
+  }
+  int64_t row = lo;
+  uint32_t nxt = bounds[row + 1];
+  float xr = xf[r0 + row];
+  if (k0 + 8 <= a1) {
+    const u32x4 s0 = stream_u4(slot + k0), s1 = stream_u4(slot + k0 + 4);
+    const f32x4 w0 = stream_f4(wf + k0), w1 = stream_f4(wf + k0 + 4);
+    const uint32_t sl[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+    const float ww[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t k = (uint32_t)(k0 + j);
+      if (k >= nxt) {
+        do {
+          ++row;
+          nxt = bounds[row + 1];
+        } while (k >= nxt);
+        xr = xf[r0 + row];
+      }
+      val[sl[j]] = xr * ww[j];
+    }
+  } else {
+    for (int64_t k = k0; k < a1; ++k) {
+      if ((uint32_t)k >= nxt) {
+        do {
+          ++row;
+          nxt = bounds[row + 1];
+        } while ((uint32_t)k >= nxt);
+        xr = xf[r0 + row];
+      }
+      val[slot[k]] = xr * wf[k];
+    }
+  }
+}
+
+// Pull: one workgroup per bin of R rows
+__global__ __launch_bounds__(kPullBlock) void k_pb_pull(const uint32_t *__restrict__ bptr,
+                                                        const uint16_t *__restrict__ bdst,
+                                                        const float *__restrict__ val, int32_t R, int32_t n,
+                                                        const double *__restrict__ tau,
+                                                        const double *__restrict__ Dv,
+                                                        const uint32_t *__restrict__ rinst, double *__restrict__ y) {
+  __shared__ unsigned long long acc[kPullRowsMax];
+  for (int i = threadIdx.x; i < R; i += kPullBlock) acc[i] = 0ull;
+  __syncthreads();
+  const uint32_t b = bptr[blockIdx.x], e = bptr[blockIdx.x + 1];
+  uint32_t j = b + threadIdx.x;
+  for (; (int64_t)j + 3 * kPullBlock < (int64_t)e; j += 4 * kPullBlock) {
+    const uint16_t d0 = bdst[j], d1 = bdst[j + kPullBlock], d2 = bdst[j + 2 * kPullBlock],
+                   d3 = bdst[j + 3 * kPullBlock];
+    const float v0 = val[j], v1 = val[j + kPullBlock], v2 = val[j + 2 * kPullBlock], v3 = val[j + 3 * kPullBlock];
+    atomicAdd(&acc[d0], (unsigned long long)(v0 * kPullScale));
+    atomicAdd(&acc[d1], (unsigned long long)(v1 * kPullScale));
+    atomicAdd(&acc[d2], (unsigned long long)(v2 * kPullScale));
+    atomicAdd(&acc[d3], (unsigned long long)(v3 * kPullScale));
+  }
+  for (; j < e; j += kPullBlock) atomicAdd(&acc[bdst[j]], (unsigned long long)(val[j] * kPullScale));
+  __syncthreads();
+  const int64_t row0 = (int64_t)blockIdx.x * R;
+  for (int i = threadIdx.x; i < R && row0 + i < n; i += kPullBlock) {
+    const int64_t r = row0 + i;
+    y[r] = (kAlpha + kBeta * Dv[rinst ? rinst[r] : 0]) * tau[r] + kBeta * ((double)acc[i] * kPullInv);
   }
 }
 
@@ -428,7 +582,6 @@ struct PullCSR {
   int32_t n = 0;
   const uint32_t *ptr = nullptr, *idx = nullptr;
   const double *w = nullptr;
-  const float *wf = nullptr;
   const double *tau = nullptr;
   const uint8_t *dangling = nullptr;
   const double *Dv = nullptr;        // dangling mass: one value, or one per instance (batch)
@@ -458,12 +611,11 @@ __global__ void k_perm_rows(const uint32_t *__restrict__ inv, int32_t n, const u
 }
 
 // G lanes per new row: the node's in-arcs in their order, tails relabelled
-template <int G, bool MIXED>
+template <int G>
 __global__ void k_perm_arcs(const uint32_t *__restrict__ inv, const uint32_t *__restrict__ perm, int32_t n,
                             const uint32_t *__restrict__ iptr, const uint32_t *__restrict__ iidx,
-                            const double *__restrict__ iw, const float *__restrict__ iwf,
-                            const uint32_t *__restrict__ pptr, uint32_t *__restrict__ pidx, double *__restrict__ pw,
-                            float *__restrict__ pwf) {
+                            const double *__restrict__ iw, const uint32_t *__restrict__ pptr,
+                            uint32_t *__restrict__ pidx, double *__restrict__ pw) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t r = t / G;
   const int lane = (int)(t % G);
@@ -473,8 +625,7 @@ __global__ void k_perm_arcs(const uint32_t *__restrict__ inv, const uint32_t *__
   for (uint32_t k = b + lane; k < e; k += G) {
     const uint32_t j = o + (k - b);
     pidx[j] = perm[iidx[k]];
-    if (MIXED) pwf[j] = iwf[k];
-    else pw[j] = iw[k];
+    pw[j] = iw[k];
   }
 }
 
@@ -495,40 +646,107 @@ struct RowBins {
 // Algorithmic bytes of one bin launch: per in-arc the source id, the weight
 // and the gathered flow (4 + w + x); per row the row id, its CSR bounds, tau
 // and the written result (4 + 8 + 8 + 8).
-template <int G, bool MIXED>
+template <int G>
 void launch_spmv_bin(int bin, const RowBins &rb, const PullCSR &L, Ctx &c, double *y) {
   // profiling entries carry the kernels' own names
-  static const char *names[kNumRowBins] = {"k_spmv<4, ", "k_spmv<8, ", "k_spmv<16, ", "k_spmv<32, ", "k_spmv<64, ",
-                                           "k_spmv_block<"};
+  static const char *names[kNumRowBins] = {"k_spmv<4>", "k_spmv<8>", "k_spmv<16>", "k_spmv<32>", "k_spmv<64>",
+                                           "k_spmv_block"};
   const int32_t cnt = rb.off[bin + 1] - rb.off[bin];
   if (cnt <= 0) return;
   const uint32_t *rows = rb.rows.p + rb.off[bin];
   if (c.prof.on) c.prof.begin(c.stream);
   if (G > 0) {
     constexpr int GG = G > 0 ? G : 1;
-    k_spmv<GG, MIXED><<<grid_for((int64_t)cnt * GG, 256), 256, 0, c.stream>>>(
-        rows, cnt, L.ptr, L.idx, L.w, L.wf, c.x.p, c.xf.p, L.tau, L.Dv, L.rinst, y);
+    k_spmv<GG><<<grid_for((int64_t)cnt * GG, 256), 256, 0, c.stream>>>(rows, cnt, L.ptr, L.idx, L.w, c.x.p,
+                                                                        L.tau, L.Dv, L.rinst, y);
   } else {
     unsigned g = cnt < 4096 ? (unsigned)cnt : 4096u;
-    k_spmv_block<MIXED><<<g, 256, 0, c.stream>>>(rows, cnt, L.ptr, L.idx, L.w, L.wf, c.x.p, c.xf.p, L.tau, L.Dv,
-                                                 L.rinst, y);
+    k_spmv_block<<<g, 256, 0, c.stream>>>(rows, cnt, L.ptr, L.idx, L.w, c.x.p, L.tau, L.Dv, L.rinst, y);
   }
   RMX_LAUNCH_CHECK();
-  if (c.prof.on) {
-    const double per_arc = MIXED ? 12.0 : 20.0;
-    c.prof.end(c.stream, std::string(names[bin]) + (MIXED ? "true>" : "false>"),
-               (double)rb.arcs[bin] * per_arc + (double)cnt * 28.0);
-  }
+  if (c.prof.on) c.prof.end(c.stream, names[bin], (double)rb.arcs[bin] * 20.0 + (double)cnt * 28.0);
 }
 
-template <bool MIXED>
 void spmv_all(Ctx &c, const PullCSR &L, const RowBins &rb, double *y) {
-  launch_spmv_bin<4, MIXED>(0, rb, L, c, y);
-  launch_spmv_bin<8, MIXED>(1, rb, L, c, y);
-  launch_spmv_bin<16, MIXED>(2, rb, L, c, y);
-  launch_spmv_bin<32, MIXED>(3, rb, L, c, y);
-  launch_spmv_bin<64, MIXED>(4, rb, L, c, y);
-  launch_spmv_bin<0, MIXED>(5, rb, L, c, y);
+  launch_spmv_bin<4>(0, rb, L, c, y);
+  launch_spmv_bin<8>(1, rb, L, c, y);
+  launch_spmv_bin<16>(2, rb, L, c, y);
+  launch_spmv_bin<32>(3, rb, L, c, y);
+  launch_spmv_bin<64>(4, rb, L, c, y);
+  launch_spmv_bin<0>(5, rb, L, c, y);
+}
+
+// The bin-major layout of the push / pull (mixed precision), built once per
+// steady state from the out-CSR.
+struct PushBins {
+  int32_t R = 0, nbins = 0, rshift = 0;
+  int64_t ntiles = 0;
+  DevBuf<uint32_t> bptr, slot, trow;
+  DevBuf<uint16_t> bdst;
+  DevBuf<float> val;
+  explicit PushBins(BlockCache &bc) : bptr(bc), slot(bc), trow(bc), bdst(bc), val(bc) {}
+};
+
+// Rows per bin: a power of two from 256 to kPullRowsMax, about n / 2048, so
+// that a large network gives thousands of bins (workgroups of the pull).
+// The slots come from a stable sort of the out-arcs by bin.
+void build_push_bins(Ctx &c, const Level &L, PushBins &pb) {
+  hipStream_t s = c.stream;
+  const int32_t n = L.n;
+  const int64_t m = L.m;
+  int rs = 8;
+  while ((1 << rs) < kPullRowsMax && ((int64_t)1 << rs) * 2048 < n) ++rs;
+  pb.rshift = rs;
+  pb.R = 1 << rs;
+  pb.nbins = (int32_t)(((int64_t)n + pb.R - 1) >> rs);
+  pb.bptr.alloc((int64_t)pb.nbins + 1);
+  if (m == 0) {
+    fill_u32(pb.bptr.p, 0u, (int64_t)pb.nbins + 1, s);
+    return;
+  }
+  pb.ntiles = (m + kPushTile - 1) / kPushTile;
+  pb.trow.alloc(pb.ntiles + 1);
+  k_pb_tile_rows<<<grid_for(pb.ntiles + 1, 256), 256, 0, s>>>(L.out_ptr.p, n, m, pb.ntiles, pb.trow.p);
+  RMX_LAUNCH_CHECK();
+  DevBuf<uint32_t> key(c.cache), arc(c.cache);
+  key.alloc(m);
+  arc.alloc(m);
+  k_pb_keys<<<grid_for(m, 256), 256, 0, s>>>(L.out_idx.p, m, rs, key.p);
+  RMX_LAUNCH_CHECK();
+  iota_u32(arc.p, m, s);
+  if (pb.nbins > 1) {
+    int bits = 1;
+    while (((int64_t)1 << bits) < pb.nbins) ++bits;
+    radix_sort_pairs_u32(key.p, arc.p, m, 0, bits, c.scratch, s);
+  }
+  k_pb_bounds<<<grid_for((int64_t)pb.nbins + 1, 256), 256, 0, s>>>(key.p, m, pb.nbins, pb.bptr.p);
+  RMX_LAUNCH_CHECK();
+  pb.slot.alloc(m);
+  pb.bdst.alloc(m);
+  pb.val.alloc(m);
+  k_pb_layout<<<grid_for(m, 256), 256, 0, s>>>(arc.p, L.out_idx.p, m, rs, pb.slot.p, pb.bdst.p);
+  RMX_LAUNCH_CHECK();
+}
+
+// y = one iteration's unnormalised p' from xf.  Algorithmic bytes: the push
+// reads per arc its slot and FP32 weight and writes its product (12), per
+// row its CSR bound and x (8); the pull reads per arc the product and its
+// 16-bit head (6), per row tau and writes y (16, and 4 for the instance).
+void push_pull(Ctx &c, const PushBins &pb, const Level &L, const PullCSR &F, double *y) {
+  hipStream_t s = c.stream;
+  if (pb.ntiles > 0) {
+    const int64_t per = (pb.ntiles + 7) / 8;
+    if (c.prof.on) c.prof.begin(s);
+    k_pb_push<<<(unsigned)(8 * per), 256, 0, s>>>(L.out_ptr.p, pb.trow.p, pb.ntiles, per, L.m, pb.slot.p,
+                                                    L.out_wf.p, c.xf.p, pb.val.p);
+    RMX_LAUNCH_CHECK();
+    if (c.prof.on) c.prof.end(s, "k_pb_push", (double)L.m * 12.0 + (double)L.n * 8.0);
+  }
+  if (c.prof.on) c.prof.begin(s);
+  k_pb_pull<<<(unsigned)pb.nbins, kPullBlock, 0, s>>>(pb.bptr.p, pb.bdst.p, pb.val.p, pb.R, L.n, F.tau, F.Dv,
+                                                      F.rinst, y);
+  RMX_LAUNCH_CHECK();
+  if (c.prof.on) c.prof.end(s, "k_pb_pull", (double)L.m * 6.0 + (double)L.n * (F.rinst ? 20.0 : 16.0));
 }
 
 }  // namespace
@@ -599,25 +817,32 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
     RMX_LAUNCH_CHECK();
   }
 
-  k_normalize<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, c.raw_w.p, c.in_of_out.p, L.out_w.p,
-                                                                  L.in_w.p, L.in_wf.p, c.dangling.p);
+  if (mixed)
+    k_normalize<16, true><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(
+        L.out_ptr.p, n, c.raw_w.p, c.in_of_out.p, L.out_w.p, L.in_w.p, L.out_wf.p, c.dangling.p);
+  else
+    k_normalize<16, false><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(
+        L.out_ptr.p, n, c.raw_w.p, c.in_of_out.p, L.out_w.p, L.in_w.p, L.out_wf.p, c.dangling.p);
   RMX_LAUNCH_CHECK();
-  // the relabelled pull copy (descending out-degree), built on the device
+  // FP64: the relabelled pull copy (descending out-degree), built on the
+  // device; mixed: the push / pull bins on the original labels
   DevBuf<uint32_t> inv(c.cache), perm(c.cache), pptr(c.cache), pidx(c.cache), pdeg(c.cache);
   DevBuf<double> pw(c.cache), ptau(c.cache);
   DevBuf<float> pwf(c.cache);
   DevBuf<uint8_t> pdang(c.cache);
+  PushBins pb(c.cache);
   PullCSR F;
   F.n = n;
-  if (batch) {   // instances keep their node ranges: no relabelling
+  const bool relabel = !batch && !mixed;
+  if (!relabel) {   // instances keep their node ranges
     F.ptr = L.in_ptr.p;
     F.idx = L.in_idx.p;
     F.w = L.in_w.p;
-    F.wf = L.in_wf.p;
     F.tau = L.tau.p;
     F.dangling = c.dangling.p;
-    F.Dv = c.b_D.p;
-    F.rinst = c.b_node.p;
+    F.Dv = batch ? c.b_D.p : c.scal.p + kD;
+    F.rinst = batch ? c.b_node.p : nullptr;
+    if (mixed) build_push_bins(c, L, pb);
   } else {
     F.Dv = c.scal.p + kD;
     inv.alloc(n);
@@ -627,8 +852,7 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
     ptau.alloc(n);
     pdang.alloc(n);
     pidx.alloc(m);
-    if (mixed) pwf.alloc(m);
-    else pw.alloc(m);
+    pw.alloc(m);
     DevBuf<uint32_t> key(c.cache);
     key.alloc(n);
     k_out_degree_key<<<grid_for(n, 256), 256, 0, s>>>(L.out_ptr.p, n, key.p);
@@ -643,23 +867,18 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
     RMX_LAUNCH_CHECK();
     exclusive_scan_u32(pdeg.p, pptr.p, (int64_t)n + 1, nullptr, c.scratch, s);
     if (m > 0) {
-      if (mixed)
-        k_perm_arcs<16, true><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(
-            inv.p, perm.p, n, L.in_ptr.p, L.in_idx.p, L.in_w.p, L.in_wf.p, pptr.p, pidx.p, nullptr, pwf.p);
-      else
-        k_perm_arcs<16, false><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(
-            inv.p, perm.p, n, L.in_ptr.p, L.in_idx.p, L.in_w.p, L.in_wf.p, pptr.p, pidx.p, pw.p, nullptr);
+      k_perm_arcs<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(inv.p, perm.p, n, L.in_ptr.p, L.in_idx.p,
+                                                                      L.in_w.p, pptr.p, pidx.p, pw.p);
       RMX_LAUNCH_CHECK();
     }
     F.ptr = pptr.p;
     F.idx = pidx.p;
     F.w = pw.p;
-    F.wf = pwf.p;
     F.tau = ptau.p;
     F.dangling = pdang.p;
   }
   RowBins rb;
-  make_row_bins(c, F, rb);
+  if (!mixed) make_row_bins(c, F, rb);
 
   hipEvent_t e0, e1;
   RMX_HIP(hipEventCreate(&e0));
@@ -679,8 +898,8 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
     const double b_tol = mixed ? kMixedTol : tol;
     uint32_t running = 1;
     while (running > 0) {
-      if (mixed) spmv_all<true>(c, F, rb, c.y.p);
-      else spmv_all<false>(c, F, rb, c.y.p);
+      if (mixed) push_pull(c, pb, L, F, c.y.p);
+      else spmv_all(c, F, rb, c.y.p);
       k_bnorm<<<nb, kRedBlock, 0, s>>>(c.b_off.p, c.y.p, c.dangling.p, done.p, c.x.p, c.xf.p, c.b_D.p, c.b_R.p);
       RMX_LAUNCH_CHECK();
       ++it;
@@ -702,8 +921,8 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   std::vector<double> hist;
   const int first_check = mixed ? 1 : min_iter;
   while (it < max_iter) {
-    if (mixed) spmv_all<true>(c, F, rb, c.y.p);
-    else spmv_all<false>(c, F, rb, c.y.p);
+    if (mixed) push_pull(c, pb, L, F, c.y.p);
+    else spmv_all(c, F, rb, c.y.p);
     if (c.prof.on) c.prof.begin(s);
     k_sum_partials<<<kRedGrid, kRedBlock, 0, s>>>(c.y.p, n, part);
     RMX_LAUNCH_CHECK();
@@ -734,7 +953,7 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   RMX_HIP(hipEventDestroy(e1));
   c.iterations = it;
   c.resid = r;
-  if (!batch) {   // p back on the original labels
+  if (relabel) {   // p back on the original labels
     k_unpermute<<<grid_for(n, 256), 256, 0, s>>>(c.x.p, perm.p, n, c.y.p);
     RMX_LAUNCH_CHECK();
     c.x.swap(c.y);

@@ -1,0 +1,86 @@
+"""Mixed-precision steady state (blocked push / pull) on shapes that reach
+every branch of its kernels, against the oracle's FP64 flows.
+
+- long runs of rows without out-arcs inside one push tile (row bounds read
+  from global memory instead of LDS);
+- fewer than 8 arcs in a tile (the scalar tail);
+- arc counts that are whole multiples of the tile;
+- many bins, and a hub that every node points to (one LDS accumulator
+  takes a whole bin's arcs).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _csr(n, src, dst, w):
+    src, dst, w = np.asarray(src, np.int64), np.asarray(dst, np.int64), np.asarray(w, np.float64)
+    keep = src != dst
+    src, dst, w = src[keep], dst[keep], w[keep]
+    key = src * n + dst
+    key, first = np.unique(key, return_index=True)
+    src, dst, w = key // n, key % n, w[first]
+    ptr = np.zeros(n + 1, np.int64)
+    np.add.at(ptr, src + 1, 1)
+    return np.cumsum(ptr), dst, w, src
+
+
+def _random_arcs(rng, nodes, n, deg_max):
+    src, dst = [], []
+    for i in nodes:
+        d = int(rng.integers(1, deg_max + 1))
+        src.append(np.full(d, i))
+        dst.append(rng.integers(0, n, d))
+    return np.concatenate(src), np.concatenate(dst)
+
+
+def _graphs():
+    rng = np.random.default_rng(17)
+    out = {}
+    n = 16000   # rows 100..13999 have no out-arcs: one tile spans ~14000 rows
+    s, d = _random_arcs(rng, [i for i in range(n) if not 100 <= i < 14000], n, 20)
+    out["dangling_run"] = (n, s, d, rng.lognormal(0, 1, len(s)))
+    out["cycle3"] = (3, [0, 1, 2], [1, 2, 0], [1.0, 2.0, 3.0])
+    n = 512     # 8 distinct targets each: 4096 arcs, two full tiles
+    s = np.repeat(np.arange(n), 8)
+    d = (s + 1 + np.tile(np.arange(8) * 37, n)) % n
+    out["two_tiles"] = (n, s, d, rng.random(len(s)) + 0.1)
+    n = 70000   # many bins; node 0 is a hub in both directions
+    s, d = _random_arcs(rng, range(1, n), n, 6)
+    s = np.concatenate([s, np.arange(1, n), np.zeros(n - 1, np.int64)])
+    d = np.concatenate([d, np.zeros(n - 1, np.int64), np.arange(1, n)])
+    out["hub_bins"] = (n, s, d, rng.lognormal(0, 0.5, len(s)))
+    return out
+
+
+@pytest.mark.parametrize("name", ["dangling_run", "cycle3", "two_tiles", "hub_bins"])
+def test_mixed_flows_match_oracle(name):
+    from oracle.pyoracle import OracleNet
+    from relaxmap_amd import Network
+    n, s, d, w = _graphs()[name]
+    ptr, idx, ww, src = _csr(n, s, d, w)
+    print(f"[gpu] push/pull {name}: n={n} m={len(idx)}", flush=True)
+    ref = OracleNet.from_arcs(n, src, idx, ww).initiate().flows()
+    for prec, tol in (("mixed", 1e-6), ("fp64", 1e-12)):
+        with Network(precision=prec) as net:
+            net.upload(n, ptr, idx, ww)
+            net.initiate()
+            p = net.flows()
+        err = np.abs(p - ref).sum()
+        print(f"[gpu]   {prec}: L1 {err:.3e}", flush=True)
+        assert abs(p.sum() - 1.0) <= 1e-12
+        assert err <= tol, (prec, err)
+
+
+def test_mixed_flows_deterministic():
+    from relaxmap_amd import Network
+    n, s, d, w = _graphs()["hub_bins"]
+    ptr, idx, ww, _ = _csr(n, s, d, w)
+    runs = []
+    for _ in range(2):
+        with Network(precision="mixed") as net:
+            net.upload(n, ptr, idx, ww)
+            net.initiate()
+            runs.append(net.flows())
+    assert np.array_equal(runs[0], runs[1])
