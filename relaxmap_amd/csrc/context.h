@@ -201,7 +201,7 @@ struct Ctx {
   int level = 0;
   DevBuf<double> node_w, raw_w;   // inputs kept for re-initiation
   DevBuf<uint8_t> dangling;       // level-0 dangling flag
-  DevBuf<uint32_t> in_of_out;     // level-0: in-CSR position of each out arc
+  DevBuf<double> raw_in;          // level-0: the raw arc weights in in-CSR order
   DevBuf<uint32_t> node_sp;       // node -> super node (level 1)
 
   // module SoA, N slots (module ids are node ids, as in the reference)
@@ -295,7 +295,7 @@ struct Ctx {
 // ----- graph.hip
 void upload_graph(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx,
                   const double *w, const double *node_w);
-void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out);
+void build_in_csr(Ctx &c, Level &L, const double *raw, DevBuf<double> *raw_in);
 void super_in_weights(Ctx &c, const Level &L, double *in_w);   // FP64 in-weights of a contracted level
 void build_buckets(Ctx &c, Level &L);
 

@@ -36,6 +36,9 @@ constexpr int kRedBlock = 256;
 constexpr double kMixedTol = 1e-8;   // L1 residual that ends the mixed-precision iteration
 enum { kS = 0, kR = 1, kD = 2, kL = 3, kSE = 4, kSS = 5, kSP = 6 };
 
+// grid of a grid-stride loop over n elements
+inline unsigned stride_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(16384, (n + 255) / 256)); }
+
 constexpr int kNumRowBins = 6;
 constexpr int kRowBinMax[kNumRowBins] = {4, 8, 16, 32, 256, 0x7fffffff};
 
@@ -53,12 +56,12 @@ __device__ inline long long group_sum_ll(long long v) {
 }
 
 // --- row-normalised out weights (Module.cpp:237-247) --------------------
-// FP64: also into the in-CSR (the pull); mixed: an FP32 copy in out-CSR
-// order (the push).
+// Also the row sums, from which the in-arcs get the same quotients; mixed
+// precision keeps an FP32 copy in out-CSR order (the push).
 template <int G, bool MIXED>
 __global__ void k_normalize(const uint32_t *__restrict__ optr, int32_t n, const double *__restrict__ raw,
-                            const uint32_t *__restrict__ in_of_out, double *__restrict__ out_w,
-                            double *__restrict__ in_w, float *__restrict__ out_wf, uint8_t *__restrict__ dangling) {
+                            double *__restrict__ out_w, float *__restrict__ out_wf, double *__restrict__ rowsum,
+                            uint8_t *__restrict__ dangling) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t r = t / G;
   const int lane = (int)(t % G);
@@ -67,12 +70,40 @@ __global__ void k_normalize(const uint32_t *__restrict__ optr, int32_t n, const 
   double s = 0.0;
   for (uint32_t k = b + lane; k < e; k += G) s += raw[k];
   s = group_sum<G>(s);
-  if (lane == 0) dangling[r] = (b == e) ? 1 : 0;
+  if (lane == 0) {
+    dangling[r] = (b == e) ? 1 : 0;
+    rowsum[r] = s;
+  }
   for (uint32_t k = b + lane; k < e; k += G) {
     double v = raw[k] / s;
     out_w[k] = v;
     if (MIXED) out_wf[k] = (float)v;
-    else in_w[in_of_out[k]] = v;
+  }
+}
+
+// The in-arcs, in in-CSR order, each from its tail: the normalised weight
+// raw / s (FP64 pull), or the flow p * (raw / s) and its FP32 copy.  The
+// operations are those of the out-arc, so both copies are identical.  The
+// tail's p and row sum are read as one 16-byte pair.
+__global__ void k_in_norm(const uint32_t *__restrict__ iidx, const double *__restrict__ raw_in, int64_t m,
+                          const double *__restrict__ rowsum, double *__restrict__ in_w) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x)
+    in_w[j] = raw_in[j] / rowsum[iidx[j]];
+}
+
+__global__ void k_pack_ps(const double *__restrict__ p, const double *__restrict__ rowsum, int32_t n,
+                          double2 *__restrict__ ps) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ps[i] = make_double2(p[i], rowsum[i]);
+}
+
+__global__ void k_in_flows(const uint32_t *__restrict__ iidx, const double *__restrict__ raw_in, int64_t m,
+                           const double2 *__restrict__ ps, double *__restrict__ in_w, float *__restrict__ in_wf) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
+    const double2 t = ps[iidx[j]];
+    const double f = t.x * (raw_in[j] / t.y);
+    in_w[j] = f;
+    in_wf[j] = (float)f;
   }
 }
 
@@ -479,8 +510,7 @@ __global__ __launch_bounds__(kRedBlock) void k_bconst(const uint32_t *__restrict
 // --- flows on arcs and node statistics (Module.cpp:267-325) ---------------
 template <int G>
 __global__ void k_flows(const uint32_t *__restrict__ optr, int32_t n, double *__restrict__ out_w,
-                        const uint32_t *__restrict__ in_of_out, double *__restrict__ in_w,
-                        float *__restrict__ out_wf, float *__restrict__ in_wf, const double *__restrict__ x,
+                        float *__restrict__ out_wf, const double *__restrict__ x,
                         const double *__restrict__ tau, const uint8_t *__restrict__ dang, int mixed,
                         double *__restrict__ p_out, Stat4 *__restrict__ us, double *__restrict__ qlit) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -496,9 +526,6 @@ __global__ void k_flows(const uint32_t *__restrict__ optr, int32_t n, double *__
     float ff = (float)f;
     out_w[k] = f;
     out_wf[k] = ff;
-    uint32_t ik = in_of_out[k];
-    in_w[ik] = f;
-    in_wf[ik] = ff;
     sf += f;
     sx += fx_of(mixed ? (double)ff : f);
   }
@@ -817,13 +844,21 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
     RMX_LAUNCH_CHECK();
   }
 
-  if (mixed)
-    k_normalize<16, true><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(
-        L.out_ptr.p, n, c.raw_w.p, c.in_of_out.p, L.out_w.p, L.in_w.p, L.out_wf.p, c.dangling.p);
-  else
-    k_normalize<16, false><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(
-        L.out_ptr.p, n, c.raw_w.p, c.in_of_out.p, L.out_w.p, L.in_w.p, L.out_wf.p, c.dangling.p);
-  RMX_LAUNCH_CHECK();
+  DevBuf<double> rowsum(c.cache);
+  rowsum.alloc(n);
+  if (mixed) {
+    k_normalize<16, true><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, c.raw_w.p, L.out_w.p,
+                                                                        L.out_wf.p, rowsum.p, c.dangling.p);
+    RMX_LAUNCH_CHECK();
+  } else {
+    k_normalize<16, false><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, c.raw_w.p, L.out_w.p,
+                                                                         L.out_wf.p, rowsum.p, c.dangling.p);
+    RMX_LAUNCH_CHECK();
+    if (m > 0) {
+      k_in_norm<<<stride_grid(m), 256, 0, s>>>(L.in_idx.p, c.raw_in.p, m, rowsum.p, L.in_w.p);
+      RMX_LAUNCH_CHECK();
+    }
+  }
   // FP64: the relabelled pull copy (descending out-degree), built on the
   // device; mixed: the push / pull bins on the original labels
   DevBuf<uint32_t> inv(c.cache), perm(c.cache), pptr(c.cache), pidx(c.cache), pdeg(c.cache);
@@ -961,10 +996,18 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   c.sync();   // the relabelled copy goes back to the cache
 
   // flows, node statistics and the literal node exits
-  k_flows<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, L.out_w.p, c.in_of_out.p, L.in_w.p,
-                                                              L.out_wf.p, L.in_wf.p, c.x.p, L.tau.p, c.dangling.p,
-                                                              mixed ? 1 : 0, L.p.p, L.us.p, c.node_q.p);
+  k_flows<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, L.out_w.p, L.out_wf.p, c.x.p, L.tau.p,
+                                                              c.dangling.p, mixed ? 1 : 0, L.p.p, L.us.p, c.node_q.p);
   RMX_LAUNCH_CHECK();
+  if (m > 0) {
+    DevBuf<double2> ps(c.cache);
+    ps.alloc(n);
+    k_pack_ps<<<grid_for(n, 256), 256, 0, s>>>(c.x.p, rowsum.p, n, ps.p);
+    RMX_LAUNCH_CHECK();
+    k_in_flows<<<stride_grid(m), 256, 0, s>>>(L.in_idx.p, c.raw_in.p, m, ps.p, L.in_w.p, L.in_wf.p);
+    RMX_LAUNCH_CHECK();
+    c.sync();   // ps goes back to the cache
+  }
   k_calib<<<kRedGrid, kRedBlock, 0, s>>>(c.node_q.p, L.p.p, n, part);
   RMX_LAUNCH_CHECK();
   k_calib_final<<<1, kRedBlock, 0, s>>>(part, c.scal.p);

@@ -83,15 +83,16 @@ __global__ void k_ptr_from_sorted(const uint32_t *__restrict__ key, int64_t m, i
   }
 }
 
-// after the stable sort: in_idx[k] = src[e_k], in_of_out[e_k] = k
+// after the stable sort: in_idx[k] = src[e_k], raw_in[k] = raw[e_k]
 __global__ void k_in_from_sorted(const uint32_t *__restrict__ sorted_e, const uint32_t *__restrict__ src,
-                                 int64_t m, uint32_t *__restrict__ in_idx, uint32_t *__restrict__ in_of_out) {
+                                 const double *__restrict__ raw, int64_t m, uint32_t *__restrict__ in_idx,
+                                 double *__restrict__ raw_in) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; k < m; k += stride) {
     uint32_t e = sorted_e[k];
     in_idx[k] = src[e];
-    if (in_of_out) in_of_out[e] = (uint32_t)k;
+    if (raw_in) raw_in[k] = raw[e];
   }
 }
 
@@ -175,21 +176,21 @@ void upload_graph(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const in
     for (int32_t i = 0; i < n; ++i) nw[i] = node_w[i] > 0.0 ? node_w[i] : 1.0;
   RMX_HIP(hipMemcpyAsync(c.node_w.p, nw.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
   c.sync();   // host staging vectors go out of scope
-  build_in_csr(c, L, &c.in_of_out);
+  build_in_csr(c, L, c.raw_w.p, &c.raw_in);
   build_buckets(c, L);
   c.initiated = false;
 }
 
 namespace {
 
-// in_of_out: the in-CSR position of every out-arc
-void transpose(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) {
+// raw_in (with raw): the raw weights of the in-arcs, in in-CSR order
+void transpose(Ctx &c, Level &L, const double *raw, DevBuf<double> *raw_in) {
   hipStream_t s = c.stream;
   const int64_t m = L.m;
   const int32_t n = L.n;
   L.in_ptr.alloc(n + 1);
   L.in_idx.alloc(m);
-  if (in_of_out) in_of_out->alloc(m);
+  if (raw_in) raw_in->alloc(m);
   if (m == 0) {
     fill_u32(L.in_ptr.p, 0u, n + 1, s);
     c.sync();
@@ -207,7 +208,7 @@ void transpose(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) {
     int bits = 1;
     while (bits < 32 && ((int64_t)1 << bits) < (int64_t)n) ++bits;
     radix_sort_pairs_u32(keys.p, vals.p, m, 0, bits, c.scratch, s);
-    k_in_from_sorted<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, m, L.in_idx.p, in_of_out ? in_of_out->p : nullptr);
+    k_in_from_sorted<<<gs_grid(m), 256, 0, s>>>(vals.p, src.p, raw, m, L.in_idx.p, raw_in ? raw_in->p : nullptr);
     RMX_LAUNCH_CHECK();
     k_ptr_from_sorted<<<gs_grid(m + 1), 256, 0, s>>>(keys.p, m, n, L.in_ptr.p);
     RMX_LAUNCH_CHECK();
@@ -217,7 +218,7 @@ void transpose(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) {
 
 }  // namespace
 
-void build_in_csr(Ctx &c, Level &L, DevBuf<uint32_t> *in_of_out) { transpose(c, L, in_of_out); }
+void build_in_csr(Ctx &c, Level &L, const double *raw, DevBuf<double> *raw_in) { transpose(c, L, raw, raw_in); }
 
 void super_in_weights(Ctx &c, const Level &L, double *in_w) {
   if (L.m == 0) return;

@@ -1051,33 +1051,47 @@ __global__ __launch_bounds__(256) void k_hubp_count(EvalArgs a, HubParts H) {
     __syncthreads();
     long long so = 0, si = 0;
     uint32_t run_q = 0xffffffffu, run_n = 0;   // runs of one partition: one counter update each
-    for (uint32_t j = h.j0 + threadIdx.x;; j += 256) {
-      const bool more = j < h.j1;
-      uint32_t q = 0xffffffffu;
-      if (more) {
+    auto flush_run = [&]() {
+      if (run_n) {
+        if (local) atomicAdd(&hist[run_q], run_n);
+        else atomicAdd(&H.bcnt[H.PB[h.i] + run_q], run_n);
+      }
+    };
+    // four arcs per lane in flight: their ids, then their neighbours' modules
+    for (uint32_t jb = h.j0 + threadIdx.x; jb < h.j1; jb += 4 * 256) {
+      uint32_t kq[4], nq[4], mq[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t j = jb + (uint32_t)q * 256;
         const bool out = j < h.od;
-        const uint32_t k = out ? h.ob + j : h.ib + (j - h.od);
-        const uint32_t m = a.mod[out ? a.oidx[k] : a.iidx[k]];
+        kq[q] = out ? h.ob + j : h.ib + (j - h.od);
+        nq[q] = j < h.j1 ? (out ? a.oidx[kq[q]] : a.iidx[kq[q]]) : 0u;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) mq[q] = jb + (uint32_t)q * 256 < h.j1 ? a.mod[nq[q]] : 0u;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t j = jb + (uint32_t)q * 256;
+        if (j >= h.j1) break;
+        const bool out = j < h.od;
+        const uint32_t m = mq[q];
         if (m == h.o) {   // own module: only its sums are needed, the arc is not partitioned
           H.mtmp[H.AB[h.i] + j] = kOwnArc;
-          if (out) so += a.aout.fxw(k);
-          else si += a.ain.fxw(k);
+          if (out) so += a.aout.fxw(kq[q]);
+          else si += a.ain.fxw(kq[q]);
           continue;
         }
         H.mtmp[H.AB[h.i] + j] = out ? m : (m | kInBit);
-        q = part_of(m, h.np);
-      }
-      if (q != run_q) {
-        if (run_n) {
-          if (local) atomicAdd(&hist[run_q], run_n);
-          else atomicAdd(&H.bcnt[H.PB[h.i] + run_q], run_n);
+        const uint32_t pq = part_of(m, h.np);
+        if (pq != run_q) {
+          flush_run();
+          run_q = pq;
+          run_n = 0;
         }
-        run_q = q;
-        run_n = 0;
+        ++run_n;
       }
-      if (!more) break;
-      ++run_n;
     }
+    flush_run();
     so = wave_sum(so);
     si = wave_sum(si);
     if ((threadIdx.x & (kWave - 1)) == 0) {
@@ -1177,26 +1191,40 @@ __global__ __launch_bounds__(256) void k_hubp_eval(EvalArgs a, HubParts H) {
     // update per run instead of one per entry on a contended slot.
     uint32_t run_m = kEmptyKey;
     long long run_o = 0, run_i = 0;
-    for (uint32_t e = e0 + threadIdx.x;; e += 256) {
-      const bool more = e < e1;
-      const uint32_t key = more ? H.ekey[e] : kEmptyKey;
-      const uint32_t m = key & ~kInBit;
-      if (!more || m != run_m) {
-        if (run_m != kEmptyKey) {
-          const int sl = lds_insert_bounded(s_keys, kHubTable, run_m);
-          if (sl >= 0) {
-            if (run_o) atomicAdd((unsigned long long *)&s_out[sl], (unsigned long long)run_o);
-            if (run_i) atomicAdd((unsigned long long *)&s_in[sl], (unsigned long long)run_i);
-          }
+    auto flush_run = [&]() {
+      if (run_m != kEmptyKey) {
+        const int sl = lds_insert_bounded(s_keys, kHubTable, run_m);
+        if (sl >= 0) {
+          if (run_o) atomicAdd((unsigned long long *)&s_out[sl], (unsigned long long)run_o);
+          if (run_i) atomicAdd((unsigned long long *)&s_in[sl], (unsigned long long)run_i);
         }
-        if (!more) break;
-        run_m = m;
-        run_o = 0;
-        run_i = 0;
       }
-      if (key & kInBit) run_i += H.ew[e];
-      else run_o += H.ew[e];
+    };
+    // four entries per lane in flight (an entry key never equals kEmptyKey)
+    for (uint32_t eb = e0 + threadIdx.x; eb < e1; eb += 4 * 256) {
+      uint32_t kq[4];
+      long long wq[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t e = eb + (uint32_t)q * 256;
+        kq[q] = e < e1 ? H.ekey[e] : kEmptyKey;
+        wq[q] = e < e1 ? H.ew[e] : 0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (kq[q] == kEmptyKey) break;
+        const uint32_t m = kq[q] & ~kInBit;
+        if (m != run_m) {
+          flush_run();
+          run_m = m;
+          run_o = 0;
+          run_i = 0;
+        }
+        if (kq[q] & kInBit) run_i += wq[q];
+        else run_o += wq[q];
+      }
     }
+    flush_run();
     __syncthreads();   // a failed insert leaves every slot occupied: detected below from the count
     int nw = 0;
     for (int s0 = 0; s0 < kHubTable; s0 += 256) {
@@ -1220,13 +1248,7 @@ __global__ __launch_bounds__(256) void k_hubp_eval(EvalArgs a, HubParts H) {
     UnitCtx U;
     unit_ctx(a, ul, H.own[2 * i], H.own[2 * i + 1], U);
     Best b{0.0, kNone, -1};
-    for (int j = wl; j < nw; j += kWave) {
-      const int sl = wave_slot(j, w);
-      const uint32_t m = s_keys[sl];
-      if (m == U.o || !allowed(a, U, m)) continue;
-      const double dl = cand_delta<F32>(U, a.ms[m], s_out[sl], s_in[sl]);
-      if (better(dl, m, b) && dl < 0.0) { b.d = dl; b.m = m; b.i = sl; }
-    }
+    eval_cands<4, kWave, F32>(a, U, s_keys, s_out, s_in, wl, nw, [w](int j) { return wave_slot(j, w); }, b);
     b = group_best<kWave>(b);
     __syncthreads();   // keys are not read again: the per-wave bests reuse them
     double *r_bd = (double *)s_keys;

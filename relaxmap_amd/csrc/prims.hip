@@ -159,7 +159,10 @@ struct SortShape {
 
 constexpr int kMaxPasses = 8;
 
-// Digit counts of every pass at once: one read of the keys.
+// Digit counts of every pass at once: one read of the keys.  Keys often
+// arrive grouped by their high bits (arcs in source order), so a wavefront's
+// digit of a high pass is usually one value: that case is found with one
+// ballot and added by a single lane, instead of 64 atomics on one slot.
 template <typename K>
 __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K *__restrict__ keys, int64_t n, int begin_bit,
                                                           int passes, uint32_t *__restrict__ counts) {
@@ -167,9 +170,22 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K *__restrict__ 
 #pragma unroll
   for (int q = 0; q < kMaxPasses; ++q) h[q][threadIdx.x] = 0;
   __syncthreads();
-  for (int64_t j = (int64_t)blockIdx.x * kSortBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kSortBlock) {
-    const K k = keys[j];
-    for (int q = 0; q < passes; ++q) atomicAdd(&h[q][(uint32_t)(k >> (begin_bit + 8 * q)) & 0xffu], 1u);
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t j0 = (int64_t)blockIdx.x * kSortBlock; j0 < n; j0 += (int64_t)gridDim.x * kSortBlock) {
+    const int64_t j = j0 + threadIdx.x;
+    const bool valid = j < n;   // the valid lanes of a wavefront are a prefix
+    const K k = valid ? keys[j] : K(0);
+    const uint64_t live = __ballot(valid);
+    if (!live) continue;
+    for (int q = 0; q < passes; ++q) {
+      const uint32_t d = (uint32_t)(k >> (begin_bit + 8 * q)) & 0xffu;
+      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+      if (__ballot(valid && d == d0) == live) {
+        if (lane == 0) atomicAdd(&h[q][d0], (uint32_t)__popcll(live));
+      } else if (valid) {
+        atomicAdd(&h[q][d], 1u);
+      }
+    }
   }
   __syncthreads();
   for (int q = 0; q < passes; ++q)
