@@ -12,6 +12,7 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "context.h"
@@ -155,23 +156,46 @@ int rmx_set_hub_path(rmx_ctx *ctx, int global_tables) {
 
 namespace {
 
+// f(first_row, end_row) on row ranges of about equal arc counts, one host
+// thread each (out_ptr monotone); returns the first range's non-empty message
+template <typename F>
+const char *rows_in_parallel(int32_t n, int64_t m, const int64_t *out_ptr, F f) {
+  const int T = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (T == 1 || m < (1 << 20)) return f(0, n);
+  std::vector<int32_t> cut(T + 1);
+  for (int t = 0; t <= T; ++t)
+    cut[t] = t == T ? n : (int32_t)(std::lower_bound(out_ptr, out_ptr + n + 1, m * t / T) - out_ptr);
+  std::vector<const char *> msg(T, "");
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t) th.emplace_back([&, t] { msg[t] = f(cut[t], std::max(cut[t], cut[t + 1])); });
+  for (auto &x : th) x.join();
+  for (int t = 0; t < T; ++t)
+    if (*msg[t]) return msg[t];
+  return "";
+}
+
 // the out-CSR rules of rmx_graph_upload; "" when they hold
 const char *check_csr(int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx, const double *w) {
   if (n <= 0 || m < 0 || !out_ptr || (m > 0 && (!out_idx || !w))) return "empty network or null arrays";
   if (out_ptr[0] != 0 || out_ptr[n] != m) return "out_ptr does not span the arcs";
-  for (int32_t i = 0; i < n; ++i) {
+  for (int32_t i = 0; i < n; ++i)
     if (out_ptr[i + 1] < out_ptr[i]) return "out_ptr not monotone";
-    for (int64_t k = out_ptr[i]; k < out_ptr[i + 1]; ++k) {
-      int32_t d = out_idx[k];
-      if (d < 0 || d >= n) return "arc head out of range";
-      if (d == i) return "self-loop in arc list";
-      if (k > out_ptr[i] && out_idx[k - 1] >= d) return "arcs not sorted or duplicated";
-      if (!(w[k] > 0.0) || !std::isfinite(w[k])) return "arc weight not positive";
+  return rows_in_parallel(n, m, out_ptr, [&](int32_t i0, int32_t i1) -> const char * {
+    for (int32_t i = i0; i < i1; ++i) {
+      for (int64_t k = out_ptr[i]; k < out_ptr[i + 1]; ++k) {
+        int32_t d = out_idx[k];
+        if (d < 0 || d >= n) return "arc head out of range";
+        if (d == i) return "self-loop in arc list";
+        if (k > out_ptr[i] && out_idx[k - 1] >= d) return "arcs not sorted or duplicated";
+        if (!(w[k] > 0.0) || !std::isfinite(w[k])) return "arc weight not positive";
+      }
     }
-  }
-  return "";
+    return "";
+  });
 }
 
+// default names "1".."n", made when first needed (an uploaded network has
+// none of its own)
 void set_names(rmx_ctx *ctx, int32_t n) {
   if ((int32_t)ctx->names.size() != n) {
     ctx->names.resize(n);
@@ -191,7 +215,7 @@ int rmx_graph_upload(rmx_ctx *ctx, int32_t n, int64_t m, const int64_t *out_ptr,
     RMX_HIP(hipSetDevice(ctx->c.device));
     ctx->c.nb = 0;
     upload_graph(ctx->c, n, m, out_ptr, out_idx, w, node_w);
-    set_names(ctx, n);
+    ctx->names.clear();
     return RMX_OK;
   });
 }
@@ -204,16 +228,22 @@ int rmx_batch_upload(rmx_ctx *ctx, int32_t n_inst, const int32_t *node_off, cons
     const char *bad = check_csr(n, m, out_ptr, out_idx, w);
     if (*bad) return fail(ctx, RMX_EINVAL, bad);
     if (node_off[0] != 0 || node_off[n_inst] != n) return fail(ctx, RMX_EINVAL, "node_off does not span the nodes");
-    std::vector<uint32_t> inst((size_t)n);
-    for (int32_t I = 0; I < n_inst; ++I) {
+    for (int32_t I = 0; I < n_inst; ++I)
       if (node_off[I + 1] < node_off[I]) return fail(ctx, RMX_EINVAL, "node_off not monotone");
-      for (int32_t i = node_off[I]; i < node_off[I + 1]; ++i) {
+    std::vector<uint32_t> inst((size_t)n);
+    bad = rows_in_parallel(n, m, out_ptr, [&](int32_t i0, int32_t i1) -> const char * {
+      if (i0 >= i1) return "";
+      // the instance of row i0: the last I with node_off[I] <= i0
+      int32_t I = (int32_t)(std::upper_bound(node_off, node_off + n_inst + 1, i0) - node_off) - 1;
+      for (int32_t i = i0; i < i1; ++i) {
+        while (i >= node_off[I + 1]) ++I;
         inst[i] = (uint32_t)I;
         for (int64_t k = out_ptr[i]; k < out_ptr[i + 1]; ++k)
-          if (out_idx[k] < node_off[I] || out_idx[k] >= node_off[I + 1])
-            return fail(ctx, RMX_EINVAL, "an arc joins two instances");
+          if (out_idx[k] < node_off[I] || out_idx[k] >= node_off[I + 1]) return "an arc joins two instances";
       }
-    }
+      return "";
+    });
+    if (*bad) return fail(ctx, RMX_EINVAL, bad);
     Ctx &c = ctx->c;
     RMX_HIP(hipSetDevice(c.device));
     c.nb = 0;
@@ -235,7 +265,7 @@ int rmx_batch_upload(rmx_ctx *ctx, int32_t n_inst, const int32_t *node_off, cons
     RMX_HIP(hipMemcpyAsync(c.L0.uinst.p, inst.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
     RMX_HIP(hipMemcpyAsync(c.b_seed.p, sd.data(), sizeof(uint64_t) * n_inst, hipMemcpyHostToDevice, c.stream));
     c.sync();   // host staging vectors go out of scope
-    set_names(ctx, n);
+    ctx->names.clear();
     return RMX_OK;
   });
 }
@@ -846,6 +876,7 @@ extern "C" int rmx_write_tree(rmx_ctx *ctx, const char *path) {
     RMX_HIP(hipMemcpyAsync(p.data(), c.L0.p.p, sizeof(double) * c.N, hipMemcpyDeviceToHost, c.stream));
     c.sync();
     std::string err;
+    set_names(ctx, c.N);
     if (write_tree(path, c.codelength, mod, p, ctx->names, err) != 0) return fail(ctx, RMX_EIO, err);
     return RMX_OK;
   });
