@@ -67,6 +67,7 @@ void free_ctx_device(Ctx &c) {
   c.b_Lold.free(); c.b_Lstep.free(); c.b_Louter.free(); c.b_nmod.free(); c.b_D.free(); c.b_R.free();
   c.b_iters.free(); c.b_step.free(); c.b_outer.free(); c.b_inner.free();
   c.b_ebeg.free(); c.b_ecnt.free(); c.b_ecur.free(); c.b_count.free();
+  c.b_cbeg.free(); c.b_cend.free(); c.b_ichunk.free(); c.b_cpart.free(); c.n_chunks = 0;
   c.act.free(); c.act_next.free(); c.target.free(); c.newmod.free(); c.stamp.free(); c.movers.free(); c.movers_st.free();
   c.empty_list.free(); c.want_empty.free(); c.rank_tmp.free(); c.agg_buf.free();
   c.batch_keys.free(); c.batch_list.free();
@@ -252,6 +253,25 @@ int rmx_batch_upload(rmx_ctx *ctx, int32_t n_inst, const int32_t *node_off, cons
     c.h_boff.assign(node_off, node_off + n_inst + 1);
     std::vector<uint64_t> sd(n_inst);
     for (int32_t I = 0; I < n_inst; ++I) sd[I] = mix64(seeds[I] ^ 0x6a09e667f3bcc909ull);
+    std::vector<uint32_t> cbeg, cend, ichunk(1, 0u);
+    for (int32_t I = 0; I < n_inst; ++I) {
+      for (int64_t a = node_off[I]; a < node_off[I + 1]; a += kTotChunk) {
+        cbeg.push_back((uint32_t)a);
+        cend.push_back((uint32_t)std::min<int64_t>(a + kTotChunk, node_off[I + 1]));
+      }
+      ichunk.push_back((uint32_t)cbeg.size());
+    }
+    c.n_chunks = (int32_t)cbeg.size();
+    c.b_cbeg.alloc(std::max<int64_t>(1, c.n_chunks));
+    c.b_cend.alloc(std::max<int64_t>(1, c.n_chunks));
+    c.b_ichunk.alloc((int64_t)n_inst + 1);
+    c.b_cpart.alloc(4 * std::max<int64_t>(1, c.n_chunks));
+    if (c.n_chunks > 0) {
+      RMX_HIP(hipMemcpyAsync(c.b_cbeg.p, cbeg.data(), sizeof(uint32_t) * c.n_chunks, hipMemcpyHostToDevice, c.stream));
+      RMX_HIP(hipMemcpyAsync(c.b_cend.p, cend.data(), sizeof(uint32_t) * c.n_chunks, hipMemcpyHostToDevice, c.stream));
+    }
+    RMX_HIP(hipMemcpyAsync(c.b_ichunk.p, ichunk.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice,
+                           c.stream));
     c.b_off.alloc((int64_t)n_inst + 1);
     c.b_node.alloc(n);
     c.b_seed.alloc(n_inst);

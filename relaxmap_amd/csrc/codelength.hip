@@ -76,13 +76,17 @@ __global__ __launch_bounds__(kRedBlock) void k_totals_final(const double *__rest
 // leave their instance).  S = sum of exits, L = phi(S) - 2 sum phi(q) +
 // sum phi(q + P) - sum phi(p) (Module.cpp:474-501 per instance), and the
 // number of non-empty modules.
-__global__ __launch_bounds__(kRedBlock) void k_batch_totals(const uint32_t *__restrict__ off,
+// Per-instance totals in two fixed-order steps: one workgroup per chunk of
+// at most kTotChunk module ids (a chunk never spans two instances, so a
+// large instance is spread over many workgroups), then one thread per
+// instance adding its chunks in order.
+__global__ __launch_bounds__(kRedBlock) void k_chunk_totals(const uint32_t *__restrict__ cbeg,
+                                                            const uint32_t *__restrict__ cend,
                                                             const Stat4 *__restrict__ ms,
                                                             const int32_t *__restrict__ mNodes,
-                                                            const double *__restrict__ C, double *__restrict__ S_out,
-                                                            double *__restrict__ L_out, int32_t *__restrict__ nmod) {
+                                                            double *__restrict__ part) {
   __shared__ double lds[kRedBlock / kWave];
-  const uint32_t b = off[blockIdx.x], e = off[blockIdx.x + 1];
+  const uint32_t b = cbeg[blockIdx.x], e = cend[blockIdx.x];
   double sq = 0, pq = 0, pqp = 0, cnt = 0;
   for (uint32_t m = b + threadIdx.x; m < e; m += kRedBlock) {
     if (mNodes[m] > 0) {
@@ -99,10 +103,30 @@ __global__ __launch_bounds__(kRedBlock) void k_batch_totals(const uint32_t *__re
   pqp = block_sum<double, kRedBlock>(pqp, lds);
   cnt = block_sum<double, kRedBlock>(cnt, lds);
   if (threadIdx.x == 0) {
-    S_out[blockIdx.x] = sq;
-    L_out[blockIdx.x] = plogp(sq) - 2.0 * pq + pqp - C[blockIdx.x];
-    nmod[blockIdx.x] = (int32_t)cnt;
+    double *o = part + 4 * (int64_t)blockIdx.x;
+    o[0] = sq;
+    o[1] = pq;
+    o[2] = pqp;
+    o[3] = cnt;
   }
+}
+
+__global__ void k_batch_totals(int32_t nb, const uint32_t *__restrict__ ichunk, const double *__restrict__ part,
+                               const double *__restrict__ C, double *__restrict__ S_out, double *__restrict__ L_out,
+                               int32_t *__restrict__ nmod) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nb) return;
+  double sq = 0, pq = 0, pqp = 0, cnt = 0;
+  for (uint32_t k = ichunk[I]; k < ichunk[I + 1]; ++k) {
+    const double *o = part + 4 * (int64_t)k;
+    sq += o[0];
+    pq += o[1];
+    pqp += o[2];
+    cnt += o[3];
+  }
+  S_out[I] = sq;
+  L_out[I] = plogp(sq) - 2.0 * pq + pqp - C[I];
+  nmod[I] = (int32_t)cnt;
 }
 
 // sums over the instances (the host's view of a batch: total codelength,
@@ -284,8 +308,13 @@ void module_totals_async(Ctx &c) {
     c.b_S.alloc(c.nb);
     c.b_L.alloc(c.nb);
     c.b_nmod.alloc(c.nb);
-    k_batch_totals<<<c.nb, kRedBlock, 0, c.stream>>>(c.b_off.p, c.ms.p, c.mNodes.p, c.b_C.p, c.b_S.p, c.b_L.p,
-                                                      c.b_nmod.p);
+    if (c.n_chunks > 0) {
+      k_chunk_totals<<<c.n_chunks, kRedBlock, 0, c.stream>>>(c.b_cbeg.p, c.b_cend.p, c.ms.p, c.mNodes.p,
+                                                             c.b_cpart.p);
+      RMX_LAUNCH_CHECK();
+    }
+    k_batch_totals<<<grid_for(c.nb, 256), 256, 0, c.stream>>>(c.nb, c.b_ichunk.p, c.b_cpart.p, c.b_C.p, c.b_S.p,
+                                                               c.b_L.p, c.b_nmod.p);
     RMX_LAUNCH_CHECK();
     k_batch_sum<<<1, kRedBlock, 0, c.stream>>>(c.nb, c.b_S.p, c.b_L.p, c.b_nmod.p, c.scal.p);
     RMX_LAUNCH_CHECK();

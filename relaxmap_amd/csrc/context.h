@@ -29,6 +29,7 @@ constexpr int kMaxBatches = 256 / kNumBuckets - 1;
 // Hubs are evaluated in partitions of about kHubPart arcs (split by a hash of
 // the neighbour's module), each aggregated in an LDS table.
 constexpr int kHubPart = 1024;
+constexpr int kTotChunk = 4096;   // module ids per workgroup of the per-instance totals
 
 // Device blocks kept for reuse within one context.  All work of a context
 // runs on one stream, so a block handed back while kernels that use it are
@@ -225,6 +226,12 @@ struct Ctx {
   DevBuf<uint8_t> b_step, b_outer, b_inner;   // still in the super-step / outer / sweep loop
   DevBuf<uint32_t> b_ebeg, b_ecnt, b_ecur; // empty modules per instance: list segment, count, taken
   DevBuf<uint32_t> b_count;                // counters read by the host driver
+  // per-instance reductions in chunks of at most kTotChunk ids, none across
+  // two instances: chunk k is [b_cbeg[k], b_cend[k]), instance I owns the
+  // chunks [b_ichunk[I], b_ichunk[I + 1])
+  int32_t n_chunks = 0;
+  DevBuf<uint32_t> b_cbeg, b_cend, b_ichunk;
+  DevBuf<double> b_cpart;                  // 4 partial sums per chunk
 
   // sweep work arrays (sized for level 0, reused by level 1)
   DevBuf<uint8_t> act, act_next;
