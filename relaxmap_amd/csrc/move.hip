@@ -1874,22 +1874,6 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   fill_u32(c.counters.p + cUnits, 0u, 1, s);
   fill_u32(c.counters.p + cErr, 0u, 1, s);
   fill_u32(c.counters.p + kStripeBase, 0u, kStripes * kStripeWords, s);
-  RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
-  c.sync();
-  n_empty = c.sub_mode ? 0u : c.h_cnt[8];
-  uint32_t empty_cursor = 0;
-  if (batch) {   // each instance's empty modules
-    c.b_ebeg.alloc(c.nb);
-    c.b_ecnt.alloc(c.nb);
-    c.b_ecur.alloc(c.nb);
-    k_bempty_seg<<<grid_for(c.nb, 256), 256, 0, s>>>(c.nb, c.b_off.p, c.empty_list.p, n_empty, c.b_ebeg.p,
-                                                      c.b_ecnt.p, c.b_ecur.p);
-    RMX_LAUNCH_CHECK();
-  }
-  if (c.prof.on) {   // h_cnt mirrors the counters from here on (profiling deltas)
-    RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
-    c.sync();
-  }
 
   // hub hash-table pool
   const int32_t n_hubs = L.bucket_off[kNumBuckets] - L.bucket_off[kNumBuckets - 1];
@@ -1948,10 +1932,26 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     k_hist256<<<(unsigned)std::min<int64_t>(1024, (n + 255) / 256), 256, 0, s>>>(c.batch_keys.p, n,
                                                                                c.counters.p + 16);
     RMX_LAUNCH_CHECK();
-    std::vector<uint32_t> h(256);
-    RMX_HIP(hipMemcpyAsync(h.data(), c.counters.p + 16, sizeof(uint32_t) * 256, hipMemcpyDeviceToHost, s));
+    // one wait for the host: the empty-module count (word 8) and the
+    // (bucket, batch) histogram (words 16..271)
+    std::vector<uint32_t> h(16 + 256);
+    RMX_HIP(hipMemcpyAsync(h.data(), c.counters.p, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost, s));
     c.sync();
-    for (int k = 0; k < NK; ++k) seg[k + 1] = seg[k] + h[k];
+    for (int k = 0; k < NK; ++k) seg[k + 1] = seg[k] + h[16 + k];
+    n_empty = c.sub_mode ? 0u : h[8];
+  }
+  uint32_t empty_cursor = 0;
+  if (batch) {   // each instance's empty modules
+    c.b_ebeg.alloc(c.nb);
+    c.b_ecnt.alloc(c.nb);
+    c.b_ecur.alloc(c.nb);
+    k_bempty_seg<<<grid_for(c.nb, 256), 256, 0, s>>>(c.nb, c.b_off.p, c.empty_list.p, n_empty, c.b_ebeg.p,
+                                                      c.b_ecnt.p, c.b_ecur.p);
+    RMX_LAUNCH_CHECK();
+  }
+  if (c.prof.on) {   // h_cnt mirrors the counters from here on (profiling deltas)
+    RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
+    c.sync();
   }
 
   EvalArgs a = sweep_args(c, L, prioritized, vthresh, sweep_seed, batch);
@@ -1965,10 +1965,12 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     a.empty_avail = empty_cursor < n_empty ? 1 : 0;
     k_reset_counters<<<1, 64, 0, s>>>(c.counters.p);
     RMX_LAUNCH_CHECK();
+    uint32_t units_b = 0;   // units evaluated in this batch
     for (int bk = 0; bk < kNumBuckets; ++bk) {
       const int key = bk * (B + 1) + b;
       const int32_t cnt = (int32_t)(seg[key + 1] - seg[key]);
       if (cnt <= 0) continue;
+      units_b += (uint32_t)cnt;
       a.list = c.batch_list.p + seg[key];
       a.list_n = cnt;
       if (c.prof.on) c.prof.begin(s);
@@ -2030,6 +2032,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       if (c.prof.on) profile_eval(c, bk, s);
     }
     fold_counters(c, s);
+    if (units_b == 0) continue;
     RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
     c.sync();
     if (c.h_cnt[cErr]) throw HipError("move sweep: mover list of a counter copy overflowed");
@@ -2043,7 +2046,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       k_hub_clear<<<4096, 256, 0, s>>>(a, P);
       k_hub_insert<<<4096, 256, 0, s>>>(a, P);
       if (f32) k_hub_eval<true><<<4096, 256, 0, s>>>(a, P);
-            else k_hub_eval<false><<<4096, 256, 0, s>>>(a, P);
+      else k_hub_eval<false><<<4096, 256, 0, s>>>(a, P);
       k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (hub_n + 3) / 4), 256, 0, s>>>(a, P);
       RMX_LAUNCH_CHECK();
       fold_counters(c, s);
@@ -2067,7 +2070,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
         RMX_LAUNCH_CHECK();
       }
     } else {
-      if (n_want > 0) exclusive_scan_u32(c.want_empty.p, c.rank_tmp.p, n, nullptr, c.scratch, s);
+      if (n_want > 0 && n_empty > 0) exclusive_scan_u32(c.want_empty.p, c.rank_tmp.p, n, nullptr, c.scratch, s);
       k_resolve<<<stride_grid(n_movers, 256), 256, 0, s>>>(c.movers.p, c.counters.p, c.target.p, c.rank_tmp.p,
                                                             c.empty_list.p, empty_cursor, n_empty, a.empty_key,
                                                             c.newmod.p, c.stamp.p, c.want_empty.p, c.stamp_now);
@@ -2110,6 +2113,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   } else {
     module_totals(c, &c.sum_exit, &c.codelength);
   }
+  if (c.h_cnt[cErr]) throw HipError("move sweep: mover list of a counter copy overflowed");
   const uint32_t moved = c.h_cnt[cMoved];
   unsigned long long vis = (unsigned long long)c.h_cnt[cVisitsLo] | ((unsigned long long)c.h_cnt[cVisitsLo + 1] << 32);
   if (edge_visits) *edge_visits = (int64_t)vis;
