@@ -59,10 +59,12 @@ __global__ void k_node_sp(const uint32_t *__restrict__ mod, const uint32_t *__re
 // one super node (bits leaves the all-ones pattern unused by valid keys)
 constexpr uint64_t kIntra = ~0ull;
 
+// The value sorted with each key: the arc's index, or (carry_wf) the bits
+// of its FP32 flow, which is then all the reduce needs of it.
 template <int G>
 __global__ void k_arc_keys(const uint32_t *__restrict__ optr, const uint32_t *__restrict__ oidx, int32_t n,
-                           const uint32_t *__restrict__ node_sp, int bits, uint64_t *__restrict__ keys,
-                           uint32_t *__restrict__ vals) {
+                           const uint32_t *__restrict__ node_sp, int bits, const float *__restrict__ carry_wf,
+                           uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t a = t / G;
   const int lane = (int)(t % G);
@@ -71,7 +73,7 @@ __global__ void k_arc_keys(const uint32_t *__restrict__ optr, const uint32_t *__
   for (uint32_t k = optr[a] + lane; k < optr[a + 1]; k += G) {
     const uint32_t sb = node_sp[oidx[k]];
     keys[k] = sb != sa ? (((uint64_t)sa << bits) | (uint64_t)sb) : kIntra;
-    vals[k] = k;
+    vals[k] = carry_wf ? __float_as_uint(carry_wf[k]) : k;
   }
 }
 
@@ -98,12 +100,12 @@ __global__ void k_key_starts(const uint32_t *__restrict__ flag, const uint32_t *
 }
 
 // one thread per (src, dst) super arc: sequential FP64 sum in arc order (in
-// mixed precision, from the node level: of the FP32 arc flows); also the
-// transpose's key (dst, src) and the fixed-point weight it carries
+// mixed precision, from the node level: of the FP32 arc flows, carried by
+// the sort in place of the arc index); also the transpose's key (dst, src)
+// and the fixed-point weight it carries
 __global__ void k_reduce_segments(const uint32_t *__restrict__ start, int64_t n_seg, int64_t m,
                                   const uint64_t *__restrict__ keys, const uint32_t *__restrict__ arc,
-                                  const double *__restrict__ w, const float *__restrict__ wf,
-                                  const long long *__restrict__ wx, int mixed, int bits,
+                                  const double *__restrict__ w, const long long *__restrict__ wx, int mixed, int bits,
                                   uint32_t *__restrict__ out_idx, double *__restrict__ out_w,
                                   long long *__restrict__ out_wx, uint64_t *__restrict__ tkey,
                                   uint64_t *__restrict__ twx) {
@@ -118,7 +120,7 @@ __global__ void k_reduce_segments(const uint32_t *__restrict__ start, int64_t n_
     // mixed precision reads only the FP32 flow of a node-level arc: its
     // FP64 sum and its fixed-point value (the one the sweeps use) both
     // come from it
-    const double f = (mixed && !wx) ? (double)wf[k] : w[k];
+    const double f = (mixed && !wx) ? (double)__uint_as_float(k) : w[k];
     sum += f;
     sx += wx ? wx[k] : fx_of(f);
   }
@@ -284,8 +286,9 @@ void contract(Ctx &c) {
   arcs.alloc(m);
   uint32_t m_inter = 0;
   if (m > 0) {
+    const bool carry = c.precision == 1 && !from_super;   // node level, mixed: the sort carries the FP32 flows
     k_arc_keys<16><<<grid_for((int64_t)ns_src * 16, 256), 256, 0, s>>>(S.out_ptr.p, S.out_idx.p, ns_src, usp, bits,
-                                                                        keys.p, arcs.p);
+                                                                        carry ? S.out_wf.p : nullptr, keys.p, arcs.p);
     RMX_LAUNCH_CHECK();
     radix_sort_pairs_u64(keys.p, arcs.p, m, 0, 2 * bits, c.scratch, s);
     k_count_inter<<<1, 1, 0, s>>>(keys.p, m, atot.p);
@@ -322,7 +325,7 @@ void contract(Ctx &c) {
 
   if (m_super > 0) {
     k_reduce_segments<<<grid_for(m_super, 256), 256, 0, s>>>(
-        kstart.p, m_super, m_inter, keys.p, arcs.p, S.out_w.p, S.out_wf.p, from_super ? S.out_wx.p : nullptr,
+        kstart.p, m_super, m_inter, keys.p, arcs.p, S.out_w.p, from_super ? S.out_wx.p : nullptr,
         c.precision == 1 ? 1 : 0, bits, D.out_idx.p, D.out_w.p, D.out_wx.p, tkey.p, twx.p);
     RMX_LAUNCH_CHECK();
   }

@@ -108,6 +108,31 @@ def test_contraction(key, pname):
         assert np.allclose(s["exit"], [sp["exit"] for sp in c["super"]], rtol=1e-10, atol=1e-17)
 
 
+@pytest.mark.parametrize("key", PART_KEYS)
+def test_contraction_mixed_precision(key):
+    """Mixed precision contracts the FP32 arc flows (carried through the key
+    sort in place of the arc index): the integer structure is the reference's
+    bit for bit, the super-arc weights and statistics are FP64 sums of FP32
+    flows of the mixed steady state, within 1e-5 relative of the reference."""
+    g = golden(f"{key}.part_swept")
+    c = g["contract"]
+    with _gpu(key, precision="mixed") as net:
+        net.set_partition(np.array(g["partition"], dtype=np.int32))
+        ns, ms = net.convert_modules_to_super_nodes()
+        s = net.super_nodes()
+    assert ns == c["n_super"]
+    assert ms == sum(len(sp["out"]) for sp in c["super"])
+    assert np.array_equal(s["out_ptr"], np.cumsum([0] + [len(sp["out"]) for sp in c["super"]]))
+    assert np.array_equal(s["in_ptr"], np.cumsum([0] + [len(sp["in"]) for sp in c["super"]]))
+    assert s["out_idx"].tolist() == [e[0] for sp in c["super"] for e in sp["out"]]
+    assert s["in_idx"].tolist() == [e[0] for sp in c["super"] for e in sp["in"]]
+    ref_out_w = [e[1] for sp in c["super"] for e in sp["out"]]
+    ref_in_w = [e[1] for sp in c["super"] for e in sp["in"]]
+    assert np.allclose(s["out_w"], ref_out_w, rtol=1e-5, atol=0)
+    assert np.allclose(s["in_w"], ref_in_w, rtol=1e-5, atol=0)
+    assert np.allclose(s["exit"], [sp["exit"] for sp in c["super"]], rtol=1e-5, atol=1e-12)
+
+
 @pytest.mark.parametrize("key", ["lfr1k_mu03", "sbm1k", "lfr5k_mu05"])
 def test_contraction_from_super_level(key):
     """Contracting an already contracted level (its super arcs summed again)
