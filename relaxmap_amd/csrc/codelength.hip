@@ -58,6 +58,38 @@ __global__ __launch_bounds__(kRedBlock) void k_module_totals(const Stat4 *__rest
   if (threadIdx.x == 0) part[3 * kRedGrid + blockIdx.x] = cnt;
 }
 
+// qv[m] = q of module m, 0 for an empty module (its statistics are then
+// exactly zero)
+__global__ void k_qv_all(const Stat4 *__restrict__ ms, const int32_t *__restrict__ mNodes, int32_t n,
+                         double *__restrict__ qv) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double q = 0.0;
+  if (mNodes[m] > 0) {
+    const Stat4 st = ms[m];
+    q = module_exit(st.p, st.t, st.d, st.x);
+  }
+  qv[m] = q;
+}
+
+// S in k_module_totals' launch shape and order: an empty module adds +0.0,
+// which leaves every partial sum unchanged
+__global__ __launch_bounds__(kRedBlock) void k_sum_qv(const double *__restrict__ qv, int32_t n,
+                                                      double *__restrict__ part) {
+  __shared__ double lds[kRedBlock / kWave];
+  double s = 0;
+  for (int64_t m = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; m < n; m += (int64_t)kRedGrid * kRedBlock)
+    s += qv[m];
+  s = block_sum<double, kRedBlock>(s, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_sum_final(const double *__restrict__ part, double *__restrict__ scal) {
+  __shared__ double lds[kRedBlock / kWave];
+  const double s = reduce_partials(part, lds);
+  if (threadIdx.x == 0) scal[kS] = s;
+}
+
 __global__ __launch_bounds__(kRedBlock) void k_totals_final(const double *__restrict__ part, double all_log_all,
                                                             double *__restrict__ scal) {
   __shared__ double lds[kRedBlock / kWave];
@@ -324,6 +356,19 @@ void module_totals_async(Ctx &c) {
                                                          c.partial.p);
   RMX_LAUNCH_CHECK();
   k_totals_final<<<1, kRedBlock, 0, c.stream>>>(c.partial.p, c.all_log_all, c.scal.p);
+  RMX_LAUNCH_CHECK();
+}
+
+void exit_probs_refresh(Ctx &c) {
+  c.qv.alloc(c.N);
+  k_qv_all<<<grid_for(c.N, 256), 256, 0, c.stream>>>(c.ms.p, c.mNodes.p, c.N, c.qv.p);
+  RMX_LAUNCH_CHECK();
+}
+
+void sum_exit_async(Ctx &c) {
+  k_sum_qv<<<kRedGrid, kRedBlock, 0, c.stream>>>(c.qv.p, c.N, c.partial.p);
+  RMX_LAUNCH_CHECK();
+  k_sum_final<<<1, kRedBlock, 0, c.stream>>>(c.partial.p, c.scal.p);
   RMX_LAUNCH_CHECK();
 }
 

@@ -207,6 +207,7 @@ struct Ctx {
 
   // module SoA, N slots (module ids are node ids, as in the reference)
   DevBuf<Stat4> ms;                     // fixed-point sums: flow, teleport, dangling, exit arcs
+  DevBuf<double> qv;                    // exit probability of every module (0 when empty), during a sweep
   DevBuf<int32_t> mNodes, mUnits, mStale;  // original nodes, units at the current level, stale count
 
   // A batch of independent instances (rmx_batch_upload): instance I owns the
@@ -314,6 +315,10 @@ void init_modules_singletons(Ctx &c);
 // Recomputes S = sum_m q_m and L from module statistics (tracked values).
 void module_totals(Ctx &c, double *S, double *L);
 void module_totals_async(Ctx &c);   // leaves S in c.scal[0], L in c.scal[1]
+// qv from the module statistics, and S = sum of qv into c.scal[0] (the same
+// sum, in the same order, as module_totals_async's S)
+void exit_probs_refresh(Ctx &c);
+void sum_exit_async(Ctx &c);
 void set_partition(Ctx &c, const uint32_t *d_part);
 double calculate_codelength(Ctx &c);
 void rebuild_exit_flows(Ctx &c);

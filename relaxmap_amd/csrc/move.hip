@@ -675,6 +675,8 @@ __device__ inline int chunk_owner(const uint32_t *__restrict__ pre, int cnt, uin
   return lo;
 }
 
+// inclusive scan over a workgroup of BS lanes; *total = the sum of all
+template <int BS = 256>
 __device__ inline uint32_t block_incl_scan(uint32_t v, uint32_t *lds, uint32_t *total) {
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   uint32_t x = v;
@@ -686,7 +688,7 @@ __device__ inline uint32_t block_incl_scan(uint32_t v, uint32_t *lds, uint32_t *
   if (lane == kWave - 1) lds[w] = x;
   __syncthreads();
   uint32_t pre = 0, all = 0;
-  for (int i = 0; i < 256 / kWave; ++i) {
+  for (int i = 0; i < BS / kWave; ++i) {
     if (i < w) pre += lds[i];
     all += lds[i];
   }
@@ -968,15 +970,17 @@ __device__ inline int lds_insert_bounded(uint32_t *keys, int H, uint32_t m) {
   return -1;   // table full
 }
 
-__global__ __launch_bounds__(256) void k_hubp_plan(EvalArgs a, HubParts H) {
-  __shared__ uint32_t lds[256 / kWave];
+// one workgroup of 1024 lanes: the hub list is scanned in steps of 1024
+constexpr int kPlanBlock = 1024;
+__global__ __launch_bounds__(kPlanBlock) void k_hubp_plan(EvalArgs a, HubParts H) {
+  __shared__ uint32_t lds[kPlanBlock / kWave];
   uint32_t ca = 0, cp = 0, cb = 0;
   if (threadIdx.x == 0) {
     H.A[0] = 0;
     H.PB[0] = 0;
     H.AB[0] = 0;
   }
-  for (int base = 0; base < a.list_n; base += 256) {
+  for (int base = 0; base < a.list_n; base += kPlanBlock) {
     const int i = base + threadIdx.x;
     uint32_t ac = 0, pc = 0, dc = 0;
     if (i < a.list_n) {
@@ -988,9 +992,9 @@ __global__ __launch_bounds__(256) void k_hubp_plan(EvalArgs a, HubParts H) {
       dc = deg;
     }
     uint32_t ta, tp, tb;
-    const uint32_t ia = block_incl_scan(ac, lds, &ta);
-    const uint32_t ip = block_incl_scan(pc, lds, &tp);
-    const uint32_t ib = block_incl_scan(dc, lds, &tb);
+    const uint32_t ia = block_incl_scan<kPlanBlock>(ac, lds, &ta);
+    const uint32_t ip = block_incl_scan<kPlanBlock>(pc, lds, &tp);
+    const uint32_t ib = block_incl_scan<kPlanBlock>(dc, lds, &tb);
     if (i < a.list_n) {
       H.A[i + 1] = ca + ia;
       H.PB[i + 1] = cp + ip;
@@ -1565,6 +1569,23 @@ __global__ __launch_bounds__(256) void k_commit_big(const uint32_t *__restrict__
   }
 }
 
+// after a batch's commits, before k_apply: q of the old and the new module
+// of every mover (a module met twice gets the same value twice)
+__global__ void k_qv_touched(const uint32_t *__restrict__ movers, const uint32_t *__restrict__ counters,
+                             const uint32_t *__restrict__ stamp, uint32_t cur, const uint32_t *__restrict__ mod,
+                             const uint32_t *__restrict__ newmod, const Stat4 *__restrict__ ms,
+                             const int32_t *__restrict__ mNodes, double *__restrict__ qv) {
+  const uint32_t nm = counters[cMovers];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nm; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = movers[i];
+    if (stamp[u] != cur) continue;
+    const uint32_t o = mod[u], n = newmod[u];
+    const Stat4 so = ms[o], sn = ms[n];
+    qv[o] = mNodes[o] > 0 ? module_exit(so.p, so.t, so.d, so.x) : 0.0;
+    qv[n] = mNodes[n] > 0 ? module_exit(sn.p, sn.t, sn.d, sn.x) : 0.0;
+  }
+}
+
 __global__ void k_apply(const uint32_t *__restrict__ movers, const uint32_t *__restrict__ counters,
                         const uint32_t *__restrict__ newmod, const uint32_t *__restrict__ stamp, uint32_t cur,
                         uint32_t *__restrict__ mod) {
@@ -1955,6 +1976,10 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   }
 
   EvalArgs a = sweep_args(c, L, prioritized, vthresh, sweep_seed, batch);
+  // one network: S after each batch from the modules' exit probabilities,
+  // refreshed here and then for the modules each batch touches
+  const bool single = !batch && !c.sub_mode;
+  if (single) exit_probs_refresh(c);
 
   ArcW aout = arcw_out(c, L);
   const bool f32 = c.precision == 1;   // mixed precision: candidates in FP32
@@ -2011,7 +2036,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
           H.hubu = c.hp_hubu.p;
           H.chub = c.hp_chub.p;
           H.phub = c.hp_phub.p;
-          k_hubp_plan<<<1, 256, 0, s>>>(a, H);
+          k_hubp_plan<<<1, kPlanBlock, 0, s>>>(a, H);
           k_hubp_owners<<<grid_for((int64_t)cnt * kWave, 256), 256, 0, s>>>(a, H);
           fill_u32(H.bcnt, 0u, NB + 1, s);
           fill_i64(H.own, 0, 2 * (int64_t)cnt, s);
@@ -2095,11 +2120,18 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
                                         c.ms.p, c.act_next.p, prioritized ? 1 : 0);
       RMX_LAUNCH_CHECK();
     }
+    if (single) {
+      k_qv_touched<<<stride_grid(n_movers, 256), 256, 0, s>>>(c.movers.p, c.counters.p, c.stamp.p, c.stamp_now,
+                                                               L.mod.p, c.newmod.p, c.ms.p, c.mNodes.p, c.qv.p);
+      RMX_LAUNCH_CHECK();
+    }
     k_apply<<<stride_grid(n_movers, 256), 256, 0, s>>>(c.movers.p, c.counters.p, c.newmod.p, c.stamp.p,
                                                         c.stamp_now, L.mod.p);
     RMX_LAUNCH_CHECK();
-    if (c.sub_mode) instance_totals_async(c);   // S of every instance for the next batch
-    else module_totals_async(c);               // S for the next batch
+    // S for the next batch
+    if (c.sub_mode) instance_totals_async(c);   // of every instance
+    else if (batch) module_totals_async(c);     // of every instance
+    else sum_exit_async(c);
   }
   if (prioritized) {
     c.act.swap(c.act_next);
