@@ -53,6 +53,14 @@ int rmx_set_batches(rmx_ctx *ctx, int batches);
  * No reference counterpart. */
 int rmx_set_hub_path(rmx_ctx *ctx, int global_tables);
 
+/* 1: sweeps of a batch of instances (rmx_batch_upload) and of sub-module
+ * partitions run without host waits, each sweep's launches captured into one
+ * HIP graph and launched at once.  0 (default): the same launches one by one,
+ * with a host wait per batch; on MI355X this is the faster of the two (a
+ * graph per sweep costs its instantiation and runs no faster).  Both give
+ * identical results.  No reference counterpart. */
+int rmx_set_graph_capture(rmx_ctx *ctx, int on);
+
 /* Network construction: replaces the arc-list build of OmpRelaxmap.cpp:113-145
  * (after load_pajek_format_network / load_linkList_format_network,
  * FileIO.cpp:32-265).  out_ptr[n+1] / out_idx[m] / w[m] is the out-CSR with

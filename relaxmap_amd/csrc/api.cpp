@@ -153,6 +153,14 @@ int rmx_set_hub_path(rmx_ctx *ctx, int global_tables) {
   });
 }
 
+int rmx_set_graph_capture(rmx_ctx *ctx, int on) {
+  return guarded(ctx, [&] {
+    if (on != 0 && on != 1) return fail(ctx, RMX_EINVAL, "graph capture must be 0 or 1");
+    ctx->c.use_graph = on;
+    return RMX_OK;
+  });
+}
+
 }  // extern "C"
 
 namespace {

@@ -193,6 +193,7 @@ struct Ctx {
   int precision = 0;   // 0 FP64, 1 mixed (FP32 bulk, FP64 accumulation)
   int batches = 8;     // sub-sweeps per sweep
   int hub_global = 0;  // 1: hubs with global-memory tables instead of LDS partitions
+  int use_graph = 0;   // sweeps of a batch or a sub-module partition as one captured graph
   BucketCuts cuts = {{kBucketMaxDeg[0], kBucketMaxDeg[1], kBucketMaxDeg[2], kBucketMaxDeg[3], kBucketMaxDeg[4],
                       kBucketMaxDeg[5], kBucketMaxDeg[6], kBucketMaxDeg[7]}};
   bool initiated = false;

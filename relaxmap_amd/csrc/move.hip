@@ -732,6 +732,7 @@ __global__ __launch_bounds__(256) void k_hub_plan(EvalArgs a, HubPlan P) {
       const uint32_t deg = (a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u]);
       ac = deg == 0 ? 1u : (deg + kHubArcChunk - 1) / kHubArcChunk;
       sc = ((uint32_t)cap + kHubChunk - 1) / kHubChunk;
+      if (P.only && !P.only[i]) ac = sc = 0;   // not flagged: no work at all
     }
     uint32_t ta, ts;
     const uint32_t ia = block_incl_scan(ac, lds, &ta);
@@ -1872,6 +1873,41 @@ void reset_active(Ctx &c) {
   fill_u8(c.act_next.p, 0, c.N, c.stream);
 }
 
+// The launches of one sweep captured into a HIP graph (when on): capture
+// starts at construction; launch() ends it, instantiates and launches the
+// graph; the destructor ends an unfinished capture (an exception in between)
+// and frees the graph, after the caller has waited for the stream.
+struct SweepGraph {
+  hipStream_t s;
+  bool on, capturing = false;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  SweepGraph(hipStream_t s_, bool on_) : s(s_), on(on_) {
+    if (on) {
+      RMX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      capturing = true;
+    }
+  }
+  void launch() {
+    if (!on) return;
+    capturing = false;
+    RMX_HIP(hipStreamEndCapture(s, &g));
+    RMX_HIP(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+    RMX_HIP(hipGraphLaunch(x, s));
+  }
+  ~SweepGraph() {
+    if (capturing) {
+      hipGraph_t t = nullptr;
+      (void)hipStreamEndCapture(s, &t);
+      if (t) (void)hipGraphDestroy(t);
+    }
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+  }
+  SweepGraph(const SweepGraph &) = delete;
+  SweepGraph &operator=(const SweepGraph &) = delete;
+};
+
 int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int64_t *edge_visits) {
   ensure_sweep_buffers(c);
   Level &L = c.cur();
@@ -1983,6 +2019,41 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
 
   ArcW aout = arcw_out(c, L);
   const bool f32 = c.precision == 1;   // mixed precision: candidates in FP32
+
+  // the global-table evaluation of this batch's hubs whose partition table
+  // overflowed (its plan gives the other hubs no work)
+  auto hub_fallback = [&](const uint32_t *list, int32_t cnt) {
+    a.list = list;
+    a.list_n = cnt;
+    HubPlan P{c.hub_plan.p, c.hub_plan.p + (cnt + 1), c.hub_plan.p + 2 * (cnt + 1), c.hub_bd.p, c.hub_bm.p,
+              c.hp_ovf.p};
+    k_hub_plan<<<1, 256, 0, s>>>(a, P);
+    k_hub_clear<<<4096, 256, 0, s>>>(a, P);
+    k_hub_insert<<<4096, 256, 0, s>>>(a, P);
+    if (f32) k_hub_eval<true><<<4096, 256, 0, s>>>(a, P);
+    else k_hub_eval<false><<<4096, 256, 0, s>>>(a, P);
+    k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, P);
+    RMX_LAUNCH_CHECK();
+    fold_counters(c, s);
+  };
+
+  // A batch of instances or a sub-module partition keeps its empty-module
+  // cursors on the device (or offers none), so its sweep needs no host wait:
+  // the resolve / commit kernels read the mover count on the device and run
+  // on grids sized for every unit of the batch, the want-empty scan runs in
+  // every batch, and the hub fallback is launched whenever hubs were
+  // evaluated.  The sweep's launches are then captured into one graph.
+  const bool quiet = c.use_graph && !c.prof.on && (batch || c.sub_mode);
+  if (quiet) {   // nothing may allocate while the stream is captured
+    const int64_t nb1 = n_hubs > 0 ? L.hub_parts + 1 : 0;
+    exclusive_scan_reserve(std::max<int64_t>(n, nb1), c.scratch);
+    if (batch) {
+      c.b_S.alloc(c.nb);
+      c.b_L.alloc(c.nb);
+      c.b_nmod.alloc(c.nb);
+    }
+  }
+  SweepGraph graph(s, quiet);
   for (int b = 0; b < c.batches; ++b) {
     const uint32_t *hub_list = nullptr;   // this batch's hubs, for the overflow fallback
     int32_t hub_n = 0;
@@ -2058,29 +2129,23 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     }
     fold_counters(c, s);
     if (units_b == 0) continue;
-    RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
-    c.sync();
-    if (c.h_cnt[cErr]) throw HipError("move sweep: mover list of a counter copy overflowed");
-    if (hub_n > 0 && c.h_cnt[cHubOvf] > 0) {
-      // a partition table overflowed: those hubs again with global tables
-      a.list = hub_list;
-      a.list_n = hub_n;
-      HubPlan P{c.hub_plan.p, c.hub_plan.p + (hub_n + 1), c.hub_plan.p + 2 * (hub_n + 1), c.hub_bd.p, c.hub_bm.p,
-                c.hp_ovf.p};
-      k_hub_plan<<<1, 256, 0, s>>>(a, P);
-      k_hub_clear<<<4096, 256, 0, s>>>(a, P);
-      k_hub_insert<<<4096, 256, 0, s>>>(a, P);
-      if (f32) k_hub_eval<true><<<4096, 256, 0, s>>>(a, P);
-      else k_hub_eval<false><<<4096, 256, 0, s>>>(a, P);
-      k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (hub_n + 3) / 4), 256, 0, s>>>(a, P);
-      RMX_LAUNCH_CHECK();
-      fold_counters(c, s);
+    uint32_t n_movers = units_b, n_want = 1;   // bounds when not read back
+    if (quiet) {
+      if (hub_n > 0) hub_fallback(hub_list, hub_n);
+    } else {
       RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
       c.sync();
       if (c.h_cnt[cErr]) throw HipError("move sweep: mover list of a counter copy overflowed");
+      if (hub_n > 0 && c.h_cnt[cHubOvf] > 0) {   // a partition table overflowed: those hubs again
+        hub_fallback(hub_list, hub_n);
+        RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
+        c.sync();
+        if (c.h_cnt[cErr]) throw HipError("move sweep: mover list of a counter copy overflowed");
+      }
+      n_movers = c.h_cnt[cMovers];
+      n_want = c.h_cnt[cWant];
+      if (n_movers == 0) continue;
     }
-    const uint32_t n_movers = c.h_cnt[cMovers], n_want = c.h_cnt[cWant];
-    if (n_movers == 0) continue;
     ++c.stamp_now;
     if (batch) {
       if (n_want > 0) exclusive_scan_u32(c.want_empty.p, c.rank_tmp.p, n, c.counters.p + 9, c.scratch, s);
@@ -2133,6 +2198,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     else if (batch) module_totals_async(c);     // of every instance
     else sum_exit_async(c);
   }
+  graph.launch();
   if (prioritized) {
     c.act.swap(c.act_next);
     fill_u8(c.act_next.p, 0, c.N, s);

@@ -18,6 +18,9 @@ struct Scratch {
 // In-place (out == in) is allowed.  n < 2^32.
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *d_total,
                         Scratch &scratch, hipStream_t s);
+// grows scratch to what exclusive_scan_u32 over n elements takes (so that
+// the scan allocates nothing, e.g. under stream capture)
+void exclusive_scan_reserve(int64_t n, Scratch &scratch);
 
 // Stable LSD radix sort of (key, val) pairs on bits [begin_bit, end_bit).
 // keys/vals are sorted in place; tmp buffers of n elements are taken from

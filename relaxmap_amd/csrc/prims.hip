@@ -464,6 +464,8 @@ void exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *
   scan_impl(in, out, n, d_total, tmp, s);
 }
 
+void exclusive_scan_reserve(int64_t n, Scratch &scratch) { scratch.get(sizeof(uint32_t) * scan_tmp_elems(n)); }
+
 void radix_sort_pairs_u32(uint32_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit,
                           Scratch &scratch, hipStream_t s) {
   radix_sort_impl<uint32_t, uint32_t>(keys, vals, n, begin_bit, end_bit, scratch, s);

@@ -170,3 +170,42 @@ def test_batch_degenerate_instances():
             ref = _oracle_codelength(graphs[gi], part[a:b] - a)
             tol = 1e-9 if prec == "fp64" else 1e-6
             assert abs(res[(gi, s)][2] - ref) <= tol * max(abs(ref), 1e-12), (prec, gi, res[(gi, s)][2], ref)
+
+
+# ------------------------------------------------------------ graph capture
+def _run(setup, graph_on, hub_mode=0, precision="mixed"):
+    from relaxmap_amd import Network
+    with Network(precision=precision) as net:
+        net.set_graph_capture(graph_on)
+        net.set_hub_path(hub_mode)
+        setup(net)
+        net.initiate()
+        r = net.run(seed=3)
+        return r.computed_codelength, r.sweeps, net.partition().tolist()
+
+
+def test_graph_capture_same_results_batch():
+    """Sweeps of a batch of instances as one captured graph each give the
+    results of the same launches made one by one with host waits."""
+    from relaxmap_amd.batch import union_of
+    graphs = _graphs()[:4]
+    _, node_off, iseeds, ptr, idx, w = union_of(graphs, [1, 2])
+    setup = lambda net: net.batch_upload(node_off, iseeds, ptr, idx, w)
+    for prec in ("mixed", "fp64"):
+        on, off = _run(setup, True, precision=prec), _run(setup, False, precision=prec)
+        print(f"[gpu] batch graph on/off {prec}: L {on[0]:.12f} / {off[0]:.12f}, sweeps {on[1]} / {off[1]}", flush=True)
+        assert on == off
+
+
+def test_graph_capture_same_results_coarse_tune_with_hubs():
+    """One network: the coarse tunes' sub-module sweeps run captured; with
+    units of more than 2048 arcs, also with every hub sent through the
+    overflow fallback (launched in every captured batch, working only on
+    flagged hubs)."""
+    from relaxmap_amd import synth
+    g = synth.chunglu(60000, 2.1, 4, 20000, 16.0, 1.2, 50, 5000, 0.4, 0.01, 11)
+    setup = lambda net: g.upload_to(net)
+    for hub_mode in (0, 2):
+        on, off = _run(setup, True, hub_mode), _run(setup, False, hub_mode)
+        print(f"[gpu] coarse graph on/off hub mode {hub_mode}: L {on[0]:.12f} / {off[0]:.12f}", flush=True)
+        assert on == off
