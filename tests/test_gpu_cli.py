@@ -1,0 +1,53 @@
+"""The command line (relaxmap_amd/relaxmap, with ompRelaxmap's arguments) end
+to end: its .tree / .clu outputs are those of the same run through the
+binding, and a network given as arrays is written with the default names
+1..n."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import INPUTS, ROOT, data_path
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(ROOT, "relaxmap_amd", "relaxmap")
+
+
+@pytest.mark.parametrize("key,mode", [("lfr1k_mu03", "prior"), ("sbm1k", "normal"), ("flow_net", "prior")])
+def test_cli_matches_binding(tmp_path, key, mode):
+    from relaxmap_amd import Network
+    if not os.path.exists(EXE):
+        pytest.fail("CLI not built")
+    src = data_path(INPUTS[key])
+    r = subprocess.run([EXE, "7", src, "4", "1", "1e-3", "0", "10", str(tmp_path), mode], capture_output=True,
+                       text=True, timeout=120)
+    print(f"[gpu] cli {key} {mode}: rc {r.returncode}", flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Computed Code Length" in r.stdout
+    name = os.path.splitext(os.path.basename(src))[0]
+    with Network() as net:
+        net.load(src)
+        net.initiate()
+        net.run(threshold=1e-3, vthresh=0.0, max_iter=10, prior=mode == "prior", seed=7)
+        net.write_tree(tmp_path / "api.tree")
+        net.write_clu(tmp_path / "api.clu")
+    assert (tmp_path / f"{name}.tree").read_text() == (tmp_path / "api.tree").read_text()
+    assert (tmp_path / f"{name}.clu").read_bytes() == (tmp_path / "api.clu").read_bytes()
+
+
+def test_tree_names_after_upload(tmp_path):
+    from relaxmap_amd import Network, synth
+    g = synth.sbm(500, 10, 10.0, 0.2, 3)
+    with Network() as net:
+        g.upload_to(net)
+        net.initiate()
+        net.run(seed=1)
+        part = net.partition()
+        net.write_tree(tmp_path / "g.tree")
+    lines = (tmp_path / "g.tree").read_text().splitlines()
+    assert lines[0].startswith("# Code length ")
+    names = [ln.rsplit('"', 2)[1] for ln in lines[1:]]
+    assert sorted(names, key=int) == [str(i + 1) for i in range(g.n)]
+    assert len(set(ln.split(":")[0] for ln in lines[1:])) == len(np.unique(part))
