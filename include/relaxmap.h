@@ -61,6 +61,15 @@ int rmx_set_hub_path(rmx_ctx *ctx, int global_tables);
  * identical results.  No reference counterpart. */
 int rmx_set_graph_capture(rmx_ctx *ctx, int on);
 
+/* How rmx_partition runs attempts > 1 on one network: 0 (default) as one
+ * batch of instances (see rmx_batch_upload), a copy of the network per
+ * attempt with its own seed, when attempts x arcs <= 2^28 and attempts x nodes
+ * <= 2^27, else one after another; 1 always one after another.  Either way
+ * every attempt starts from singletons and the partition with the lowest
+ * codelength is kept.  No reference counterpart (the reference parses the
+ * attempt count and does not use it, OmpRelaxmap.cpp:50). */
+int rmx_set_attempt_mode(rmx_ctx *ctx, int sequential);
+
 /* Network construction: replaces the arc-list build of OmpRelaxmap.cpp:113-145
  * (after load_pajek_format_network / load_linkList_format_network,
  * FileIO.cpp:32-265).  out_ptr[n+1] / out_idx[m] / w[m] is the out-CSR with

@@ -74,6 +74,7 @@ def load_library():
         "rmx_set_batches": (C.c_int, [vp, C.c_int]),
         "rmx_set_hub_path": (C.c_int, [vp, C.c_int]),
         "rmx_set_graph_capture": (C.c_int, [vp, C.c_int]),
+        "rmx_set_attempt_mode": (C.c_int, [vp, C.c_int]),
         "rmx_net_read": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(vp), C.c_char_p, C.c_int]),
         "rmx_net_free": (None, [vp]),
         "rmx_net_info": (C.c_int, [vp, _I32P, _I64P, _I32P, _I32P]),
@@ -199,6 +200,11 @@ class Network:
         default), with one global hash table each (1), or by partitions with
         every hub sent through the overflow fallback (2); same moves."""
         self._check(self._L.rmx_set_hub_path(self._h, int(mode)))
+
+    def set_attempt_mode(self, sequential):
+        """run(attempts > 1) as one batch of copies of the network (False, the
+        default, where the copies fit) or one attempt after another (True)."""
+        self._check(self._L.rmx_set_attempt_mode(self._h, 1 if sequential else 0))
 
     def set_graph_capture(self, on):
         """Sweeps of a batch of instances and of sub-module partitions as one

@@ -153,6 +153,14 @@ int rmx_set_hub_path(rmx_ctx *ctx, int global_tables) {
   });
 }
 
+int rmx_set_attempt_mode(rmx_ctx *ctx, int sequential) {
+  return guarded(ctx, [&] {
+    if (sequential != 0 && sequential != 1) return fail(ctx, RMX_EINVAL, "attempt mode must be 0 or 1");
+    ctx->c.attempts_sequential = sequential;
+    return RMX_OK;
+  });
+}
+
 int rmx_set_graph_capture(rmx_ctx *ctx, int on) {
   return guarded(ctx, [&] {
     if (on != 0 && on != 1) return fail(ctx, RMX_EINVAL, "graph capture must be 0 or 1");
@@ -831,6 +839,111 @@ void reset_to_singletons(Ctx &c) {
 
 }  // namespace
 
+namespace {
+
+// Attempts > 1 on one network as one batch of instances: a temporary context
+// holds a copy of the network per attempt (seeds seed, seed + 1, ...), runs
+// them together, and the copy with the lowest codelength (the first, on a
+// tie) gives this context its partition.  Returns a status.
+int batched_attempts(rmx_ctx *ctx, const rmx_partition_opts &o, rmx_partition_result &r) {
+  Ctx &c = ctx->c;
+  const int A = o.attempts;
+  const int32_t n = c.N;
+  const int64_t m = c.E;
+  const double t_start = now_s();
+  std::vector<uint32_t> p32((size_t)n + 1), idx((size_t)m);
+  std::vector<double> w((size_t)m), nw((size_t)n);
+  RMX_HIP(hipMemcpyAsync(p32.data(), c.L0.out_ptr.p, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToHost, c.stream));
+  if (m > 0) {
+    RMX_HIP(hipMemcpyAsync(idx.data(), c.L0.out_idx.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, c.stream));
+    RMX_HIP(hipMemcpyAsync(w.data(), c.raw_w.p, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+  }
+  RMX_HIP(hipMemcpyAsync(nw.data(), c.node_w.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  const int64_t N = (int64_t)A * n, M = (int64_t)A * m;
+  std::vector<int64_t> uptr((size_t)N + 1);
+  std::vector<int32_t> uidx((size_t)M), off((size_t)A + 1);
+  std::vector<double> uw((size_t)M), unw((size_t)N);
+  std::vector<uint64_t> seeds((size_t)A);
+  for (int a = 0; a < A; ++a) {
+    off[a] = (int32_t)((int64_t)a * n);
+    seeds[a] = o.seed + (uint64_t)a;
+    const int64_t bn = (int64_t)a * n, bm = (int64_t)a * m;
+    for (int32_t i = 0; i < n; ++i) {
+      uptr[bn + i] = bm + p32[i];
+      unw[bn + i] = nw[i];
+    }
+    for (int64_t k = 0; k < m; ++k) {
+      uidx[bm + k] = (int32_t)(idx[k] + bn);
+      uw[bm + k] = w[k];
+    }
+  }
+  uptr[N] = M;
+  off[A] = (int32_t)N;
+  rmx_ctx *t = nullptr;
+  if (rmx_create(&t, c.device) != RMX_OK) return fail(ctx, RMX_EDEVICE, "attempts: no second context");
+  struct Drop {
+    rmx_ctx *t;
+    ~Drop() { rmx_destroy(t); }
+  } drop{t};
+  t->c.precision = c.precision;
+  t->c.batches = c.batches;
+  t->c.cuts = c.cuts;
+  t->c.hub_global = c.hub_global;
+  t->c.use_graph = c.use_graph;
+  auto check = [&](int rc, const char *what) {
+    if (rc != RMX_OK) throw HipError(std::string("attempts: ") + what + ": " + rmx_last_error(t));
+  };
+  check(rmx_batch_upload(t, A, off.data(), seeds.data(), (int32_t)N, M, uptr.data(), uidx.data(), uw.data(),
+                         unw.data()),
+        "upload");
+  check(rmx_initiate(t, nullptr, nullptr, nullptr), "initiate");
+  rmx_partition_opts o1 = o;
+  o1.attempts = 1;
+  o1.verbose = 0;
+  rmx_partition_result rt;
+  check(rmx_partition(t, &o1, &rt), "partition");
+  std::vector<double> L((size_t)A);
+  check(rmx_batch_results(t, L.data(), nullptr, nullptr), "results");
+  int best = 0;
+  for (int a = 1; a < A; ++a)
+    if (L[a] < L[best]) best = a;
+  if (o.verbose)
+    for (int a = 0; a < A; ++a) std::printf("Attempt %d: Computed Code Length = %g\n", a + 1, L[a] / std::log(2.0));
+  std::vector<int32_t> part((size_t)N);
+  check(rmx_get_partition(t, part.data()), "partition");
+  std::vector<uint32_t> bp((size_t)n);
+  const int64_t b0 = (int64_t)best * n;
+  for (int32_t i = 0; i < n; ++i) bp[i] = (uint32_t)(part[b0 + i] - b0);   // the copy's modules lie in its range
+  DevBuf<uint32_t> d(c.cache);
+  d.alloc(n);
+  RMX_HIP(hipMemcpyAsync(d.p, bp.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
+  rmx::to_node_level(c);
+  set_partition(c, d.p);
+  c.sync();
+  r.super_steps = rt.super_steps;
+  r.sweeps = rt.sweeps;
+  r.edge_visits = rt.edge_visits;
+  r.time_sweeps_s = rt.time_sweeps_s;
+  r.time_contract_s = rt.time_contract_s;
+  r.time_coarse_s = rt.time_coarse_s;
+  r.best_attempt = best;
+  r.time_partition_s = now_s() - t_start;
+  r.codelength = c.codelength;
+  r.n_module = c.n_module;
+  r.computed_codelength = calculate_codelength(c);
+  c.timing.sweep_ms = rt.time_sweeps_s * 1e3;
+  c.timing.contract_ms = rt.time_contract_s * 1e3;
+  if (o.verbose) {
+    std::printf("Time for partitioning : %g (sec)\n", r.time_partition_s);
+    std::printf("DONE: Code Length = %g in %d modules, with %d nodes (attempt %d of %d).\n\n",
+                c.codelength / std::log(2.0), c.n_module, c.N, best + 1, A);
+  }
+  return RMX_OK;
+}
+
+}  // namespace
+
 extern "C" int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_partition_result *res) {
   return guarded(ctx, [&] {
     Ctx &c = ctx->c;
@@ -847,6 +960,9 @@ extern "C" int rmx_partition(rmx_ctx *ctx, const rmx_partition_opts *opts, rmx_p
     const double ln2 = std::log(2.0);
     const double t_start = now_s();
     const int attempts = opts->attempts > 1 ? opts->attempts : 1;
+    if (attempts > 1 && !c.attempts_sequential && (int64_t)attempts * c.E <= ((int64_t)1 << 28) &&
+        (int64_t)attempts * c.N <= ((int64_t)1 << 27))
+      return batched_attempts(ctx, *opts, *res);
     DevBuf<uint32_t> best(c.cache);
     double best_L = 0.0;
     Driver d{c, *opts, *res, now_s()};

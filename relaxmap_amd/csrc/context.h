@@ -194,6 +194,7 @@ struct Ctx {
   int batches = 8;     // sub-sweeps per sweep
   int hub_global = 0;  // 1: hubs with global-memory tables instead of LDS partitions
   int use_graph = 0;   // sweeps of a batch or a sub-module partition as one captured graph
+  int attempts_sequential = 0;   // 1: attempts one after another even where a batch of copies fits
   BucketCuts cuts = {{kBucketMaxDeg[0], kBucketMaxDeg[1], kBucketMaxDeg[2], kBucketMaxDeg[3], kBucketMaxDeg[4],
                       kBucketMaxDeg[5], kBucketMaxDeg[6], kBucketMaxDeg[7]}};
   bool initiated = false;

@@ -43,6 +43,7 @@ def test_instances_match_single_runs():
 
 
 def test_attempts_keep_the_best():
+    """Attempts one after another: the best of the same runs made one by one."""
     from relaxmap_amd import Network
     g = _graphs()[3]
     single = []
@@ -53,12 +54,42 @@ def test_attempts_keep_the_best():
             single.append(net.run(seed=s).computed_codelength)
     with Network() as net:
         g.upload_to(net)
+        net.set_attempt_mode(True)
         net.initiate()
         r = net.run(seed=7, attempts=3)
         best = int(np.argmin(single))
         assert r.best_attempt == best
         assert r.computed_codelength == single[best]
         assert abs(net.codelength - r.computed_codelength) <= 1e-9 * abs(r.computed_codelength)
+
+
+def test_attempts_as_one_batch():
+    """Attempts as one batch of copies of the network (the default where they
+    fit): the kept partition is the best copy's of the same batch (copies with
+    seeds seed, seed + 1, ..., partitioned with seed), and the context's
+    codelength is that copy's."""
+    from relaxmap_amd import Network
+    from relaxmap_amd.batch import union_of
+    g = _graphs()[3]
+    seeds = [7, 8, 9, 10]
+    _, node_off, iseeds, ptr, idx, w = union_of([g], seeds)
+    with Network() as bnet:
+        bnet.batch_upload(node_off, iseeds, ptr, idx, w)
+        bnet.initiate()
+        bnet.run(seed=7)
+        Ls = bnet.batch_results()[0]
+        bpart = bnet.partition()
+    best = int(np.argmin(Ls))
+    with Network() as net:
+        g.upload_to(net)
+        net.initiate()
+        r = net.run(seed=7, attempts=len(seeds))
+        part = net.partition()
+        assert abs(net.codelength - r.computed_codelength) <= 1e-9 * abs(r.computed_codelength)
+    print(f"[gpu] batched attempts: {[round(float(x), 6) for x in Ls]}, kept {r.best_attempt}", flush=True)
+    assert r.best_attempt == best
+    assert abs(r.computed_codelength - Ls[best]) <= 1e-9 * abs(Ls[best])
+    assert np.array_equal(part, bpart[best * g.n:(best + 1) * g.n] - best * g.n)
 
 
 # ---------------------------------------------------------------- one batch

@@ -15,22 +15,23 @@ pytestmark = pytest.mark.gpu
 EXE = os.path.join(ROOT, "relaxmap_amd", "relaxmap")
 
 
-@pytest.mark.parametrize("key,mode", [("lfr1k_mu03", "prior"), ("sbm1k", "normal"), ("flow_net", "prior")])
-def test_cli_matches_binding(tmp_path, key, mode):
+@pytest.mark.parametrize("key,mode,attempts", [("lfr1k_mu03", "prior", 1), ("sbm1k", "normal", 1),
+                                               ("flow_net", "prior", 1), ("lfr5k_mu05", "prior", 3)])
+def test_cli_matches_binding(tmp_path, key, mode, attempts):
     from relaxmap_amd import Network
     if not os.path.exists(EXE):
         pytest.fail("CLI not built")
     src = data_path(INPUTS[key])
-    r = subprocess.run([EXE, "7", src, "4", "1", "1e-3", "0", "10", str(tmp_path), mode], capture_output=True,
+    r = subprocess.run([EXE, "7", src, "4", str(attempts), "1e-3", "0", "10", str(tmp_path), mode], capture_output=True,
                        text=True, timeout=120)
-    print(f"[gpu] cli {key} {mode}: rc {r.returncode}", flush=True)
+    print(f"[gpu] cli {key} {mode} attempts {attempts}: rc {r.returncode}", flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Computed Code Length" in r.stdout
     name = os.path.splitext(os.path.basename(src))[0]
     with Network() as net:
         net.load(src)
         net.initiate()
-        net.run(threshold=1e-3, vthresh=0.0, max_iter=10, prior=mode == "prior", seed=7)
+        net.run(threshold=1e-3, vthresh=0.0, max_iter=10, prior=mode == "prior", seed=7, attempts=attempts)
         net.write_tree(tmp_path / "api.tree")
         net.write_clu(tmp_path / "api.clu")
     assert (tmp_path / f"{name}.tree").read_text() == (tmp_path / "api.tree").read_text()
