@@ -241,7 +241,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the hot path here is a single-GPU build "
-              "(see DESIGN.md for how it would shard); use --gpus 1", file=sys.stderr)
+              "(a row partition of the steady state and an all-gather of module labels per batch "
+              "would spread it over several); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from relaxmap_amd import device_count, synth
