@@ -74,7 +74,7 @@ int rmx_set_attempt_mode(rmx_ctx *ctx, int sequential);
  * (after load_pajek_format_network / load_linkList_format_network,
  * FileIO.cpp:32-265).  out_ptr[n+1] / out_idx[m] / w[m] is the out-CSR with
  * arcs sorted by (src, dst), no self-loops, no duplicates, w > 0; node_w[n]
- * are node (teleport) weights, NULL for 1.0.  m < 2^32 - 1. */
+ * are node (teleport) weights, NULL for 1.0.  m < 2^31 (larger m is refused). */
 int rmx_graph_upload(rmx_ctx *ctx, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx,
                      const double *w, const double *node_w);
 

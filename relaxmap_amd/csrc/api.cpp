@@ -68,7 +68,7 @@ void free_ctx_device(Ctx &c) {
   c.b_iters.free(); c.b_step.free(); c.b_outer.free(); c.b_inner.free();
   c.b_ebeg.free(); c.b_ecnt.free(); c.b_ecur.free(); c.b_count.free();
   c.b_cbeg.free(); c.b_cend.free(); c.b_ichunk.free(); c.b_cpart.free(); c.n_chunks = 0;
-  c.act.free(); c.act_next.free(); c.target.free(); c.newmod.free(); c.stamp.free(); c.movers.free(); c.movers_st.free();
+  c.act.free(); c.act_next.free(); c.wmark.free(); c.target.free(); c.newmod.free(); c.stamp.free(); c.movers.free(); c.movers_st.free();
   c.empty_list.free(); c.want_empty.free(); c.rank_tmp.free(); c.agg_buf.free();
   c.batch_keys.free(); c.batch_list.free();
   c.hub_keys.free(); c.hub_out.free(); c.hub_in.free(); c.hub_plan.free(); c.hub_bm.free(); c.hub_bd.free();
@@ -194,6 +194,7 @@ const char *rows_in_parallel(int32_t n, int64_t m, const int64_t *out_ptr, F f) 
 // the out-CSR rules of rmx_graph_upload; "" when they hold
 const char *check_csr(int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx, const double *w) {
   if (n <= 0 || m < 0 || !out_ptr || (m > 0 && (!out_idx || !w))) return "empty network or null arrays";
+  if (m > 0x7fffffffLL) return "more than 2^31 - 1 arcs (arc offsets on the device are 32-bit)";
   if (out_ptr[0] != 0 || out_ptr[n] != m) return "out_ptr does not span the arcs";
   for (int32_t i = 0; i < n; ++i)
     if (out_ptr[i + 1] < out_ptr[i]) return "out_ptr not monotone";

@@ -238,6 +238,7 @@ struct Ctx {
 
   // sweep work arrays (sized for level 0, reused by level 1)
   DevBuf<uint8_t> act, act_next;
+  DevBuf<uint8_t> wmark;             // units with a better module this sweep (neighbour marks by pull)
   DevBuf<uint32_t> target, newmod, stamp, movers, empty_list, want_empty, rank_tmp;
   DevBuf<long long> agg_buf;         // per mover: flows to/from its old and new module
   DevBuf<uint32_t> batch_keys, batch_list;   // this sweep's units by (bucket, batch)

@@ -232,6 +232,97 @@ __global__ void k_pb_layout(const uint32_t *__restrict__ sarc, const uint32_t *_
   bdst[j] = (uint16_t)(oidx[a] & ((1u << rshift) - 1u));
 }
 
+// Slots without a sort (bins that fit a workgroup's LDS counters).  The arcs
+// are cut into layout tiles; slots of bin b are handed out tile by tile:
+// tile t owns the range [base(t, b), base(t, b) + count(t, b)) with
+//   base(t, b) = bptr[b] + sum over t' < t of count(t', b),
+// and bptr[b] = the arcs whose head lies below bin b, read off the in-CSR.
+// Inside a tile the order of one bin's arcs follows LDS atomic order; the
+// pull's fixed-point sums do not depend on it.  Each tile's arcs of a bin
+// sit in one short run, so the push still writes a bin's lines from one
+// XCD in nearly sequential order, and slot[] is written in arc order.
+constexpr int kLayoutMaxBins = 16384;   // 64 KB of LDS counters
+constexpr int kLayoutBlock = 1024;
+constexpr int kLayoutChunk = 64;        // tiles per partial sum of the column scan
+
+__global__ void k_pb_bptr(const uint32_t *__restrict__ iptr, int32_t n, int32_t nbins, int rshift,
+                          uint32_t *__restrict__ bptr) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b <= nbins) bptr[b] = iptr[std::min<int64_t>(b << rshift, n)];
+}
+
+// count(t, b) into C[t * nbins + b]
+__global__ __launch_bounds__(kLayoutBlock) void k_pb_count(const uint32_t *__restrict__ oidx, int64_t m,
+                                                           int64_t tile_arcs, int32_t nbins, int rshift,
+                                                           uint32_t *__restrict__ C) {
+  __shared__ uint32_t cnt[kLayoutMaxBins];
+  for (int b = threadIdx.x; b < nbins; b += kLayoutBlock) cnt[b] = 0u;
+  __syncthreads();
+  const int64_t a0 = (int64_t)blockIdx.x * tile_arcs, a1 = std::min<int64_t>(m, a0 + tile_arcs);
+  for (int64_t k = a0 + threadIdx.x; k < a1; k += kLayoutBlock) atomicAdd(&cnt[oidx[k] >> rshift], 1u);
+  __syncthreads();
+  uint32_t *row = C + (int64_t)blockIdx.x * nbins;
+  for (int b = threadIdx.x; b < nbins; b += kLayoutBlock) row[b] = cnt[b];
+}
+
+// P[q * nbins + b] = sum of C[t][b] over the tiles t of chunk q
+__global__ void k_pb_colsum(const uint32_t *__restrict__ C, int64_t ntl, int32_t nbins, int64_t nq,
+                            uint32_t *__restrict__ P) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq * nbins) return;
+  const int64_t q = i / nbins, b = i - q * nbins;
+  const int64_t t1 = std::min<int64_t>(ntl, (q + 1) * kLayoutChunk);
+  uint32_t s = 0;
+  for (int64_t t = q * kLayoutChunk; t < t1; ++t) s += C[t * nbins + b];
+  P[i] = s;
+}
+
+// P[q][b] = bptr[b] + the arcs of bin b in the chunks before q
+__global__ void k_pb_colscan(const uint32_t *__restrict__ bptr, int32_t nbins, int64_t nq, uint32_t *__restrict__ P) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  uint32_t run = bptr[b];
+  for (int64_t q = 0; q < nq; ++q) {
+    const uint32_t v = P[q * nbins + b];
+    P[q * nbins + b] = run;
+    run += v;
+  }
+}
+
+// C[t][b] = base(t, b)
+__global__ void k_pb_colbase(const uint32_t *__restrict__ P, int64_t ntl, int32_t nbins, int64_t nq,
+                             uint32_t *__restrict__ C) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq * nbins) return;
+  const int64_t q = i / nbins, b = i - q * nbins;
+  const int64_t t1 = std::min<int64_t>(ntl, (q + 1) * kLayoutChunk);
+  uint32_t run = P[i];
+  for (int64_t t = q * kLayoutChunk; t < t1; ++t) {
+    const uint32_t v = C[t * nbins + b];
+    C[t * nbins + b] = run;
+    run += v;
+  }
+}
+
+// the slot of every arc of a tile, and the head of every slot relative to its bin
+__global__ __launch_bounds__(kLayoutBlock) void k_pb_place(const uint32_t *__restrict__ oidx, int64_t m,
+                                                           int64_t tile_arcs, int32_t nbins, int rshift,
+                                                           const uint32_t *__restrict__ C,
+                                                           uint32_t *__restrict__ slot, uint16_t *__restrict__ bdst) {
+  __shared__ uint32_t base[kLayoutMaxBins];
+  const uint32_t *row = C + (int64_t)blockIdx.x * nbins;
+  for (int b = threadIdx.x; b < nbins; b += kLayoutBlock) base[b] = row[b];
+  __syncthreads();
+  const uint32_t mask = (1u << rshift) - 1u;
+  const int64_t a0 = (int64_t)blockIdx.x * tile_arcs, a1 = std::min<int64_t>(m, a0 + tile_arcs);
+  for (int64_t k = a0 + threadIdx.x; k < a1; k += kLayoutBlock) {
+    const uint32_t d = oidx[k];
+    const uint32_t r = atomicAdd(&base[d >> rshift], 1u);
+    slot[k] = r;
+    bdst[r] = (uint16_t)(d & mask);
+  }
+}
+
 // Push: workgroups are spread so that each of the 8 XCDs (workgroup i runs
 // on XCD i mod 8) takes one contiguous eighth of the tiles; every bin then
 // receives each XCD's writes in one contiguous range.  The streamed slots
@@ -716,7 +807,9 @@ struct PushBins {
 
 // Rows per bin: a power of two from 256 to kPullRowsMax, about n / 2048, so
 // that a large network gives thousands of bins (workgroups of the pull).
-// The slots come from a stable sort of the out-arcs by bin.
+// The slots come from the per-tile counts above when the bins fit the LDS
+// counters and the level has its in-CSR, else from a stable sort of the
+// out-arcs by bin.
 void build_push_bins(Ctx &c, const Level &L, PushBins &pb) {
   hipStream_t s = c.stream;
   const int32_t n = L.n;
@@ -735,6 +828,32 @@ void build_push_bins(Ctx &c, const Level &L, PushBins &pb) {
   pb.trow.alloc(pb.ntiles + 1);
   k_pb_tile_rows<<<grid_for(pb.ntiles + 1, 256), 256, 0, s>>>(L.out_ptr.p, n, m, pb.ntiles, pb.trow.p);
   RMX_LAUNCH_CHECK();
+  if (pb.nbins <= kLayoutMaxBins && L.in_ptr.n > n) {
+    // at most 8192 layout tiles of at least 64k arcs
+    const int64_t tile_arcs = std::max<int64_t>(65536, (m + 8191) / 8192);
+    const int64_t ntl = (m + tile_arcs - 1) / tile_arcs;
+    const int64_t nq = (ntl + kLayoutChunk - 1) / kLayoutChunk;
+    k_pb_bptr<<<grid_for((int64_t)pb.nbins + 1, 256), 256, 0, s>>>(L.in_ptr.p, n, pb.nbins, rs, pb.bptr.p);
+    RMX_LAUNCH_CHECK();
+    DevBuf<uint32_t> C(c.cache), P(c.cache);
+    C.alloc(ntl * pb.nbins);
+    P.alloc(nq * pb.nbins);
+    k_pb_count<<<(unsigned)ntl, kLayoutBlock, 0, s>>>(L.out_idx.p, m, tile_arcs, pb.nbins, rs, C.p);
+    RMX_LAUNCH_CHECK();
+    k_pb_colsum<<<grid_for(nq * pb.nbins, 256), 256, 0, s>>>(C.p, ntl, pb.nbins, nq, P.p);
+    RMX_LAUNCH_CHECK();
+    k_pb_colscan<<<grid_for(pb.nbins, 256), 256, 0, s>>>(pb.bptr.p, pb.nbins, nq, P.p);
+    RMX_LAUNCH_CHECK();
+    k_pb_colbase<<<grid_for(nq * pb.nbins, 256), 256, 0, s>>>(P.p, ntl, pb.nbins, nq, C.p);
+    RMX_LAUNCH_CHECK();
+    pb.slot.alloc(m);
+    pb.bdst.alloc(m);
+    pb.val.alloc(m);
+    k_pb_place<<<(unsigned)ntl, kLayoutBlock, 0, s>>>(L.out_idx.p, m, tile_arcs, pb.nbins, rs, C.p, pb.slot.p,
+                                                      pb.bdst.p);
+    RMX_LAUNCH_CHECK();
+    return;   // C and P go back to the cache: later work on this stream runs after the layout
+  }
   DevBuf<uint32_t> key(c.cache), arc(c.cache);
   key.alloc(m);
   arc.alloc(m);

@@ -42,7 +42,7 @@ namespace {
 
 enum { kS = 0, kL = 3, kNM = 7 };
 enum { cMovers = 0, cWant = 1, cMoved = 2, cVisitsLo = 4, cCandsLo = 6, cUnits = 10, cBig = 11, cHubOvf = 12,
-       cErr = 13 };
+       cErr = 13, cPull = 14 };
 
 // Counters that every workgroup of a large launch updates (movers, empty
 // requests, moved nodes, edge visits, candidates, units) are kept in
@@ -1417,6 +1417,12 @@ __global__ void k_bempty_seg(int32_t nb, const uint32_t *__restrict__ off, const
 // The arcs of movers with more than big_deg arcs are left to k_commit_big,
 // which spreads them over many workgroups (all updates are exact integer
 // atomics, so the split does not change the result).
+// Activity marks (mark != 0): every mover's out- and in-neighbours become
+// active in the next sweep (Module.cpp:1661-1668), movers rejected for want
+// of an empty module included.  A batch with at least pull_min movers only
+// flags its movers in wmark and sets *pull_flag; k_pull_marks then marks
+// their neighbours at the end of the sweep from the receiving side, where a
+// unit stops at its first flagged neighbour.
 template <int G>
 __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ movers,
                                                 const uint32_t *__restrict__ counters, const uint32_t *__restrict__ optr,
@@ -1427,8 +1433,11 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
                                                 const long long *__restrict__ agg, const Stat4 *__restrict__ us,
                                                 const int32_t *__restrict__ nmem, Stat4 *ms, int32_t *mNodes,
                                                 int32_t *mUnits, uint8_t *act_next, int mark, uint32_t *stripes,
-                                                uint32_t big_deg, uint32_t *big_list, uint32_t *big_cnt) {
+                                                uint32_t big_deg, uint32_t *big_list, uint32_t *big_cnt,
+                                                uint32_t pull_min, uint8_t *wmark, uint32_t *pull_flag) {
   const uint32_t nm = counters[cMovers];
+  const bool pull = mark && nm >= pull_min;
+  if (pull && blockIdx.x == 0 && threadIdx.x == 0) *pull_flag = 1u;
   const int lane = threadIdx.x % G;
   const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x / G;
@@ -1437,9 +1446,10 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
     const uint32_t u = movers[i];
     const uint32_t ob = optr[u], oe = optr[u + 1];
     const bool big = (oe - ob) + (iptr[u + 1] - iptr[u]) > big_deg;
+    if (pull && lane == 0) wmark[u] = 1;
     if (big) {
       if (lane == 0) big_list[atomicAdd(big_cnt, 1u)] = u;
-    } else if (mark) {
+    } else if (mark && !pull) {
       for (uint32_t k = ob + lane; k < oe; k += G) act_next[oidx[k]] = 1;
       for (uint32_t k = iptr[u] + lane; k < iptr[u + 1]; k += G) act_next[iidx[k]] = 1;
     }
@@ -1486,6 +1496,8 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
 
 // Arc chunks of the deferred (high-degree) movers: prefix of chunk counts.
 constexpr int kCommitChunk = 4096;
+// smallest batch (units) whose marks may be pulled
+constexpr uint32_t kPullMinUnits = 65536;
 
 __global__ __launch_bounds__(256) void k_commit_plan(const uint32_t *__restrict__ big_list,
                                                      const uint32_t *__restrict__ big_cnt,
@@ -1510,8 +1522,9 @@ __global__ __launch_bounds__(256) void k_commit_plan(const uint32_t *__restrict_
   }
 }
 
-// one workgroup per arc chunk of a deferred mover: activity marks and the
-// co-mover corrections of its out-arcs
+// one workgroup per arc chunk of a deferred mover: activity marks (unless
+// the batch leaves them to k_pull_marks) and the co-mover corrections of its
+// out-arcs
 __global__ __launch_bounds__(256) void k_commit_big(const uint32_t *__restrict__ big_list,
                                                     const uint32_t *__restrict__ big_cnt,
                                                     const uint32_t *__restrict__ A, const uint32_t *__restrict__ optr,
@@ -1520,10 +1533,12 @@ __global__ __launch_bounds__(256) void k_commit_big(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ mod,
                                                     const uint32_t *__restrict__ newmod,
                                                     const uint32_t *__restrict__ stamp, uint32_t cur, Stat4 *ms,
-                                                    uint8_t *act_next, int mark) {
+                                                    uint8_t *act_next, int mark_in,
+                                                    const uint32_t *__restrict__ counters, uint32_t pull_min) {
   __shared__ long long lds[2][256 / kWave];
   const int cnt = (int)*big_cnt;
   if (cnt == 0) return;
+  const bool mark = mark_in && counters[cMovers] < pull_min;
   const uint32_t total = A[cnt];
   for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
     const int i = chunk_owner(A, cnt, c);
@@ -1567,6 +1582,56 @@ __global__ __launch_bounds__(256) void k_commit_big(const uint32_t *__restrict__
       if (sn) atomicAdd((unsigned long long *)&ms[n].x, (unsigned long long)sn);
     }
     __syncthreads();
+  }
+}
+
+template <int G>
+__device__ inline int group_or(int v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v |= __shfl_xor(v, off, G);
+  return v;
+}
+
+// whether flag[idx[k]] is set for some k in [b, e): G lanes, one arc per
+// lane in the first round and twice as many in each next one (up to 8), up to
+// the first round that finds one
+template <int G>
+__device__ inline bool any_flagged(const uint32_t *__restrict__ idx, uint32_t b, uint32_t e,
+                                   const uint8_t *__restrict__ flag, int lane) {
+  int per = 1;
+  for (uint32_t k0 = b; k0 < e; k0 += (uint32_t)(per * G), per = per < 8 ? 2 * per : 8) {
+    int h = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t k = k0 + (uint32_t)(j * G + lane);
+      if (j < per && k < e) h |= flag[idx[k]];
+    }
+    if (group_or<G>(h)) return true;
+  }
+  return false;
+}
+
+// End of a sweep in which some batch left its activity marks to the
+// receiving side (*pull_flag set by k_commit): a unit not yet active for the
+// next sweep becomes active when an out- or in-neighbour is flagged in
+// wmark.  With most units moving, a unit usually stops at its first
+// neighbour, where pushing marks writes one byte per arc of every mover.
+template <int G>
+__global__ __launch_bounds__(256) void k_pull_marks(const uint32_t *__restrict__ pull_flag, int32_t n,
+                                                    const uint32_t *__restrict__ optr,
+                                                    const uint32_t *__restrict__ oidx,
+                                                    const uint32_t *__restrict__ iptr,
+                                                    const uint32_t *__restrict__ iidx,
+                                                    const uint8_t *__restrict__ wmark, uint8_t *__restrict__ act_next) {
+  if (*pull_flag == 0u) return;
+  const int lane = threadIdx.x % G;
+  const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x / G;
+  for (int64_t u = g0; u < n; u += gstride) {
+    if (act_next[u]) continue;
+    const bool hit = any_flagged<G>(oidx, optr[u], optr[u + 1], wmark, lane) ||
+                     any_flagged<G>(iidx, iptr[u], iptr[u + 1], wmark, lane);
+    if (hit && lane == 0) act_next[u] = 1;
   }
 }
 
@@ -1847,6 +1912,7 @@ void ensure_sweep_buffers(Ctx &c) {
   if (c.target.n >= n) return;
   c.act.alloc(n);
   c.act_next.alloc(n);
+  c.wmark.alloc(n);
   c.target.alloc(n);
   c.newmod.alloc(n);
   c.stamp.alloc(n);
@@ -1862,6 +1928,7 @@ void ensure_sweep_buffers(Ctx &c) {
   c.batch_list.alloc(n);
   fill_u8(c.act.p, 1, n, c.stream);
   fill_u8(c.act_next.p, 0, n, c.stream);
+  fill_u8(c.wmark.p, 0, n, c.stream);
   fill_u32(c.stamp.p, 0u, n, c.stream);
   fill_u32(c.want_empty.p, 0u, n, c.stream);
   fill_u32(c.counters.p, 0u, kCounterWords, c.stream);
@@ -1871,6 +1938,7 @@ void reset_active(Ctx &c) {
   ensure_sweep_buffers(c);
   fill_u8(c.act.p, 1, c.N, c.stream);
   fill_u8(c.act_next.p, 0, c.N, c.stream);
+  fill_u8(c.wmark.p, 0, c.N, c.stream);
 }
 
 // The launches of one sweep captured into a HIP graph (when on): capture
@@ -1929,7 +1997,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   }
   fill_u32(c.counters.p + 2, 0u, 6, s);   // moved, visits, candidates
   fill_u32(c.counters.p + cUnits, 0u, 1, s);
-  fill_u32(c.counters.p + cErr, 0u, 1, s);
+  fill_u32(c.counters.p + cErr, 0u, 2, s);   // error, pull flag
   fill_u32(c.counters.p + kStripeBase, 0u, kStripes * kStripeWords, s);
 
   // hub hash-table pool
@@ -2054,6 +2122,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     }
   }
   SweepGraph graph(s, quiet);
+  bool maybe_pull = false;   // some batch may have left its marks to k_pull_marks
   for (int b = 0; b < c.batches; ++b) {
     const uint32_t *hub_list = nullptr;   // this batch's hubs, for the overflow fallback
     int32_t hub_n = 0;
@@ -2171,10 +2240,14 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       }
     }
     const uint32_t big_deg = n_hubs > 0 ? (uint32_t)c.cuts.v[kNumBuckets - 2] : 0xffffffffu;
+    // marks by pull when at least half of a large batch moves
+    const uint32_t pull_min = prioritized && units_b >= kPullMinUnits ? (units_b + 1) / 2 : 0xffffffffu;
+    if (pull_min != 0xffffffffu && (quiet || n_movers >= pull_min)) maybe_pull = true;
     k_commit<16><<<stride_grid((int64_t)n_movers * 16, 256), 256, 0, s>>>(
         c.movers.p, c.counters.p, L.out_ptr.p, L.out_idx.p, L.in_ptr.p, L.in_idx.p, aout, L.mod.p, c.newmod.p,
         c.stamp.p, c.stamp_now, c.agg_buf.p, L.us.p, L.nmem.p, c.ms.p, c.mNodes.p, c.mUnits.p, c.act_next.p,
-        prioritized ? 1 : 0, c.counters.p + kStripeBase, big_deg, c.big_list.p, c.counters.p + cBig);
+        prioritized ? 1 : 0, c.counters.p + kStripeBase, big_deg, c.big_list.p, c.counters.p + cBig, pull_min,
+        c.wmark.p, c.counters.p + cPull);
     RMX_LAUNCH_CHECK();
     if (n_hubs > 0) {
       // only hubs (more than big_deg arcs) are deferred: the list holds at most n_hubs
@@ -2182,7 +2255,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       RMX_LAUNCH_CHECK();
       k_commit_big<<<2048, 256, 0, s>>>(c.big_list.p, c.counters.p + cBig, c.big_plan.p, L.out_ptr.p, L.out_idx.p,
                                         L.in_ptr.p, L.in_idx.p, aout, L.mod.p, c.newmod.p, c.stamp.p, c.stamp_now,
-                                        c.ms.p, c.act_next.p, prioritized ? 1 : 0);
+                                        c.ms.p, c.act_next.p, prioritized ? 1 : 0, c.counters.p, pull_min);
       RMX_LAUNCH_CHECK();
     }
     if (single) {
@@ -2200,6 +2273,12 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   }
   graph.launch();
   if (prioritized) {
+    if (maybe_pull) {
+      k_pull_marks<8><<<stride_grid((int64_t)n * 8, 256), 256, 0, s>>>(
+          c.counters.p + cPull, n, L.out_ptr.p, L.out_idx.p, L.in_ptr.p, L.in_idx.p, c.wmark.p, c.act_next.p);
+      RMX_LAUNCH_CHECK();
+      fill_u8(c.wmark.p, 0, n, s);
+    }
     c.act.swap(c.act_next);
     fill_u8(c.act_next.p, 0, c.N, s);
   }
