@@ -217,6 +217,59 @@ def test_supernode_sweeps_consistent(key):
         assert abs(net.codelength - net.calculate_codelength()) <= 1e-9 * abs(net.codelength)
 
 
+def _activity_graph():
+    """60k SBM nodes with teleport weight 1 and 20k nodes with weight 1e-6
+    on a cycle of their own.  5k of the light nodes have an arc into the SBM
+    and 5k an arc from it; the other 10k see the SBM only through them."""
+    from relaxmap_amd import synth
+    g = synth.sbm(60000, 600, 10.0, 0.2, 3)
+    na, nb = 60000, 20000
+    src = [np.repeat(np.arange(na), np.diff(g.ptr)), na + np.arange(nb), na + np.arange(5000),
+           (np.arange(5000, 10000) * 13) % na]
+    dst = [g.idx.astype(np.int64), na + (np.arange(nb) + 1) % nb, (np.arange(5000) * 7) % na,
+           na + np.arange(5000, 10000)]
+    w = [g.w, np.ones(nb), np.ones(5000), np.ones(5000)]
+    src, dst, w = np.concatenate(src), np.concatenate(dst), np.concatenate(w)
+    order = np.lexsort((dst, src))
+    src, dst, w = src[order], dst[order], w[order]
+    n = na + nb
+    ptr = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(src, minlength=n), out=ptr[1:])
+    node_w = np.concatenate([np.ones(na), np.full(nb, 1e-6)])
+    return n, ptr, src, dst.astype(np.int32), w, node_w
+
+
+@pytest.mark.parametrize("batches", [1, 4])
+def test_activity_after_a_sweep(batches):
+    """A prioritized sweep makes every out- and in-neighbour of its movers
+    active for the next one (Module.cpp:1661-1668), so the next sweep visits
+    exactly those units' arcs.  With vthresh 1e-8 the SBM nodes move and the
+    light cycle does not (its gains are about 1e-6 times smaller), so part
+    of the cycle stays inactive, part is reached through out-arcs and part
+    through in-arcs.  One batch of 80k units with more than half moving
+    takes the receiving-side pass for its marks; four batches of 20k push
+    them from the movers.  From singletons every mover has a neighbour
+    module to go to, so the movers are the nodes whose module changed."""
+    from relaxmap_amd import Network
+    n, ptr, src, dst, w, node_w = _activity_graph()
+    deg = np.diff(ptr) + np.bincount(dst, minlength=n)
+    with Network(batches=batches) as net:
+        net.upload(n, ptr, dst, w, node_w)
+        net.initiate()
+        before = net.partition()
+        net.move(prioritized=True, vthresh=1e-8, seed=3)
+        mover = net.partition() != before
+        active = np.zeros(n, bool)
+        active[dst[mover[src]]] = True
+        active[src[mover[dst]]] = True
+        net.move(prioritized=True, vthresh=1e-8, seed=4)
+        print(f"[gpu] activity batches={batches}: movers {mover.sum()}, active {active.sum()} "
+              f"(light {active[60000:].sum()}), visits {net.last_edge_visits}", flush=True)
+        assert mover.sum() >= n // 2
+        assert 0 < active[60000:].sum() < 20000
+        assert net.last_edge_visits == int(deg[active].sum())
+
+
 def _hub_graph():
     """60k nodes, 753k arcs, 23 units with more than 2048 arcs (up to 4786):
     several partitions and arc chunks per hub."""
