@@ -41,7 +41,9 @@ void rmx_destroy(rmx_ctx *ctx);
 const char *rmx_last_error(const rmx_ctx *ctx);
 
 /* Precision of the steady state and of the sweeps; call before
- * rmx_initiate.  Sub-sweeps per sweep (default 8, at most 31). */
+ * rmx_initiate (a change after it requires rmx_initiate again: the calls
+ * that need flows fail with RMX_ESTATE until then).  Sub-sweeps per sweep
+ * (default 8, at most 31). */
 int rmx_set_precision(rmx_ctx *ctx, int precision);
 int rmx_set_batches(rmx_ctx *ctx, int batches);
 

@@ -131,6 +131,8 @@ const char *rmx_last_error(const rmx_ctx *ctx) { return ctx ? ctx->last_error.c_
 int rmx_set_precision(rmx_ctx *ctx, int precision) {
   return guarded(ctx, [&] {
     if (precision != RMX_FP64 && precision != RMX_MIXED) return fail(ctx, RMX_EINVAL, "precision must be 0 or 1");
+    // the arc weights each precision reads are made by rmx_initiate
+    if (precision != ctx->c.precision) ctx->c.initiated = false;
     ctx->c.precision = precision;
     return RMX_OK;
   });

@@ -51,6 +51,7 @@ __global__ void k_intra_ptr(const uint32_t *__restrict__ optr, int32_t n, int64_
   sptr[i] = (i < n && optr[i] < (uint64_t)m) ? apos[optr[i]] : total;
 }
 
+// the flagged arcs, compacted; a null weight destination is left out
 __global__ void k_intra_copy(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ apos, int64_t m,
                              const uint32_t *__restrict__ oidx, const double *__restrict__ w,
                              const float *__restrict__ wf, uint32_t *__restrict__ sidx, double *__restrict__ sw,
@@ -61,8 +62,8 @@ __global__ void k_intra_copy(const uint32_t *__restrict__ flag, const uint32_t *
     if (!flag[k]) continue;
     const uint32_t j = apos[k];
     sidx[j] = oidx[k];
-    sw[j] = w[k];
-    swf[j] = wf[k];
+    if (sw) sw[j] = w[k];
+    if (swf) swf[j] = wf[k];
   }
 }
 
@@ -308,7 +309,8 @@ int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t se
     RMX_LAUNCH_CHECK();
     if (m_in > 0) {
       k_intra_copy<<<(unsigned)std::min<int64_t>(16384, (m + 255) / 256), 256, 0, s>>>(
-          flag.p, apos.p, m, L0.in_idx.p, L0.in_w.p, L0.in_wf.p, S.in_idx.p, S.in_w.p, S.in_wf.p);
+          flag.p, apos.p, m, L0.in_idx.p, L0.in_w.p, L0.in_wf.p, S.in_idx.p,
+          c.precision == 1 ? nullptr : S.in_w.p, c.precision == 1 ? S.in_wf.p : nullptr);   // the weights the sweeps read
       RMX_LAUNCH_CHECK();
     }
     c.sync();

@@ -82,9 +82,10 @@ __global__ void k_normalize(const uint32_t *__restrict__ optr, int32_t n, const 
 }
 
 // The in-arcs, in in-CSR order, each from its tail: the normalised weight
-// raw / s (FP64 pull), or the flow p * (raw / s) and its FP32 copy.  The
-// operations are those of the out-arc, so both copies are identical.  The
-// tail's p and row sum are read as one 16-byte pair.
+// raw / s (FP64 pull), or the flow p * (raw / s) in FP64 (in_w) or, in mixed
+// precision, rounded to FP32 (in_wf): the sweeps of each precision read only
+// their own array.  The operations are those of the out-arc, so both copies
+// are identical.  The tail's p and row sum are read as one 16-byte pair.
 __global__ void k_in_norm(const uint32_t *__restrict__ iidx, const double *__restrict__ raw_in, int64_t m,
                           const double *__restrict__ rowsum, double *__restrict__ in_w) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x)
@@ -102,8 +103,8 @@ __global__ void k_in_flows(const uint32_t *__restrict__ iidx, const double *__re
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
     const double2 t = ps[iidx[j]];
     const double f = t.x * (raw_in[j] / t.y);
-    in_w[j] = f;
-    in_wf[j] = (float)f;
+    if (in_w) in_w[j] = f;
+    if (in_wf) in_wf[j] = (float)f;
   }
 }
 
@@ -1141,7 +1142,8 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
     ps.alloc(n);
     k_pack_ps<<<grid_for(n, 256), 256, 0, s>>>(c.x.p, rowsum.p, n, ps.p);
     RMX_LAUNCH_CHECK();
-    k_in_flows<<<stride_grid(m), 256, 0, s>>>(L.in_idx.p, c.raw_in.p, m, ps.p, L.in_w.p, L.in_wf.p);
+    k_in_flows<<<stride_grid(m), 256, 0, s>>>(L.in_idx.p, c.raw_in.p, m, ps.p, mixed ? nullptr : L.in_w.p,
+                                               mixed ? L.in_wf.p : nullptr);
     RMX_LAUNCH_CHECK();
     c.sync();   // ps goes back to the cache
   }

@@ -62,3 +62,27 @@ def test_batch_upload_rejects_cross_instance_arcs(n):
     with Network() as net:
         with pytest.raises(RuntimeError, match="joins two instances"):
             net.batch_upload(node_off, [1, 2], ptr, idx, w)
+
+
+def test_precision_change_needs_initiate():
+    """rmx_initiate makes the arc weights of the precision set at the time;
+    after a change of precision the sweeps refuse to run until it is called
+    again, and the re-initiated run is that of a context created with the
+    new precision."""
+    from relaxmap_amd import Network, synth
+    from relaxmap_amd.core import RMX_MIXED
+    g = synth.sbm(2000, 20, 8.0, 0.2, 5)
+    with Network(precision="fp64") as net:
+        g.upload_to(net)
+        net.initiate()
+        net._check(net._L.rmx_set_precision(net._h, RMX_MIXED))
+        with pytest.raises(RuntimeError, match="rmx_initiate first"):
+            net.run(seed=2)
+        net.initiate()
+        net.run(seed=2)
+        part, L = net.partition(), net.codelength
+    with Network(precision="mixed") as ref:
+        g.upload_to(ref)
+        ref.initiate()
+        ref.run(seed=2)
+        assert np.array_equal(part, ref.partition()) and L == ref.codelength
