@@ -639,7 +639,7 @@ __global__ void k_flows(const uint32_t *__restrict__ optr, int32_t n, double *__
     double f = pr * out_w[k];
     float ff = (float)f;
     out_w[k] = f;
-    out_wf[k] = ff;
+    if (mixed) out_wf[k] = ff;   // the FP64 sweeps read out_w only
     sf += f;
     sx += fx_of(mixed ? (double)ff : f);
   }
