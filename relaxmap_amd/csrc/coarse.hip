@@ -100,6 +100,31 @@ __global__ void k_singleton_state(int32_t n, Stat4 *__restrict__ ms, const Stat4
   stale[i] = 1;
 }
 
+// An instance whose sub-module partition ended above the codelength of its
+// singletons returns to singletons.  The reference partitions an instance
+// with sequential moves that each lower its codelength (move(),
+// OmpRelaxmap.cpp:450), so it never ends above them; relaxed sub-sweeps can.
+// Every module id of an instance is one of its own nodes (sub-module
+// partitions offer no empty modules), so resetting the rows of its nodes
+// resets all of its modules.
+__global__ void k_keep_singletons_if_worse(int32_t n, const uint32_t *__restrict__ par,
+                                           const double *__restrict__ L_single, const double *__restrict__ L,
+                                           Stat4 *__restrict__ ms, const Stat4 *__restrict__ us,
+                                           uint32_t *__restrict__ mod, int32_t *__restrict__ nmem,
+                                           int32_t *__restrict__ nodes, int32_t *__restrict__ units,
+                                           int32_t *__restrict__ stale) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t I = par[i];
+  if (!(L[I] > L_single[I])) return;
+  ms[i] = us[i];
+  mod[i] = (uint32_t)i;
+  nmem[i] = 1;
+  nodes[i] = 1;
+  units[i] = 1;
+  stale[i] = 1;
+}
+
 __global__ void k_heads(const uint32_t *__restrict__ key, int32_t n, uint32_t *__restrict__ flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1u : 0u;
@@ -377,8 +402,11 @@ int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t se
   swap_in_sub(c);
   c.sub_mode = 1;
   const double thr_nats = threshold * std::log(2.0);
+  DevBuf<double> L_single(c.cache);   // each instance's codelength as singletons
+  L_single.alloc(n);
   try {
     instance_totals_async(c);
+    RMX_HIP(hipMemcpyAsync(L_single.p, c.inst_L.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     k_node_act<<<grid_for(n, 256), 256, 0, s>>>(n, c.sub_par.p, c.inst_act.p, c.act.p);
     RMX_LAUNCH_CHECK();
     fill_u8(c.act_next.p, 0, n, s);
@@ -410,6 +438,10 @@ int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t se
   c.sub_mode = 0;
   swap_in_sub(c);
   reset_active(c);
+  k_keep_singletons_if_worse<<<grid_for(n, 256), 256, 0, s>>>(n, c.sub_par.p, L_single.p, c.inst_L.p, c.sub_ms.p,
+                                                              S.us.p, S.mod.p, S.nmem.p, c.sub_nodes.p,
+                                                              c.sub_units.p, c.sub_stale.p);
+  RMX_LAUNCH_CHECK();
 
   // number of sub-modules
   DevBuf<uint32_t> cnt(c.cache);
