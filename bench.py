@@ -231,7 +231,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nodes", type=int, default=DEFAULT_NODES)
     ap.add_argument("--mean-deg", type=float, default=DEFAULT_MEAN_DEG)
-    ap.add_argument("--batches", type=int, default=8)
+    ap.add_argument("--batches", type=int, default=0,
+                    help="sub-sweeps per sweep; 0: one per 32768 units of a level, 4 to 8")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -275,7 +276,7 @@ def main():
                       "community_exponent": 1.2, "community_range": [50, 50000], "mu": 0.4,
                       "weights": "fp32 lognormal(0,1)", "dangling": 0.01, "seed": 5,
                       "threshold": 1e-3, "vthresh": 0.0, "max_iter": 10, "mode": "prior",
-                      "batches": args.batches}}
+                      "batches": args.batches or "per level (8 at every level of this network)"}}
     mixed = {"time_to_converged_s": step_ms / 1e3, "move_sweep_gteps": gteps,
              "codelength_bits": rec_m[-1]["codelength"] / math.log(2), "modules": rec_m[-1]["n_module"],
              "flow_iterations": rec_m[-1]["iterations"], "sweeps": rec_m[-1]["sweeps"],

@@ -50,7 +50,7 @@ def union_of(graphs, seeds):
     return pairs, node_off, iseeds, ptr, idx, w
 
 
-def run_batched(graphs, seeds, precision="mixed", batches=8, threshold=1e-3, vthresh=0.0, max_iter=10, prior=True,
+def run_batched(graphs, seeds, precision="mixed", batches=0, threshold=1e-3, vthresh=0.0, max_iter=10, prior=True,
                 return_net=False):
     """Partition every graph with every seed as one batch of instances.
 
@@ -77,7 +77,7 @@ def run_batched(graphs, seeds, precision="mixed", batches=8, threshold=1e-3, vth
     return results, dt
 
 
-def run_instances(graphs, seeds, workers=8, precision="mixed", batches=8, threshold=1e-3, vthresh=0.0,
+def run_instances(graphs, seeds, workers=8, precision="mixed", batches=0, threshold=1e-3, vthresh=0.0,
                   max_iter=10, prior=True):
     """Partition every graph (objects with ``upload_to(net)`` and ``n``) with
     every seed.

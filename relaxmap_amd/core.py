@@ -182,7 +182,7 @@ class NetworkFile:
 class Network:
     """One network resident on one GPU (the reference's ``Network``)."""
 
-    def __init__(self, device=0, precision="fp64", batches=8):
+    def __init__(self, device=0, precision="fp64", batches=0):
         L = load_library()
         self._L = L
         h = C.c_void_p()

@@ -140,8 +140,8 @@ int rmx_set_precision(rmx_ctx *ctx, int precision) {
 
 int rmx_set_batches(rmx_ctx *ctx, int batches) {
   return guarded(ctx, [&] {
-    if (batches < 1 || batches > rmx::kMaxBatches)
-      return fail(ctx, RMX_EINVAL, "batches must be in 1.." + std::to_string(rmx::kMaxBatches));
+    if (batches < 0 || batches > rmx::kMaxBatches)
+      return fail(ctx, RMX_EINVAL, "batches must be in 0.." + std::to_string(rmx::kMaxBatches) + " (0: per level)");
     ctx->c.batches = batches;
     return RMX_OK;
   });

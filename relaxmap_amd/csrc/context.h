@@ -1,6 +1,7 @@
 // Device-resident state of one network (one HIP stream, one GPU).
 #pragma once
 
+#include <algorithm>
 #include <string>
 #include <utility>
 #include <vector>
@@ -191,7 +192,7 @@ struct Ctx {
   int32_t N = 0;
   int64_t E = 0;
   int precision = 0;   // 0 FP64, 1 mixed (FP32 bulk, FP64 accumulation)
-  int batches = 8;     // sub-sweeps per sweep
+  int batches = 0;     // sub-sweeps per sweep; 0: chosen per level (sweep_batches)
   int hub_global = 0;  // 1: hubs with global-memory tables instead of LDS partitions
   int use_graph = 0;   // sweeps of a batch or a sub-module partition as one captured graph
   int attempts_sequential = 0;   // 1: attempts one after another even where a batch of copies fits
@@ -331,6 +332,19 @@ void ensure_sweep_buffers(Ctx &c);
 int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int64_t *edge_visits);
 void reset_active(Ctx &c);
 int64_t n_active(Ctx &c);
+// Sub-sweeps of one sweep of level L: the count set on the context, or, when
+// it is 0, one per 32768 units of the level, at least 4 and at most 8.  Each
+// sub-sweep costs about 20 dependent launches and a host wait (about
+// 0.15 ms), which bounds the sweeps of levels up to a few hundred thousand
+// units; larger levels keep 8, where fewer sub-sweeps raise the codelength.
+// Sub-module partitions keep 8: their instances are single modules of a few
+// units, which in fewer sub-sweeps move together against stale statistics
+// and can end above their singletons' codelength.
+inline int sweep_batches(const Ctx &c, const Level &L) {
+  if (c.batches > 0) return c.batches;
+  if (c.sub_mode) return 8;
+  return (int)std::min<int64_t>(8, std::max<int64_t>(4, (int64_t)L.n >> 15));
+}
 
 // ----- coarse.hip
 // generate_sub_modules (OmpRelaxmap.cpp:500-632): every module's induced

@@ -1821,7 +1821,7 @@ EvalArgs sweep_args(Ctx &c, Level &L, bool prioritized, double vthresh, uint64_t
   a.act = c.act.p;
   a.all_active = prioritized ? 0 : 1;
   a.seed = sweep_seed;
-  a.batches = c.batches;
+  a.batches = sweep_batches(c, L);
   a.node_level = c.level == 0 ? 1 : 0;
   a.empty_key = (uint32_t)c.N + 1u;
   a.scal = c.scal.p;
@@ -2042,7 +2042,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
 
   // this sweep's units per (bucket, batch), inactive units last: one stable
   // 8-bit radix pass over the bucket positions
-  const int B = c.batches;
+  const int B = sweep_batches(c, L);
   const int NK = kNumBuckets * (B + 1);
   std::vector<uint32_t> seg(NK + 1, 0);
   {
@@ -2123,7 +2123,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   }
   SweepGraph graph(s, quiet);
   bool maybe_pull = false;   // some batch may have left its marks to k_pull_marks
-  for (int b = 0; b < c.batches; ++b) {
+  for (int b = 0; b < B; ++b) {
     const uint32_t *hub_list = nullptr;   // this batch's hubs, for the overflow fallback
     int32_t hub_n = 0;
     a.batch = b;

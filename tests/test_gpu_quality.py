@@ -50,18 +50,20 @@ def test_nmi_closed_forms():
     assert abs(nmi([0, 1, 0, 1], [0, 0, 1, 1])) < 1e-12
 
 
+@pytest.mark.parametrize("batches", [0, 8])
 @pytest.mark.parametrize("key", ["lfr1k_mu03", "lfr5k_mu05", "sbm1k"])
 @pytest.mark.parametrize("mode", ["prior", "normal"])
-def test_codelength_and_nmi(key, mode):
+def test_codelength_and_nmi(key, mode, batches):
+    """batches=0 is the default, per-level sub-sweep count (4 at these sizes)."""
     from relaxmap_amd import Network
     g = golden(f"{key}.run_{mode}_s1")
     ref_L = g["calculated"]
-    with Network() as net:
+    with Network(batches=batches) as net:
         net.load(data_path(INPUTS[key]))
         net.initiate()
         r = net.run(threshold=1e-3, vthresh=0.0, max_iter=10, prior=(mode == "prior"), seed=1)
         L = r.computed_codelength
-        print(f"[gpu] {key} {mode}: L={L / math.log(2):.5f} ref={ref_L / math.log(2):.5f} bits, "
+        print(f"[gpu] {key} {mode} batches={batches}: L={L / math.log(2):.5f} ref={ref_L / math.log(2):.5f} bits, "
               f"modules={r.n_module}, steps={r.super_steps}", flush=True)
         assert abs(r.codelength - L) <= 1e-9 * L
         assert L <= ref_L * 1.005
@@ -118,3 +120,35 @@ def test_rerun_on_same_network_repeats():
             r = net.run(seed=3)
             out.append((net.partition(), r.codelength))
     assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_per_level_batches_small_network(prec):
+    """Per-level sub-sweeps (batches=0) give 4 sub-sweeps at every level of a
+    network of fewer than 163840 nodes: the same run as batches=4, bit for bit."""
+    from relaxmap_amd import Network
+    out = []
+    for batches in (0, 4):
+        with Network(precision=prec, batches=batches) as net:
+            net.load(data_path(INPUTS["lfr5k_mu05"]))
+            net.initiate()
+            r = net.run(seed=4)
+            out.append((net.partition(), r.codelength, r.sweeps))
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1] and out[0][2] == out[1][2]
+
+
+def test_per_level_batches_quality_100k_nodes():
+    """On the 100k-node LFR-style network (4 sub-sweeps at every level) the
+    per-level count reaches the codelength and NMI of 8 sub-sweeps per sweep."""
+    from relaxmap_amd import Network, synth
+    g = synth.config2()
+    res = {}
+    for batches in (0, 8):
+        with Network(precision="mixed", batches=batches) as net:
+            g.upload_to(net)
+            net.initiate()
+            r = net.run(threshold=1e-3, vthresh=0.0, max_iter=10, prior=True, seed=1)
+            res[batches] = (r.computed_codelength, nmi(net.partition(), g.community), r.sweeps)
+    print(f"[gpu] 100k per-level batches: {res}", flush=True)
+    assert res[0][0] <= res[8][0] * 1.001
+    assert res[0][1] >= res[8][1] - 0.005
