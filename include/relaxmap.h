@@ -44,7 +44,8 @@ const char *rmx_last_error(const rmx_ctx *ctx);
  * rmx_initiate (a change after it requires rmx_initiate again: the calls
  * that need flows fail with RMX_ESTATE until then).  Sub-sweeps per sweep:
  * 1..31, or 0 (the default) for one per 32768 units of the level being
- * swept, at least 4 and at most 8. */
+ * swept (of its average instance in a batch of instances), at least 4 and
+ * at most 8. */
 int rmx_set_precision(rmx_ctx *ctx, int precision);
 int rmx_set_batches(rmx_ctx *ctx, int batches);
 

@@ -337,13 +337,16 @@ int64_t n_active(Ctx &c);
 // sub-sweep costs about 20 dependent launches and a host wait (about
 // 0.15 ms), which bounds the sweeps of levels up to a few hundred thousand
 // units; larger levels keep 8, where fewer sub-sweeps raise the codelength.
+// A batch of instances counts the units of its average instance, so that
+// attempts run as copies of one network sweep as that network alone does.
 // Sub-module partitions keep 8: their instances are single modules of a few
 // units, which in fewer sub-sweeps move together against stale statistics
 // and can end above their singletons' codelength.
 inline int sweep_batches(const Ctx &c, const Level &L) {
   if (c.batches > 0) return c.batches;
   if (c.sub_mode) return 8;
-  return (int)std::min<int64_t>(8, std::max<int64_t>(4, (int64_t)L.n >> 15));
+  const int64_t units = c.nb > 0 ? (int64_t)L.n / c.nb : (int64_t)L.n;
+  return (int)std::min<int64_t>(8, std::max<int64_t>(4, units >> 15));
 }
 
 // ----- coarse.hip
