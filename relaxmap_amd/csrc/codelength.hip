@@ -90,6 +90,36 @@ __global__ __launch_bounds__(kRedBlock) void k_sum_final(const double *__restric
   if (threadIdx.x == 0) scal[kS] = s;
 }
 
+// sum of phi(p) in the launch shape and order of the initial calibrate's
+// (k_calib in flow.hip: a grid-stride sum per thread, then the block and
+// partial trees), so the value equals the one the context keeps
+__global__ __launch_bounds__(kRedBlock) void k_plogp_partials(const double *__restrict__ p, int32_t n,
+                                                              double *__restrict__ part) {
+  __shared__ double lds[kRedBlock / kWave];
+  double sp = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n; i += (int64_t)kRedGrid * kRedBlock)
+    sp += plogp(p[i]);
+  sp = block_sum<double, kRedBlock>(sp, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = sp;
+}
+
+// k_totals_final with sum phi(p) from device partials
+__global__ __launch_bounds__(kRedBlock) void k_totals_final_dev(const double *__restrict__ part,
+                                                                const double *__restrict__ sp_part,
+                                                                double *__restrict__ scal) {
+  __shared__ double lds[kRedBlock / kWave];
+  double s = reduce_partials(part, lds);
+  double se = reduce_partials(part + kRedGrid, lds);
+  double ss = reduce_partials(part + 2 * kRedGrid, lds);
+  double cnt = reduce_partials(part + 3 * kRedGrid, lds);
+  double sp = reduce_partials(sp_part, lds);
+  if (threadIdx.x == 0) {
+    scal[kS] = s;
+    scal[kL] = plogp(s) - 2.0 * se + ss - sp;
+    scal[kNM] = cnt;
+  }
+}
+
 __global__ __launch_bounds__(kRedBlock) void k_totals_final(const double *__restrict__ part, double all_log_all,
                                                             double *__restrict__ scal) {
   __shared__ double lds[kRedBlock / kWave];
@@ -407,6 +437,37 @@ void set_partition(Ctx &c, const uint32_t *d_part) {
   module_totals(c, &c.sum_exit, &c.codelength);
 }
 
+void p5_launch(int32_t n, const uint32_t *optr, const uint32_t *oidx, const double *ow, const uint32_t *mod,
+               const double *p, const double *tau, const uint8_t *dang, double all_log_all, bool device_log_all,
+               const P5Work &w, Scratch &scratch, hipStream_t s) {
+  RMX_HIP(hipMemcpyAsync(w.key, mod, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+  iota_u32(w.node, n, s);
+  int bits = 1;
+  while (bits < 32 && ((int64_t)1 << bits) < (int64_t)n) ++bits;
+  radix_sort_pairs_u32(w.key, w.node, n, 0, bits, scratch, s);
+  unsigned g = grid_for(n, 256);
+  k_seg_heads<<<g, 256, 0, s>>>(w.key, n, w.flag);
+  RMX_LAUNCH_CHECK();
+  exclusive_scan_u32(w.flag, w.pos, n, w.total, scratch, s);
+  k_seg_starts<<<g, 256, 0, s>>>(w.flag, w.pos, n, w.start);
+  RMX_LAUNCH_CHECK();
+  k_p5_ext<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(n, optr, oidx, ow, mod, w.ext);
+  RMX_LAUNCH_CHECK();
+  k_p5_modules<<<grid_for((int64_t)n * kWave, 256), 256, 0, s>>>(w.start, w.total, n, w.node, p, tau, dang, w.ext,
+                                                                 w.segq, w.segqp);
+  RMX_LAUNCH_CHECK();
+  k_p5_partials<<<kRedGrid, kRedBlock, 0, s>>>(w.segq, w.segqp, w.total, w.partial);
+  RMX_LAUNCH_CHECK();
+  if (device_log_all) {
+    k_plogp_partials<<<kRedGrid, kRedBlock, 0, s>>>(p, n, w.partial + 4 * kRedGrid);
+    RMX_LAUNCH_CHECK();
+    k_totals_final_dev<<<1, kRedBlock, 0, s>>>(w.partial, w.partial + 4 * kRedGrid, w.scal);
+  } else {
+    k_totals_final<<<1, kRedBlock, 0, s>>>(w.partial, all_log_all, w.scal);
+  }
+  RMX_LAUNCH_CHECK();
+}
+
 double calculate_codelength(Ctx &c) {
   Level &L = c.L0;
   hipStream_t s = c.stream;
@@ -422,31 +483,15 @@ double calculate_codelength(Ctx &c) {
   total.alloc(1);
   segq.alloc(n);
   segqp.alloc(n);
-  RMX_HIP(hipMemcpyAsync(key.p, L.mod.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
-  iota_u32(node.p, n, s);
-  int bits = 1;
-  while (bits < 32 && ((int64_t)1 << bits) < (int64_t)n) ++bits;
-  radix_sort_pairs_u32(key.p, node.p, n, 0, bits, c.scratch, s);
-  unsigned g = grid_for(n, 256);
-  k_seg_heads<<<g, 256, 0, s>>>(key.p, n, flag.p);
-  RMX_LAUNCH_CHECK();
-  exclusive_scan_u32(flag.p, pos.p, n, total.p, c.scratch, s);
-  k_seg_starts<<<g, 256, 0, s>>>(flag.p, pos.p, n, start.p);
-  RMX_LAUNCH_CHECK();
-  k_p5_ext<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(n, L.out_ptr.p, L.out_idx.p, L.out_w.p, L.mod.p,
-                                                               ext.p);
-  RMX_LAUNCH_CHECK();
-  k_p5_modules<<<grid_for((int64_t)n * kWave, 256), 256, 0, s>>>(start.p, total.p, n, node.p, L.p.p, L.tau.p,
-                                                                 c.dangling.p, ext.p, segq.p, segqp.p);
-  RMX_LAUNCH_CHECK();
-  k_p5_partials<<<kRedGrid, kRedBlock, 0, s>>>(segq.p, segqp.p, total.p, c.partial.p);
-  RMX_LAUNCH_CHECK();
-  k_totals_final<<<1, kRedBlock, 0, s>>>(c.partial.p, c.all_log_all, c.scal.p + 8);
-  RMX_LAUNCH_CHECK();
+  P5Work w{key.p, node.p, flag.p, pos.p, start.p, total.p, ext.p, segq.p, segqp.p, c.partial.p, c.scal.p + 8};
+  p5_launch(n, L.out_ptr.p, L.out_idx.p, L.out_w.p, L.mod.p, L.p.p, L.tau.p, c.dangling.p, c.all_log_all, false, w,
+            c.scratch, s);
   double h[8];
   RMX_HIP(hipMemcpyAsync(h, c.scal.p + 8, sizeof(h), hipMemcpyDeviceToHost, s));
   c.sync();
   return h[kL];
 }
+
+size_t p5_partial_doubles() { return 5 * (size_t)kRedGrid; }
 
 }  // namespace rmx

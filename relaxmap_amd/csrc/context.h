@@ -326,6 +326,19 @@ void sum_exit_async(Ctx &c);
 void set_partition(Ctx &c, const uint32_t *d_part);
 double calculate_codelength(Ctx &c);
 void rebuild_exit_flows(Ctx &c);
+// Network::calculateCodeLength on raw device arrays (the node level): the
+// temporaries of P5Work hold n elements each, `partial` p5_partial_doubles()
+// and `scal` 8; the result is left in scal[3] (L, nats), scal[0] (S) and
+// scal[7] (non-empty modules).  Sum phi(p) is all_log_all, or, with
+// device_log_all, summed from p on the device in the initial calibrate's order.
+struct P5Work {
+  uint32_t *key, *node, *flag, *pos, *start, *total;
+  double *ext, *segq, *segqp, *partial, *scal;
+};
+size_t p5_partial_doubles();
+void p5_launch(int32_t n, const uint32_t *optr, const uint32_t *oidx, const double *ow, const uint32_t *mod,
+               const double *p, const double *tau, const uint8_t *dang, double all_log_all, bool device_log_all,
+               const P5Work &w, Scratch &scratch, hipStream_t s);
 
 // ----- move.hip
 void ensure_sweep_buffers(Ctx &c);

@@ -6,13 +6,21 @@
 namespace rmx {
 
 // Scratch arena owned by the caller; grows on demand (hipMallocAsync-free:
-// the context allocates once and reuses).
+// the context allocates once and reuses).  A fixed arena wraps memory it does
+// not own (a region of a static device array): it never allocates or frees,
+// and a request beyond its size throws.
 struct Scratch {
   void *ptr = nullptr;
   size_t bytes = 0;
+  bool fixed = false;
   void *get(size_t need);
   void release();
 };
+
+// Bytes of scratch that a radix sort of n (K, V) pairs on `bits` key bits
+// takes (the largest need of radix_sort_pairs_*), and an exclusive scan of n.
+size_t radix_sort_scratch_bytes(int64_t n, int key_bytes, int val_bytes, int bits);
+size_t exclusive_scan_scratch_bytes(int64_t n);
 
 // out[i] = sum_{j<i} in[i]; *d_total (device, may be null) = sum of all.
 // In-place (out == in) is allowed.  n < 2^32.

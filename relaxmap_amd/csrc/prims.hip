@@ -2,10 +2,15 @@
 //
 // Scan: reduce-then-scan over tiles of 4096 elements (256 threads x 16),
 // recursing on the tile sums; no inter-block spinning.
-// Radix sort: 8-bit digits; per pass an upsweep (per-tile digit histogram,
-// digit-major so that one exclusive scan yields every tile's scatter base)
-// and a downsweep that ranks each tile's elements stably with wave ballots
-// (equal-digit peers within a wave) and wave-private digit counts.
+// Radix sort: stable LSD over 8-bit digits.  One kernel (k_sort_hist) reads
+// the keys once and counts the digits of every pass; a 256-lane scan per pass
+// (k_sort_digit_scan) turns them into each digit's first output position.
+// Each pass is then one kernel (k_sort_onesweep): tiles are numbered in the
+// order their workgroups start (atomic ticket), each wave ranks its share of
+// the tile with ballots and a wave-private digit count, and every tile finds
+// its digits' offsets by a decoupled look-back over its predecessors'
+// published counts (bounded waits that raise an error flag instead of
+// hanging), then writes coalesced digit runs from LDS.
 #include "prims.h"
 #include "context.h"
 
@@ -13,6 +18,7 @@ namespace rmx {
 
 void *Scratch::get(size_t need) {
   if (need > bytes) {
+    if (fixed) throw HipError("fixed scratch arena too small");
     if (ptr) RMX_HIP(hipFree(ptr));
     size_t b = need + need / 4 + 4096;
     RMX_HIP(hipMalloc(&ptr, b));
@@ -47,6 +53,7 @@ void Profiler::end(hipStream_t s, const std::string &name, double bytes) {
 }
 
 void Scratch::release() {
+  if (fixed) return;
   if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
   bytes = 0;
@@ -384,6 +391,26 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_onesweep(const K *__restric
 // inputs or the scratch) and the pointers say where; otherwise it is copied
 // back into keys/vals.  Waits for the stream at the end, to report a failed
 // look-back (which leaves the output incomplete) as an exception.
+// Scratch layout of one sort: temporary keys and values, look-back words,
+// digit counts, tile counters and the error flag.
+struct SortLayout {
+  size_t off_v, off_l, off_c, total;
+  int64_t n_tiles, n_small;
+};
+
+template <typename K, typename V>
+SortLayout sort_layout(int64_t n, int passes) {
+  auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  SortLayout l;
+  l.n_tiles = (n + SortShape<K, V>::tile - 1) / SortShape<K, V>::tile;
+  l.n_small = (int64_t)passes * kRadix + passes + 1;
+  l.off_v = align(sizeof(K) * n);
+  l.off_l = l.off_v + align(sizeof(V) * n);
+  l.off_c = l.off_l + align(sizeof(uint64_t) * l.n_tiles * kRadix);
+  l.total = l.off_c + align(sizeof(uint32_t) * l.n_small);
+  return l;
+}
+
 template <typename K, typename V>
 void radix_sort_impl(K *keys, V *vals, int64_t n, int begin_bit, int end_bit, Scratch &scratch, hipStream_t s,
                      K **out_k = nullptr, V **out_v = nullptr) {
@@ -393,15 +420,11 @@ void radix_sort_impl(K *keys, V *vals, int64_t n, int begin_bit, int end_bit, Sc
   if (n > (int64_t)0xffffffffLL) throw HipError("radix sort: n >= 2^32");
   const int passes = (end_bit - begin_bit + 7) / 8;
   if (passes > kMaxPasses) throw HipError("radix sort: more than 64 key bits");
-  const int64_t n_tiles = (n + SortShape<K, V>::tile - 1) / SortShape<K, V>::tile;
-  auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const int64_t n_small = (int64_t)passes * kRadix + passes + 1;   // digit counts, tile counters, error flag
-  size_t off_k = 0;
-  size_t off_v = off_k + align(sizeof(K) * n);
-  size_t off_l = off_v + align(sizeof(V) * n);
-  size_t off_c = off_l + align(sizeof(uint64_t) * n_tiles * kRadix);
-  size_t total = off_c + align(sizeof(uint32_t) * n_small);
-  char *base = (char *)scratch.get(total);
+  const SortLayout lay = sort_layout<K, V>(n, passes);
+  const int64_t n_tiles = lay.n_tiles;
+  const int64_t n_small = lay.n_small;   // digit counts, tile counters, error flag
+  const size_t off_k = 0, off_v = lay.off_v, off_l = lay.off_l, off_c = lay.off_c;
+  char *base = (char *)scratch.get(lay.total);
   K *tk = (K *)(base + off_k);
   V *tv = (V *)(base + off_v);
   uint64_t *look = (uint64_t *)(base + off_l);
@@ -465,6 +488,17 @@ void exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *
 }
 
 void exclusive_scan_reserve(int64_t n, Scratch &scratch) { scratch.get(sizeof(uint32_t) * scan_tmp_elems(n)); }
+
+size_t exclusive_scan_scratch_bytes(int64_t n) { return sizeof(uint32_t) * scan_tmp_elems(n); }
+
+size_t radix_sort_scratch_bytes(int64_t n, int key_bytes, int val_bytes, int bits) {
+  if (n <= 1 || bits <= 0) return 0;
+  const int passes = (bits + 7) / 8;
+  if (key_bytes == 4 && val_bytes == 4) return sort_layout<uint32_t, uint32_t>(n, passes).total;
+  if (key_bytes == 8 && val_bytes == 4) return sort_layout<uint64_t, uint32_t>(n, passes).total;
+  if (key_bytes == 8 && val_bytes == 8) return sort_layout<uint64_t, uint64_t>(n, passes).total;
+  throw HipError("radix sort: no such key / value width");
+}
 
 void radix_sort_pairs_u32(uint32_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit,
                           Scratch &scratch, hipStream_t s) {
