@@ -249,6 +249,30 @@ int rmx_write_clu(rmx_ctx *ctx, const char *path);
 int rmx_set_profiling(rmx_ctx *ctx, int on);
 int rmx_kernel_stats(rmx_ctx *ctx, int i, char *name, int name_len, int64_t *launches, double *ms, double *bytes);
 
+/* No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It computes what rmx_calculate_codelength
+ * does (Network::calculateCodeLength, Module.cpp:2966-3002) for a network and
+ * partition held in the caller's device memory, on the caller's stream
+ * (hipStream_t, NULL for the default stream) of GPU `device`:
+ *   d_out_ptr[n+1], d_out_idx[m]  out-CSR (device), heads in [0, n)
+ *   d_flow[m]                     arc flows f_ab = p_a * w_ab / sum_b w_ab
+ *   d_p[n], d_tau[n]              steady-state flows and teleport weights
+ *   d_dangling[n]                 1 for a node without out-arcs, else 0
+ *   d_module[n]                   module index of every node, in [0, n)
+ * *codelength (host) receives the codelength in nats.  Every temporary lives
+ * in a static device array of the library (one per GPU, held until the
+ * process ends), sized for n <= RMX_NOALLOC_MAX_NODES; larger n is refused
+ * with RMX_EINVAL.  The call allocates nothing, checks the CSR and the module
+ * indices on the device first (RMX_EINVAL on a violation), and returns after
+ * its stream has finished.  Calls that overlap on one GPU share that array
+ * and race.  On failure a message goes to err[err_len] (may be NULL). */
+#define RMX_NOALLOC_MAX_NODES (1 << 20)
+int rmx_calculate_codelength_noalloc(int device, void *stream, int32_t n, int64_t m, const uint32_t *d_out_ptr,
+                                     const uint32_t *d_out_idx, const double *d_flow, const double *d_p,
+                                     const double *d_tau, const uint8_t *d_dangling, const int32_t *d_module,
+                                     double *codelength, char *err, int err_len);
+
 /* Kernel timings of the last rmx_initiate / rmx_partition (milliseconds). */
 int rmx_timing(rmx_ctx *ctx, double *flow_ms, double *sweep_ms, double *contract_ms, int64_t *edge_visits,
                int64_t *sweeps);

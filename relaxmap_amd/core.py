@@ -113,6 +113,8 @@ def load_library():
         "rmx_timing": (C.c_int, [vp, _F64P, _F64P, _F64P, _I64P, _I64P]),
         "rmx_set_profiling": (C.c_int, [vp, C.c_int]),
         "rmx_kernel_stats": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int, _I64P, _F64P, _F64P]),
+        "rmx_calculate_codelength_noalloc": (C.c_int, [C.c_int, vp, C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp,
+                                                       vp, _F64P, C.c_char_p, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -128,6 +130,21 @@ def device_count():
 
 def _p(a, t):
     return a.ctypes.data_as(t)
+
+
+def calculate_codelength_noalloc(device, stream, n, m, out_ptr, out_idx, flow, p, tau, dangling, module):
+    """rmx_calculate_codelength_noalloc on device addresses (ints): the
+    codelength (nats) of partition ``module`` of a network held in device
+    memory, computed with no device allocation.  Raises ValueError when the
+    library refuses the input (status -22) and RuntimeError otherwise."""
+    L = load_library()
+    out = C.c_double(0.0)
+    err = C.create_string_buffer(512)
+    rc = L.rmx_calculate_codelength_noalloc(int(device), stream, int(n), int(m), out_ptr, out_idx, flow, p, tau,
+                                            dangling, module, C.byref(out), err, 512)
+    if rc != 0:
+        raise (ValueError if rc == -22 else RuntimeError)(err.value.decode() or f"status {rc}")
+    return out.value
 
 
 class NetworkFile:
