@@ -170,6 +170,21 @@ int rmx_reset_active(rmx_ctx *ctx);
 int rmx_move_sweep(rmx_ctx *ctx, int prioritized, double vthresh, uint64_t seed, int64_t *moved,
                    double *codelength, int64_t *edge_visits);
 
+/* The decision half of Network::prioritize_parallelMove / parallelMove
+ * (Module.cpp:1355-1488, and the super-node twins Module.cpp:2154-2869
+ * after rmx_contract): every unit of the current level is evaluated against
+ * the same module statistics, as one batch, and nothing is committed.
+ * best_module[u] is the candidate the unit would move to (strict < over the
+ * neighbour modules in ascending id, the empty-module option, offered as in
+ * Module.cpp:1435-1439, as id N + 1 after them), or -1 when no candidate
+ * lowers the codelength; best_delta[u] is its delta codelength in nats (0
+ * for -1).  The sweeps' own batch rule applies: a unit alone in its module
+ * does not take another such module of larger id.  Not on a batch of
+ * instances.  The arrays hold `capacity` entries (N always suffices);
+ * *n_units receives the level's unit count. */
+int rmx_evaluate_moves(rmx_ctx *ctx, int32_t capacity, int32_t *n_units, int32_t *best_module,
+                       double *best_delta);
+
 /* Network::updateMembersInModule (Module.cpp:2881-2902). */
 int rmx_update_members(rmx_ctx *ctx);
 /* Network::convertModulesToSuperNodes (Module.cpp:3008-3077): contracts the

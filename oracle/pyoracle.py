@@ -68,6 +68,8 @@ def lib():
             "orc_reset_active": (None, [vp, C.c_int]),
             "orc_n_active": (C.c_int, [vp]),
             "orc_run": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+            "orc_evaluate_all": (None, [vp, C.c_int, C.c_int, _I32P, _F64P]),
+            "orc_candidates": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, _I32P, _F64P]),
             "orc_trace_len": (C.c_int, [vp]),
             "orc_get_trace": (None, [vp, _I32P]),
         }
@@ -230,6 +232,23 @@ class OracleNet:
 
     def sweep(self, variant="move", sp=False, vthresh=0.0):
         return self._L.orc_sweep(self._h, VARIANTS[variant], 1 if sp else 0, float(vthresh))
+
+    def evaluate_all(self, sp=False, no_singleton_swaps=False):
+        """Every unit's best move against one module snapshot (nothing moves):
+        (best module, -1 for none and n + 1 for the empty module; its dL)."""
+        k = self._L.orc_n_super(self._h) if sp else self.n
+        bm = np.empty(k, dtype=np.int32)
+        bd = np.empty(k, dtype=np.float64)
+        self._L.orc_evaluate_all(self._h, 1 if sp else 0, 1 if no_singleton_swaps else 0, _p(bm, _I32P),
+                                 _p(bd, _F64P))
+        return bm, bd
+
+    def candidates(self, u, sp=False, cap=100000):
+        """Unit u's candidate modules and their dL, in the order evaluated."""
+        mods = np.empty(cap, dtype=np.int32)
+        diffs = np.empty(cap, dtype=np.float64)
+        k = self._L.orc_candidates(self._h, 1 if sp else 0, int(u), cap, _p(mods, _I32P), _p(diffs, _F64P))
+        return mods[:k].copy(), diffs[:k].copy()
 
     def run(self, threshold=1e-3, vthresh=0.0, max_iter=10, prior=True):
         """main()'s super-step loop at one thread, until the first coarse tune."""

@@ -132,6 +132,12 @@ struct orc_net {
   /* numMoved of every sweep run by orc_run, in order */
   int32_t *trace;
   int n_trace, trace_cap;
+  /* orc_evaluate_all / orc_candidates: units per module for the batch rule
+   * (NULL: not applied), and the candidate record (NULL: not recorded) */
+  int32_t *eval_units;
+  int32_t *cand_mod;
+  double *cand_diff;
+  int cand_n, cand_cap;
   /* per-node scratch for the neighbour-module table */
   int tab_cap;
   int32_t *tab_key;
@@ -951,6 +957,8 @@ static void evaluate(orc_net *net, level_t *V, int sp, int variant, int u, move_
   for (int j = 0; j < nused; j++) {
     int h = net->tab_order[j], m = net->tab_key[h];
     if (m == oldMod) continue;
+    if (net->eval_units && m > oldMod && m < net->n && net->eval_units[oldMod] == 1 && net->eval_units[m] == 1)
+      continue;   /* the batch rule of orc_evaluate_all (no singleton joins a larger singleton) */
     double outNew = net->tab_out[h], inNew = net->tab_in[h];
     int real = m < net->n;
     double oldExit2 = real ? net->m_exit[m] : 0.0, oldSum2 = real ? net->m_sum[m] : 0.0;
@@ -965,9 +973,63 @@ static void evaluate(orc_net *net, level_t *V, int sp, int variant, int u, move_
     double dExit = plogp(c.exit1) + plogp(c.exit2) - plogp(oldExit1) - plogp(oldExit2);
     double dStay = plogp(c.exit1 + c.sum1) + plogp(c.exit2 + c.sum2) - plogp(oldExit1 + oldSum1) - plogp(oldExit2 + oldSum2);
     c.diff = dAll - 2.0 * dExit + dStay;
+    if (net->cand_mod && net->cand_n < net->cand_cap) {
+      net->cand_mod[net->cand_n] = m;
+      net->cand_diff[net->cand_n] = c.diff;
+      net->cand_n++;
+    }
     if (c.diff < best->diff) *best = c;
   }
   for (int j = 0; j < nused; j++) net->tab_key[net->tab_used[j]] = -1;
+}
+
+/* Every unit of the level evaluated against the same module statistics, as
+ * prioritize_parallelMove's decision loop does for each node before its
+ * commit (Module.cpp:1355-1488, and the SP twin Module.cpp:2154-2320):
+ * candidates in ascending module id with the empty-module sentinel n + 1
+ * last (Module.cpp:1348, 1435-1439), strict < from a best of 0.0
+ * (Module.cpp:1451, 1484).  best_mod[u] = -1 when no candidate has dL < 0.
+ * no_singleton_swaps applies a rule that is not the reference's: a unit
+ * alone in its module skips a candidate module of larger id that also holds
+ * one unit (the GPU's batch rule, which stands in for the commit-time check
+ * of Module.cpp:1508-1515 that rejects the second move of a swap). */
+void orc_evaluate_all(orc_net *net, int sp, int no_singleton_swaps, int32_t *best_mod, double *best_diff) {
+  level_t *V = sp ? &net->S : &net->L;
+  int32_t *units = NULL;
+  if (no_singleton_swaps) {
+    units = (int32_t *)xcalloc((size_t)net->n + 2, sizeof(int32_t));
+    for (int u = 0; u < V->n; u++) units[V->mod[u]]++;
+  }
+  net->eval_units = units;
+  for (int u = 0; u < V->n; u++) {
+    move_t best;
+    double oo, io;
+    evaluate(net, V, sp, 3, u, &best, &oo, &io);
+    best_mod[u] = best.diff < 0.0 ? best.mod : -1;
+    best_diff[u] = best.diff < 0.0 ? best.diff : 0.0;
+  }
+  net->eval_units = NULL;
+  free(units);
+}
+
+/* The candidates of unit u with their dL, in the order evaluated (ascending
+ * module id, the empty-module sentinel n + 1 last), the unit's own module
+ * left out.  Returns the number written (at most cap). */
+int orc_candidates(orc_net *net, int sp, int u, int cap, int32_t *mods, double *diffs) {
+  level_t *V = sp ? &net->S : &net->L;
+  if (u < 0 || u >= V->n) return -1;
+  net->cand_mod = mods;
+  net->cand_diff = diffs;
+  net->cand_n = 0;
+  net->cand_cap = cap;
+  move_t best;
+  double oo, io;
+  evaluate(net, V, sp, 3, u, &best, &oo, &io);
+  int k = net->cand_n;
+  net->cand_mod = NULL;
+  net->cand_diff = NULL;
+  net->cand_n = net->cand_cap = 0;
+  return k;
 }
 
 /* Applies a move of unit u to newMod with precomputed statistics

@@ -90,6 +90,18 @@ int orc_sweep(orc_net *net, int variant, int sp, double vthresh);
 void orc_reset_active(orc_net *net, int sp);
 int orc_n_active(const orc_net *net);
 
+/* The decision half of prioritize_parallelMove (Module.cpp:1355-1488) for
+ * every unit of the level (sp != 0: super nodes) against the same module
+ * statistics; nothing moves.  best_mod[u] is the argmin (strict <, ascending
+ * module id, the empty-module sentinel n + 1 last) or -1 when no candidate
+ * has dL < 0; best_diff[u] its dL (0 for -1).  no_singleton_swaps: a unit
+ * alone in its module skips a larger-id module that also holds one unit
+ * (not the reference's rule: the GPU's batch rule, for comparison). */
+void orc_evaluate_all(orc_net *net, int sp, int no_singleton_swaps, int32_t *best_mod, double *best_diff);
+/* Unit u's candidates and their dL as evaluated (its own module left out);
+ * returns the count written (at most cap). */
+int orc_candidates(orc_net *net, int sp, int u, int cap, int32_t *mods, double *diffs);
+
 /* stochastic_greedy_partition (OmpRelaxmap.cpp:263-429) with numTh == 1
  * and main()'s super-step loop (:181-219) up to the first coarse tune.
  * Returns the number of super steps run; *coarse_next is set when the

@@ -102,6 +102,7 @@ def load_library():
         "rmx_get_partition": (C.c_int, [vp, _I32P]),
         "rmx_reset_active": (C.c_int, [vp]),
         "rmx_move_sweep": (C.c_int, [vp, C.c_int, C.c_double, C.c_uint64, _I64P, _F64P, _I64P]),
+        "rmx_evaluate_moves": (C.c_int, [vp, C.c_int32, _I32P, _I32P, _F64P]),
         "rmx_update_members": (C.c_int, [vp]),
         "rmx_contract": (C.c_int, [vp, _I32P, _I64P]),
         "rmx_get_super": (C.c_int, [vp, _F64P, _F64P, _F64P, _F64P, _I32P, _I32P]),
@@ -383,6 +384,17 @@ class Network:
                                            C.byref(mv), C.byref(L), C.byref(ev)))
         self.last_edge_visits = ev.value
         return mv.value
+
+    def evaluate_moves(self):
+        """The decision half of one sweep over every unit of the current level,
+        all against the same module statistics, nothing committed: (best
+        module per unit, -1 for none and n + 1 for the empty module; its
+        delta codelength in nats)."""
+        bm = np.empty(self.n, dtype=np.int32)
+        bd = np.empty(self.n, dtype=np.float64)
+        k = C.c_int32()
+        self._check(self._L.rmx_evaluate_moves(self._h, self.n, C.byref(k), _p(bm, _I32P), _p(bd, _F64P)))
+        return bm[:k.value].copy(), bd[:k.value].copy()
 
     def update_members_in_module(self):
         self._check(self._L.rmx_update_members(self._h))

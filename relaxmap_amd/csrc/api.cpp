@@ -588,6 +588,38 @@ int rmx_move_sweep(rmx_ctx *ctx, int prioritized, double vthresh, uint64_t seed,
   });
 }
 
+int rmx_evaluate_moves(rmx_ctx *ctx, int32_t capacity, int32_t *n_units, int32_t *best_module,
+                       double *best_delta) {
+  return guarded(ctx, [&] {
+    Ctx &c = ctx->c;
+    if (!c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");
+    if (c.nb > 0) return fail(ctx, RMX_ESTATE, "not on a batch of instances");
+    const int32_t n = c.cur().n;
+    if (n_units) *n_units = n;
+    if (!best_module || !best_delta) return fail(ctx, RMX_EINVAL, "NULL output");
+    if (capacity < n) return fail(ctx, RMX_EINVAL, "output arrays hold fewer entries than the level's units");
+    DevBuf<uint32_t> bm(c.cache);
+    DevBuf<double> bd(c.cache);
+    bm.alloc(n);
+    bd.alloc(n);
+    evaluate_moves(c, bm.p, bd.p);
+    std::vector<uint32_t> hm(n);
+    RMX_HIP(hipMemcpyAsync(hm.data(), bm.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c.stream));
+    RMX_HIP(hipMemcpyAsync(best_delta, bd.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    for (int32_t u = 0; u < n; ++u) {
+      // no candidate with dL < 0 (the reference keeps bestResult.diffCodeLen = 0.0, Module.cpp:1451)
+      if (hm[u] == 0xffffffffu || !(best_delta[u] < 0.0)) {
+        best_module[u] = -1;
+        best_delta[u] = 0.0;
+      } else {
+        best_module[u] = (int32_t)hm[u];
+      }
+    }
+    return RMX_OK;
+  });
+}
+
 int rmx_update_members(rmx_ctx *ctx) {
   return guarded(ctx, [&] {
     if (!ctx->c.initiated) return fail(ctx, RMX_ESTATE, "rmx_initiate first");

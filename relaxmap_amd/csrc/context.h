@@ -344,6 +344,12 @@ void p5_launch(int32_t n, const uint32_t *optr, const uint32_t *oidx, const doub
 void ensure_sweep_buffers(Ctx &c);
 int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int64_t *edge_visits);
 void reset_active(Ctx &c);
+// One decision pass over every unit of the current level against the same
+// module statistics (one batch, nothing committed), as the per-node decision
+// of prioritize_parallelMove / parallelMove (Module.cpp:1355-1488): each
+// unit's best candidate module (0xffffffff for none; N + 1 for the empty
+// module) and its dL in nats, on the device (n units each).
+void evaluate_moves(Ctx &c, uint32_t *d_best_m, double *d_best_d);
 int64_t n_active(Ctx &c);
 // Sub-sweeps of one sweep of level L: the count set on the context, or, when
 // it is 0, one per 32768 units of the level, at least 4 and at most 8.  Each

@@ -101,6 +101,10 @@ struct EvalArgs {
   uint32_t *hkeys;
   long long *hout, *hin;
   const long long *hub_off;
+  // evaluation only (rmx_evaluate_moves): every evaluated unit's best
+  // candidate and its dL, written before the mover test (else null)
+  uint32_t *best_m;
+  double *best_d;
 };
 
 // Every unit draws its batch independently.  (Drawing by blocks of
@@ -250,8 +254,16 @@ __device__ inline bool allowed(const EvalArgs &a, const UnitCtx &U, uint32_t m) 
   return !(m > U.o && U.single && a.mUnits[m] == 1);
 }
 
+__device__ inline void record_best(const EvalArgs &a, uint32_t u, const Best &b) {
+  if (a.best_m) {
+    a.best_m[u] = b.m;
+    a.best_d[u] = b.d;
+  }
+}
+
 __device__ inline void emit(const EvalArgs &a, uint32_t u, const Best &b, long long fo_o, long long fi_o,
                             long long fo_n, long long fi_n) {
+  record_best(a, u, b);
   if (!(b.d < a.vthresh) || b.m == kNone) return;
   a.target[u] = b.m;
   long long *g = a.agg + 4 * (int64_t)u;
@@ -278,6 +290,7 @@ __device__ inline uint64_t lanes_below() {
 // counter per wavefront instead of one per mover.
 __device__ inline void emit_wave(const EvalArgs &a, bool active, uint32_t u, const Best &b, long long fo_o,
                                  long long fi_o, long long fo_n, long long fi_n) {
+  if (active) record_best(a, u, b);
   const bool mv = active && b.d < a.vthresh && b.m != kNone;
   const uint64_t bal = __ballot(mv);
   if (bal == 0) return;
@@ -1976,32 +1989,9 @@ struct SweepGraph {
   SweepGraph &operator=(const SweepGraph &) = delete;
 };
 
-int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int64_t *edge_visits) {
-  ensure_sweep_buffers(c);
-  Level &L = c.cur();
-  hipStream_t s = c.stream;
-  const int32_t n = L.n;
-  const uint64_t sweep_seed = mix64(seed ^ mix64(++c.sweep_counter));
-  const bool batch = c.nb > 0 && !c.sub_mode;   // independent instances (not a sub-module partition)
-
-  // empty modules available to this sweep, ascending id (none in a
-  // sub-module partition: its stale member lists never exceed one member,
-  // so the reference never offers the option there)
-  uint32_t n_empty = 0;
-  if (!c.sub_mode) {
-    k_empty_flags<<<grid_for(c.N, 256), 256, 0, s>>>(c.mNodes.p, c.N, c.rank_tmp.p);
-    RMX_LAUNCH_CHECK();
-    exclusive_scan_u32(c.rank_tmp.p, c.target.p, c.N, c.counters.p + 8, c.scratch, s);
-    k_compact<<<grid_for(c.N, 256), 256, 0, s>>>(c.rank_tmp.p, c.target.p, c.N, c.empty_list.p);
-    RMX_LAUNCH_CHECK();
-  }
-  fill_u32(c.counters.p + 2, 0u, 6, s);   // moved, visits, candidates
-  fill_u32(c.counters.p + cUnits, 0u, 1, s);
-  fill_u32(c.counters.p + cErr, 0u, 2, s);   // error, pull flag
-  fill_u32(c.counters.p + kStripeBase, 0u, kStripes * kStripeWords, s);
-
-  // hub hash-table pool
-  const int32_t n_hubs = L.bucket_off[kNumBuckets] - L.bucket_off[kNumBuckets - 1];
+// Hub buffers of level L: the hash-table pool and the partitioned
+// evaluation's buffers, sized once for all hubs of the level.
+static void ensure_hub_buffers(Ctx &c, Level &L, int32_t n_hubs) {
   if (n_hubs > 0 && c.hub_keys.n < L.hub_pool) {
     c.hub_keys.alloc(L.hub_pool);
     c.hub_out.alloc(L.hub_pool);
@@ -2039,12 +2029,28 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       c.hp_hubu.alloc(n_hubs);
     }
   }
+}
 
-  // this sweep's units per (bucket, batch), inactive units last: one stable
-  // 8-bit radix pass over the bucket positions
-  const int B = sweep_batches(c, L);
+// The empty modules available to a sweep (ascending id; none in a
+// sub-module partition: its stale member lists never exceed one member, so
+// the reference never offers the option there), and the sweep's units per
+// (bucket, batch), inactive units last: one stable 8-bit radix pass over the
+// bucket positions.  seg[k]..seg[k+1] is the range of key
+// k = bucket * (B + 1) + batch in c.batch_list.
+static void sweep_lists(Ctx &c, Level &L, bool prioritized, uint64_t sweep_seed, bool batch, int B,
+                        std::vector<uint32_t> &seg, uint32_t &n_empty) {
+  hipStream_t s = c.stream;
+  const int32_t n = L.n;
+  n_empty = 0;
+  if (!c.sub_mode) {
+    k_empty_flags<<<grid_for(c.N, 256), 256, 0, s>>>(c.mNodes.p, c.N, c.rank_tmp.p);
+    RMX_LAUNCH_CHECK();
+    exclusive_scan_u32(c.rank_tmp.p, c.target.p, c.N, c.counters.p + 8, c.scratch, s);
+    k_compact<<<grid_for(c.N, 256), 256, 0, s>>>(c.rank_tmp.p, c.target.p, c.N, c.empty_list.p);
+    RMX_LAUNCH_CHECK();
+  }
   const int NK = kNumBuckets * (B + 1);
-  std::vector<uint32_t> seg(NK + 1, 0);
+  seg.assign(NK + 1, 0);
   {
     const int *o = L.bucket_off;
     k_batch_keys<<<grid_for(n, 256), 256, 0, s>>>(L.bucket.p, n, make_int4(o[0], o[1], o[2], o[3]),
@@ -2065,6 +2071,119 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     for (int k = 0; k < NK; ++k) seg[k + 1] = seg[k] + h[16 + k];
     n_empty = c.sub_mode ? 0u : h[8];
   }
+}
+
+// The global-table evaluation of a batch's hubs whose partition table
+// overflowed (its plan gives the other hubs no work).
+static void hub_fallback(Ctx &c, EvalArgs a, const uint32_t *list, int32_t cnt, bool f32, hipStream_t s) {
+  a.list = list;
+  a.list_n = cnt;
+  HubPlan P{c.hub_plan.p, c.hub_plan.p + (cnt + 1), c.hub_plan.p + 2 * (cnt + 1), c.hub_bd.p, c.hub_bm.p,
+            c.hp_ovf.p};
+  k_hub_plan<<<1, 256, 0, s>>>(a, P);
+  k_hub_clear<<<4096, 256, 0, s>>>(a, P);
+  k_hub_insert<<<4096, 256, 0, s>>>(a, P);
+  if (f32) k_hub_eval<true><<<4096, 256, 0, s>>>(a, P);
+  else k_hub_eval<false><<<4096, 256, 0, s>>>(a, P);
+  k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, P);
+  RMX_LAUNCH_CHECK();
+  fold_counters(c, s);
+}
+
+// The evaluation launches of batch b of a sweep (every degree bucket, hubs
+// included); returns the units evaluated, and the batch's hubs for the
+// overflow fallback in hub_list / hub_n.
+static uint32_t eval_batch(Ctx &c, Level &L, EvalArgs &a, const std::vector<uint32_t> &seg, int B, int b,
+                           bool f32, hipStream_t s, const uint32_t *&hub_list, int32_t &hub_n) {
+  hub_list = nullptr;
+  hub_n = 0;
+  uint32_t units_b = 0;   // units evaluated in this batch
+  for (int bk = 0; bk < kNumBuckets; ++bk) {
+    const int key = bk * (B + 1) + b;
+    const int32_t cnt = (int32_t)(seg[key + 1] - seg[key]);
+    if (cnt <= 0) continue;
+    units_b += (uint32_t)cnt;
+    a.list = c.batch_list.p + seg[key];
+    a.list_n = cnt;
+    if (c.prof.on) c.prof.begin(s);
+    switch (bk) {
+      case 0: case 1: case 2: case 3: case 4: case 5: case 6:
+        launch_bucket(bk, a, cnt, f32, s);
+        break;
+      default: {
+        if (c.hub_global == 1) {
+          HubPlan P{c.hub_plan.p, c.hub_plan.p + (cnt + 1), c.hub_plan.p + 2 * (cnt + 1), c.hub_bd.p,
+                    c.hub_bm.p, nullptr};
+          k_hub_plan<<<1, 256, 0, s>>>(a, P);
+          k_hub_clear<<<4096, 256, 0, s>>>(a, P);
+          k_hub_insert<<<4096, 256, 0, s>>>(a, P);
+          if (f32) k_hub_eval<true><<<4096, 256, 0, s>>>(a, P);
+          else k_hub_eval<false><<<4096, 256, 0, s>>>(a, P);
+          k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, P);
+          break;
+        }
+        const int64_t NB = L.hub_parts;
+        HubParts H;
+        H.A = c.hp_plan.p;
+        H.PB = H.A + (cnt + 1);
+        H.AB = H.PB + (cnt + 1);
+        H.tot = H.AB + (cnt + 1);
+        H.bcnt = c.hp_bcnt.p;
+        H.boff = c.hp_boff.p;
+        H.mtmp = c.hp_mtmp.p;
+        H.ekey = c.hp_ekey.p;
+        H.ew = c.hp_ew.p;
+        H.own = c.hp_own.p;
+        H.ovf = c.hp_ovf.p;
+        H.bd = c.hp_bd.p;
+        H.bm = c.hp_bm.p;
+        H.bfo = c.hp_bfo.p;
+        H.bfi = c.hp_bfi.p;
+        H.force_ovf = c.hub_global == 2 ? 1 : 0;
+        H.hubu = c.hp_hubu.p;
+        H.chub = c.hp_chub.p;
+        H.phub = c.hp_phub.p;
+        k_hubp_plan<<<1, kPlanBlock, 0, s>>>(a, H);
+        k_hubp_owners<<<grid_for((int64_t)cnt * kWave, 256), 256, 0, s>>>(a, H);
+        fill_u32(H.bcnt, 0u, NB + 1, s);
+        fill_i64(H.own, 0, 2 * (int64_t)cnt, s);
+        fill_u32(H.ovf, 0u, cnt, s);
+        k_hubp_count<<<4096, 256, 0, s>>>(a, H);
+        exclusive_scan_u32(H.bcnt, H.boff, NB + 1, nullptr, c.scratch, s);
+        fill_u32(H.bcnt, 0u, NB + 1, s);
+        k_hubp_scatter<<<4096, 256, 0, s>>>(a, H);
+        if (f32) k_hubp_eval<true><<<8192, 256, 0, s>>>(a, H);
+        else k_hubp_eval<false><<<8192, 256, 0, s>>>(a, H);
+        k_hubp_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, H);
+        hub_list = a.list;
+        hub_n = cnt;
+        break;
+      }
+    }
+    RMX_LAUNCH_CHECK();
+    if (c.prof.on) profile_eval(c, bk, s);
+  }
+  return units_b;
+}
+
+int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int64_t *edge_visits) {
+  ensure_sweep_buffers(c);
+  Level &L = c.cur();
+  hipStream_t s = c.stream;
+  const int32_t n = L.n;
+  const uint64_t sweep_seed = mix64(seed ^ mix64(++c.sweep_counter));
+  const bool batch = c.nb > 0 && !c.sub_mode;   // independent instances (not a sub-module partition)
+
+  fill_u32(c.counters.p + 2, 0u, 6, s);   // moved, visits, candidates
+  fill_u32(c.counters.p + cUnits, 0u, 1, s);
+  fill_u32(c.counters.p + cErr, 0u, 2, s);   // error, pull flag
+  fill_u32(c.counters.p + kStripeBase, 0u, kStripes * kStripeWords, s);
+  const int32_t n_hubs = L.bucket_off[kNumBuckets] - L.bucket_off[kNumBuckets - 1];
+  ensure_hub_buffers(c, L, n_hubs);
+  const int B = sweep_batches(c, L);
+  std::vector<uint32_t> seg;
+  uint32_t n_empty = 0;
+  sweep_lists(c, L, prioritized, sweep_seed, batch, B, seg, n_empty);
   uint32_t empty_cursor = 0;
   if (batch) {   // each instance's empty modules
     c.b_ebeg.alloc(c.nb);
@@ -2087,23 +2206,6 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
 
   ArcW aout = arcw_out(c, L);
   const bool f32 = c.precision == 1;   // mixed precision: candidates in FP32
-
-  // the global-table evaluation of this batch's hubs whose partition table
-  // overflowed (its plan gives the other hubs no work)
-  auto hub_fallback = [&](const uint32_t *list, int32_t cnt) {
-    a.list = list;
-    a.list_n = cnt;
-    HubPlan P{c.hub_plan.p, c.hub_plan.p + (cnt + 1), c.hub_plan.p + 2 * (cnt + 1), c.hub_bd.p, c.hub_bm.p,
-              c.hp_ovf.p};
-    k_hub_plan<<<1, 256, 0, s>>>(a, P);
-    k_hub_clear<<<4096, 256, 0, s>>>(a, P);
-    k_hub_insert<<<4096, 256, 0, s>>>(a, P);
-    if (f32) k_hub_eval<true><<<4096, 256, 0, s>>>(a, P);
-    else k_hub_eval<false><<<4096, 256, 0, s>>>(a, P);
-    k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, P);
-    RMX_LAUNCH_CHECK();
-    fold_counters(c, s);
-  };
 
   // A batch of instances or a sub-module partition keeps its empty-module
   // cursors on the device (or offers none), so its sweep needs no host wait:
@@ -2130,83 +2232,18 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     a.empty_avail = empty_cursor < n_empty ? 1 : 0;
     k_reset_counters<<<1, 64, 0, s>>>(c.counters.p);
     RMX_LAUNCH_CHECK();
-    uint32_t units_b = 0;   // units evaluated in this batch
-    for (int bk = 0; bk < kNumBuckets; ++bk) {
-      const int key = bk * (B + 1) + b;
-      const int32_t cnt = (int32_t)(seg[key + 1] - seg[key]);
-      if (cnt <= 0) continue;
-      units_b += (uint32_t)cnt;
-      a.list = c.batch_list.p + seg[key];
-      a.list_n = cnt;
-      if (c.prof.on) c.prof.begin(s);
-      switch (bk) {
-        case 0: case 1: case 2: case 3: case 4: case 5: case 6:
-          launch_bucket(bk, a, cnt, f32, s);
-          break;
-        default: {
-          if (c.hub_global == 1) {
-            HubPlan P{c.hub_plan.p, c.hub_plan.p + (cnt + 1), c.hub_plan.p + 2 * (cnt + 1), c.hub_bd.p,
-                      c.hub_bm.p, nullptr};
-            k_hub_plan<<<1, 256, 0, s>>>(a, P);
-            k_hub_clear<<<4096, 256, 0, s>>>(a, P);
-            k_hub_insert<<<4096, 256, 0, s>>>(a, P);
-            if (f32) k_hub_eval<true><<<4096, 256, 0, s>>>(a, P);
-            else k_hub_eval<false><<<4096, 256, 0, s>>>(a, P);
-            k_hub_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, P);
-            break;
-          }
-          const int64_t NB = L.hub_parts;
-          HubParts H;
-          H.A = c.hp_plan.p;
-          H.PB = H.A + (cnt + 1);
-          H.AB = H.PB + (cnt + 1);
-          H.tot = H.AB + (cnt + 1);
-          H.bcnt = c.hp_bcnt.p;
-          H.boff = c.hp_boff.p;
-          H.mtmp = c.hp_mtmp.p;
-          H.ekey = c.hp_ekey.p;
-          H.ew = c.hp_ew.p;
-          H.own = c.hp_own.p;
-          H.ovf = c.hp_ovf.p;
-          H.bd = c.hp_bd.p;
-          H.bm = c.hp_bm.p;
-          H.bfo = c.hp_bfo.p;
-          H.bfi = c.hp_bfi.p;
-          H.force_ovf = c.hub_global == 2 ? 1 : 0;
-          H.hubu = c.hp_hubu.p;
-          H.chub = c.hp_chub.p;
-          H.phub = c.hp_phub.p;
-          k_hubp_plan<<<1, kPlanBlock, 0, s>>>(a, H);
-          k_hubp_owners<<<grid_for((int64_t)cnt * kWave, 256), 256, 0, s>>>(a, H);
-          fill_u32(H.bcnt, 0u, NB + 1, s);
-          fill_i64(H.own, 0, 2 * (int64_t)cnt, s);
-          fill_u32(H.ovf, 0u, cnt, s);
-          k_hubp_count<<<4096, 256, 0, s>>>(a, H);
-          exclusive_scan_u32(H.bcnt, H.boff, NB + 1, nullptr, c.scratch, s);
-          fill_u32(H.bcnt, 0u, NB + 1, s);
-          k_hubp_scatter<<<4096, 256, 0, s>>>(a, H);
-          if (f32) k_hubp_eval<true><<<8192, 256, 0, s>>>(a, H);
-          else k_hubp_eval<false><<<8192, 256, 0, s>>>(a, H);
-          k_hubp_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, H);
-          hub_list = a.list;
-          hub_n = cnt;
-          break;
-        }
-      }
-      RMX_LAUNCH_CHECK();
-      if (c.prof.on) profile_eval(c, bk, s);
-    }
+    const uint32_t units_b = eval_batch(c, L, a, seg, B, b, f32, s, hub_list, hub_n);
     fold_counters(c, s);
     if (units_b == 0) continue;
     uint32_t n_movers = units_b, n_want = 1;   // bounds when not read back
     if (quiet) {
-      if (hub_n > 0) hub_fallback(hub_list, hub_n);
+      if (hub_n > 0) hub_fallback(c, a, hub_list, hub_n, f32, s);
     } else {
       RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
       c.sync();
       if (c.h_cnt[cErr]) throw HipError("move sweep: mover list of a counter copy overflowed");
       if (hub_n > 0 && c.h_cnt[cHubOvf] > 0) {   // a partition table overflowed: those hubs again
-        hub_fallback(hub_list, hub_n);
+        hub_fallback(c, a, hub_list, hub_n, f32, s);
         RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
         c.sync();
         if (c.h_cnt[cErr]) throw HipError("move sweep: mover list of a counter copy overflowed");
@@ -2297,6 +2334,46 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
   c.timing.edge_visits += (int64_t)vis;
   c.timing.sweeps++;
   return moved;
+}
+
+void evaluate_moves(Ctx &c, uint32_t *d_best_m, double *d_best_d) {
+  ensure_sweep_buffers(c);
+  Level &L = c.cur();
+  hipStream_t s = c.stream;
+  const int32_t n = L.n;
+  fill_u32(c.counters.p + 2, 0u, 6, s);
+  fill_u32(c.counters.p + cUnits, 0u, 1, s);
+  fill_u32(c.counters.p + cErr, 0u, 2, s);
+  fill_u32(c.counters.p + kStripeBase, 0u, kStripes * kStripeWords, s);
+  const int32_t n_hubs = L.bucket_off[kNumBuckets] - L.bucket_off[kNumBuckets - 1];
+  ensure_hub_buffers(c, L, n_hubs);
+  std::vector<uint32_t> seg;
+  uint32_t n_empty = 0;
+  sweep_lists(c, L, false, 0, false, 1, seg, n_empty);   // every unit, one batch
+  module_totals_async(c);                                 // S of the current partition in c.scal
+  EvalArgs a = sweep_args(c, L, false, 0.0, 0, false);
+  a.batches = 1;
+  a.batch = 0;
+  a.vthresh = -HUGE_VAL;   // records every unit's best and emits no mover
+  a.empty_avail = n_empty > 0 ? 1 : 0;
+  a.best_m = d_best_m;
+  a.best_d = d_best_d;
+  fill_u32(d_best_m, kNone, n, s);
+  k_reset_counters<<<1, 64, 0, s>>>(c.counters.p);
+  RMX_LAUNCH_CHECK();
+  const bool f32 = c.precision == 1;
+  const uint32_t *hub_list = nullptr;
+  int32_t hub_n = 0;
+  eval_batch(c, L, a, seg, 1, 0, f32, s, hub_list, hub_n);
+  fold_counters(c, s);
+  RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
+  c.sync();
+  if (hub_n > 0 && c.h_cnt[cHubOvf] > 0) {   // a partition table overflowed: those hubs again
+    hub_fallback(c, a, hub_list, hub_n, f32, s);
+    RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
+    c.sync();
+  }
+  if (c.h_cnt[cErr]) throw HipError("evaluation: an error flag was raised");
 }
 
 int64_t n_active(Ctx &c) {
