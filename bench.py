@@ -88,61 +88,197 @@ def profile_pass(graph, precision, args):
     return stats
 
 
-def cpu_baseline(args):
+REF_EXE = os.path.join(ROOT, "oracle", "_ref", "ompRelaxmap")
+
+
+def host_info():
+    """Threads this process may run on (what nproc reports), CPU model, RAM."""
+    threads = len(os.sched_getaffinity(0))
+    model = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    mem = None
+    try:
+        for ln in open("/proc/meminfo"):
+            if ln.startswith("MemTotal"):
+                mem = round(int(ln.split()[1]) / 2 ** 20, 1)
+                break
+    except OSError:
+        pass
+    return {"threads": threads, "cpu_model": model, "ram_gib": mem}
+
+
+def run_reference(path, outdir, threads, timeout):
+    """ompRelaxmap seed 1, prior, threshold 1e-3, vThresh 0, maxIter 10, one
+    attempt; the timings and codelength it prints (OmpRelaxmap.cpp:101, 222,
+    229, 233, 334-335)."""
+    cmd = [REF_EXE, "1", path, str(threads), "1", "1e-3", "0", "10", outdir, "prior"]
+    t = time.perf_counter()
+    try:
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout).stdout
+        timed_out = False
+    except subprocess.TimeoutExpired as e:
+        out = (e.stdout or b"").decode() if isinstance(e.stdout, bytes) else (e.stdout or "")
+        timed_out = True
+    wall = time.perf_counter() - t
+    num = r"([0-9.eE+-]+)"
+    it1 = re.search(r"Iteration 1: code length =\s*" + num + r"\s*, elapsed time:\s*" + num, out)
+    total = re.search(r"Overall Elapsed Time for Module Detection \(w/o file IO\): " + num, out)
+    read = re.search(r"Time for reading input data : " + num, out)
+    final = re.search(r"Computed Code Length = " + num, out)
+    return {"first_sweep_s": float(it1.group(2)) if it1 else None,
+            "time_to_converged_s": float(total.group(1)) if total else None,
+            "read_s": float(read.group(1)) if read else None,
+            "codelength_bits": float(final.group(1)) if final else None,
+            "threads": threads, "wall_s": wall, "timed_out": timed_out}
+
+
+def reference_partition(outdir, name, n):
+    """The module of every node from the reference's .clu file (one raw
+    module id per line after the header, OmpRelaxmap.cpp:240-249)."""
+    path = os.path.join(outdir, name + ".clu")
+    try:
+        lines = open(path).read().split()
+    except OSError:
+        return None
+    vals = np.array([int(x) for x in lines[2:]], dtype=np.int64)   # "*Vertices n" header
+    return vals if len(vals) == n else None
+
+
+def cpu_baseline(args, host):
     """The reference OpenMP binary (built from its sources into oracle/_ref)
-    on a bounded sample: the 100k-node LFR-style network, all its sweeps."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "ompRelaxmap")
-    if not os.path.exists(exe):
+    on a bounded sample: the 100k-node LFR-style network, all its sweeps, on
+    every host thread this process may use."""
+    if not os.path.exists(REF_EXE):
         log("reference binary oracle/_ref/ompRelaxmap not built; no CPU baseline")
         return None, None
     from relaxmap_amd import synth
     g = synth.config2()
-    threads = min(16, os.cpu_count() or 1)
+    threads = host["threads"]
     with tempfile.TemporaryDirectory() as d:
         path = os.path.join(d, "lfr100k.net")
         g.write(path)
-        cmd = [exe, "1", path, str(threads), "1", "1e-3", "0", "10", d, "prior"]
-        t = time.perf_counter()
-        try:
-            out = subprocess.run(cmd, capture_output=True, text=True, timeout=args.cpu_timeout).stdout
-        except subprocess.TimeoutExpired as e:
-            out = (e.stdout or b"").decode() if isinstance(e.stdout, bytes) else (e.stdout or "")
-        wall = time.perf_counter() - t
-    it1 = re.search(r"Iteration 1: code length =\s*([0-9.eE+-]+)\s*, elapsed time:\s*([0-9.eE+-]+)", out)
-    total = re.search(r"Overall Elapsed Time for Module Detection \(w/o file IO\): ([0-9.eE+-]+)", out)
-    final = re.search(r"Computed Code Length = ([0-9.eE+-]+)", out)
-    if not it1:
+        info = run_reference(path, d, threads, args.cpu_timeout)
+        part = reference_partition(d, "lfr100k", g.n)
+    if info["first_sweep_s"] is None:
         log("reference run produced no sweep timing")
         return None, None
-    t_it1 = float(it1.group(2))
+    t_it1 = info["first_sweep_s"]
     gteps = 2.0 * g.m / t_it1 / 1e9
-    info = {"graph_nodes": g.n, "graph_arcs": g.m, "first_sweep_s": t_it1,
-            "time_to_converged_s": float(total.group(1)) if total else None,
-            "codelength_bits": float(final.group(1)) if final else None, "threads": threads, "wall_s": wall}
+    info.update({"graph_nodes": g.n, "graph_arcs": g.m})
+    if part is not None and g.community[0] >= 0:
+        info["nmi_planted"] = nmi(part, g.community)
     log(f"reference OpenMP ({threads} threads) on {g.n} nodes / {g.m} arcs: first sweep {t_it1:.3f} s "
         f"({gteps:.4f} GTEPS), converged in {info['time_to_converged_s']} s")
     base = {"value": gteps, "unit": "GTEPS", "cores": threads, "kind": "reference",
-            "sample": f"reference ompRelaxmap (unmodified sources, -O3 -fopenmp, {threads} threads, prior, "
+            "sample": f"reference ompRelaxmap (unmodified sources, -O3 -fopenmp, {threads} threads = nproc, prior, "
                       f"threshold 1e-3, maxIter 10) on the 100k-node LFR-style network ({g.m} arcs); "
                       f"value = 2 x arcs / time of its first full sweep"}
     return base, info
 
 
-def gpu_on_sample(args):
-    """The GPU path on the CPU baseline's sample, for a same-input wall-time ratio."""
-    from relaxmap_amd import Network, synth
-    g = synth.config2()
+def gpu_same_input(g, args):
+    """The GPU path on a network the reference also ran: the time from the
+    host's parsed arrays to the converged codelength (upload, in-CSR and
+    buckets, steady state, partition), the span of the reference's
+    "w/o file IO" timer (OmpRelaxmap.cpp:103-229), and the kernel part alone
+    (initiate + partition on the resident network)."""
+    from relaxmap_amd import Network
     out = {}
     for prec in ("mixed", "fp64"):
+        net = Network(precision=prec, batches=args.batches)   # warm-up run of the whole path
+        g.upload_to(net)
+        net.initiate()
+        net.run(threshold=1e-3, vthresh=0.0, max_iter=10, prior=True, seed=1)
+        net.close()
+        t0 = time.perf_counter()
+        net = Network(precision=prec, batches=args.batches)
+        g.upload_to(net)
+        t1 = time.perf_counter()
+        net.initiate()
+        r = net.run(threshold=1e-3, vthresh=0.0, max_iter=10, prior=True, seed=1)
+        t2 = time.perf_counter()
+        rec = {"time_to_converged_s": t2 - t0, "upload_s": t1 - t0, "initiate_and_partition_s": t2 - t1,
+               "codelength_bits": r.computed_codelength / math.log(2), "modules": r.n_module}
+        if g.community[0] >= 0:
+            rec["nmi_planted"] = nmi(net.partition(), g.community)
+        out[prec] = rec
+        net.close()
+    return out
+
+
+def reference_large(args, host):
+    """A second same-input point: the Chung-Lu generator of the 500M-arc
+    workload at 2M nodes (about 31M arcs), the reference and the GPU."""
+    from relaxmap_amd import synth
+    g = synth.config5(n=args.ref_nodes, mean_deg=DEFAULT_MEAN_DEG)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "chunglu.net")
+        t = time.perf_counter()
+        g.write(path)
+        log(f"second reference point: {g.n} nodes / {g.m} arcs written in {time.perf_counter() - t:.1f} s")
+        ref = run_reference(path, d, host["threads"], args.ref_timeout)
+        part = reference_partition(d, "chunglu", g.n)
+    ref.update({"graph_nodes": g.n, "graph_arcs": g.m})
+    if part is not None:
+        ref["nmi_planted"] = nmi(part, g.community)
+    log(f"reference on {g.n} nodes: converged in {ref['time_to_converged_s']} s (read {ref['read_s']} s), "
+        f"L = {ref['codelength_bits']} bits")
+    gpu = gpu_same_input(g, args)
+    out = {"workload": f"directed Chung-Lu, the 500M-arc configuration's generator at {g.n} nodes",
+           "reference": ref, "gpu": gpu}
+    if ref["time_to_converged_s"]:
+        out["speedup_vs_reference"] = {k: ref["time_to_converged_s"] / v["time_to_converged_s"]
+                                       for k, v in gpu.items()}
+    return out
+
+
+def config3(args):
+    """R-MAT scale 22 (edge factor 16, a, b, c, d = .57, .19, .19, .05, FP32
+    U(0,1] weights, dangling nodes kept): the precision pattern's error on a
+    skewed, weakly modular network.  Flows of both precisions, the
+    codelength of one fixed partition (FP64's converged one) in both, and
+    each precision's own converged codelength and time."""
+    from relaxmap_amd import Network, synth
+    t = time.perf_counter()
+    g = synth.config3()
+    log(f"config 3: R-MAT scale 22, {g.n} nodes / {g.m} arcs generated in {time.perf_counter() - t:.1f} s")
+    res = {}
+    part = None
+    for prec in ("fp64", "mixed"):
         net = Network(precision=prec, batches=args.batches)
         g.upload_to(net)
         net.initiate()   # warm-up
-        t = time.perf_counter()
-        net.initiate()
+        t0 = time.perf_counter()
+        it, _ = net.initiate()
+        t1 = time.perf_counter()
         r = net.run(threshold=1e-3, vthresh=0.0, max_iter=10, prior=True, seed=1)
-        out[prec] = {"time_to_converged_s": time.perf_counter() - t,
-                     "codelength_bits": r.computed_codelength / math.log(2)}
+        t2 = time.perf_counter()
+        rec = {"flows": net.flows(), "iterations": it, "initiate_s": t1 - t0, "time_to_converged_s": t2 - t0,
+               "codelength_bits": r.computed_codelength / math.log(2), "modules": r.n_module,
+               "move_sweep_gteps": r.edge_visits / max(r.time_sweeps_s, 1e-9) / 1e9}
+        if part is None:
+            part = net.partition()
+        net.set_partition(part)
+        rec["fixed_partition_codelength"] = net.calculate_codelength()
+        res[prec] = rec
         net.close()
+    d, m = res["fp64"], res["mixed"]
+    out = {"workload": "R-MAT scale 22, edge factor 16, (0.57, 0.19, 0.19, 0.05), FP32 U(0,1] weights",
+           "nodes": g.n, "arcs": g.m,
+           "flow_l1_mixed_vs_fp64": float(np.abs(m["flows"] - d["flows"]).sum()),
+           "codelength_rel": (m["fixed_partition_codelength"] - d["fixed_partition_codelength"])
+           / d["fixed_partition_codelength"],
+           "final_codelength_rel": (m["codelength_bits"] - d["codelength_bits"]) / d["codelength_bits"]}
+    for prec in ("fp64", "mixed"):
+        out[prec] = {k: v for k, v in res[prec].items() if k != "flows"}
+    log(f"config 3: flow L1 mixed vs FP64 {out['flow_l1_mixed_vs_fp64']:.2e}, fixed-partition codelength rel "
+        f"{out['codelength_rel']:.2e}")
     return out
 
 
@@ -202,6 +338,45 @@ def config4(args):
     out["codelength_rel_batch_vs_contexts"] = {"median": float(np.median(rel)), "max": float(np.max(rel))}
     log(f"config 4: {len(res_c)} instances by {args.workers} contexts in {dt_c:.2f} s "
         f"({len(res_c) / dt_c:.1f} instances/s)")
+    if not args.no_cpu and os.path.exists(REF_EXE):
+        out["reference"] = config4_reference(graphs, res, args)
+    return out
+
+
+def config4_reference(graphs, res, args):
+    """The reference on the 64 networks with seed 1 (one process per network
+    on all host threads, as it runs), scaled to the 512 instances by x 8.  A
+    network on which the multi-threaded reference crashes (its parallel
+    super-node sweep reads the module array past its end, Module.cpp:2642)
+    is run again on one thread, the reference's sequential path."""
+    threads = host_info()["threads"]
+    no_io = wall = 0.0
+    rel = []
+    crashed = []
+    with tempfile.TemporaryDirectory() as d:
+        for gi, g in enumerate(graphs):
+            path = os.path.join(d, f"sbm{gi}.net")
+            g.write(path)
+            r = run_reference(path, d, threads, args.cpu_timeout)
+            if r["time_to_converged_s"] is None:
+                crashed.append(gi)
+                r = run_reference(path, d, 1, args.cpu_timeout)
+                if r["time_to_converged_s"] is None:
+                    log(f"config 4 reference: network {gi} gave no timing on {threads} threads or on one")
+                    return None
+            no_io += r["time_to_converged_s"]
+            wall += r["wall_s"]
+            if r["codelength_bits"]:
+                gpu_bits = res[(gi, 1)][0]   # bits
+                rel.append((gpu_bits - r["codelength_bits"]) / r["codelength_bits"])
+    out = {"networks": len(graphs), "seed": 1, "threads": threads, "seconds_without_io": no_io,
+           "seconds_wall": wall, "instances_per_s": len(graphs) / no_io,
+           "scaled_512_instances_s": 8 * no_io,
+           "crashed_multithreaded_rerun_on_one_thread": crashed,
+           "gpu_batch_codelength_rel_vs_reference": {"median": float(np.median(rel)), "max": float(np.max(rel)),
+                                                     "min": float(np.min(rel))} if rel else None}
+    log(f"config 4 reference: 64 networks in {no_io:.2f} s without file IO ({wall:.1f} s wall), "
+        f"{out['instances_per_s']:.1f} instances/s; 512 instances ~ {8 * no_io:.1f} s")
     return out
 
 
@@ -239,6 +414,10 @@ def main():
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
     ap.add_argument("--no-config4", action="store_true", help="skip the 512-instance batch measurement")
     ap.add_argument("--workers", type=int, default=8, help="concurrent contexts of the batch measurement")
+    ap.add_argument("--no-config3", action="store_true", help="skip the R-MAT scale-22 precision measurement")
+    ap.add_argument("--no-ref-large", action="store_true", help="skip the second same-input reference point")
+    ap.add_argument("--ref-nodes", type=int, default=2_000_000)
+    ap.add_argument("--ref-timeout", type=float, default=300.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the hot path here is a single-GPU build "
@@ -302,18 +481,25 @@ def main():
         out["kernels"] = {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in
                           sorted(table.items(), key=lambda kv: -kv[1]["total_ms"])[:12]}
     del g
+    if not args.no_config3:
+        out["config3"] = config3(args)
     if not args.no_config4:
         out["config4"] = config4(args)
     if not args.no_cpu:
-        base, info = cpu_baseline(args)
+        host = host_info()
+        out["host"] = host
+        base, info = cpu_baseline(args, host)
         out["cpu_baseline"] = base
         if info:
             out["cpu_baseline_detail"] = info
-            gs = gpu_on_sample(args)
+            from relaxmap_amd import synth
+            gs = gpu_same_input(synth.config2(), args)
             out["gpu_on_cpu_sample"] = gs
             if info.get("time_to_converged_s"):
                 out["speedup_vs_reference_on_sample"] = {
                     k: info["time_to_converged_s"] / v["time_to_converged_s"] for k, v in gs.items()}
+            if not args.no_ref_large:
+                out["reference_large"] = reference_large(args, host)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -14,13 +14,40 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
+
+namespace {
+// An allocator whose value construction leaves the memory uninitialised, so
+// that resizing the multi-GB arc arrays does not zero them on one thread (the
+// parallel loops that fill them touch their pages first).
+template <typename T>
+struct NoInit : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInit<U>;
+  };
+  NoInit() = default;
+  template <typename U>
+  NoInit(const NoInit<U> &) {}
+  template <typename U>
+  void construct(U *p) noexcept {
+    ::new ((void *)p) U;
+  }
+  template <typename U, typename... A>
+  void construct(U *p, A &&...a) {
+    ::new ((void *)p) U(std::forward<A>(a)...);
+  }
+};
+template <typename T>
+using big_vector = std::vector<T, NoInit<T>>;
+}  // namespace
 
 struct rmx_syn {
   int32_t n = 0;
   std::vector<int64_t> ptr;
-  std::vector<int32_t> idx;
-  std::vector<double> w;
+  big_vector<int32_t> idx;
+  big_vector<double> w;
   std::vector<int32_t> com;
 };
 
@@ -133,8 +160,8 @@ std::vector<int64_t> community_bounds(int32_t n, double g, int32_t cmin, int32_t
 struct Builder {
   int32_t n;
   std::vector<int64_t> off;   // raw offsets (before dedup)
-  std::vector<int32_t> tgt;
-  std::vector<double> wt;
+  big_vector<int32_t> tgt;
+  big_vector<double> wt;
   explicit Builder(int32_t n_) : n(n_), off((size_t)n_ + 1, 0) {}
   void alloc() {
     for (int32_t i = 0; i < n; ++i) off[i + 1] += off[i];
@@ -143,30 +170,33 @@ struct Builder {
   }
   void finish(rmx_syn &g) {
     std::vector<int64_t> deg((size_t)n + 1, 0);
-#pragma omp parallel for schedule(dynamic, 1024)
-    for (int32_t u = 0; u < n; ++u) {
-      int64_t b = off[u], e = off[u + 1];
-      // sort by target, stable in arc order so duplicate weights sum in order
-      std::vector<std::pair<int32_t, int64_t>> tmp;
-      tmp.reserve(e - b);
-      for (int64_t k = b; k < e; ++k)
-        if (tgt[k] != u) tmp.push_back({tgt[k], k});
-      std::sort(tmp.begin(), tmp.end());
+#pragma omp parallel
+    {
+      std::vector<std::pair<int32_t, int64_t>> tmp;   // per thread, reused across nodes
       std::vector<double> ws;
-      ws.reserve(tmp.size());
-      int64_t o = b;
-      for (size_t i = 0; i < tmp.size(); ++i) {
-        double wv = wt[tmp[i].second];
-        if (i > 0 && tmp[i].first == tmp[i - 1].first) {
-          ws.back() += wv;
-        } else {
-          tgt[o] = tmp[i].first;
-          ws.push_back(wv);
-          ++o;
+#pragma omp for schedule(dynamic, 1024)
+      for (int32_t u = 0; u < n; ++u) {
+        int64_t b = off[u], e = off[u + 1];
+        // sort by target, stable in arc order so duplicate weights sum in order
+        tmp.clear();
+        for (int64_t k = b; k < e; ++k)
+          if (tgt[k] != u) tmp.push_back({tgt[k], k});
+        std::sort(tmp.begin(), tmp.end());
+        ws.clear();
+        int64_t o = b;
+        for (size_t i = 0; i < tmp.size(); ++i) {
+          double wv = wt[tmp[i].second];
+          if (i > 0 && tmp[i].first == tmp[i - 1].first) {
+            ws.back() += wv;
+          } else {
+            tgt[o] = tmp[i].first;
+            ws.push_back(wv);
+            ++o;
+          }
         }
+        for (size_t i = 0; i < ws.size(); ++i) wt[b + i] = ws[i];
+        deg[u + 1] = o - b;
       }
-      for (size_t i = 0; i < ws.size(); ++i) wt[b + i] = ws[i];
-      deg[u + 1] = o - b;
     }
     g.n = n;
     g.ptr.assign((size_t)n + 1, 0);
@@ -179,8 +209,8 @@ struct Builder {
       std::memcpy(&g.idx[g.ptr[u]], &tgt[off[u]], sizeof(int32_t) * d);
       std::memcpy(&g.w[g.ptr[u]], &wt[off[u]], sizeof(double) * d);
     }
-    std::vector<int32_t>().swap(tgt);
-    std::vector<double>().swap(wt);
+    big_vector<int32_t>().swap(tgt);
+    big_vector<double>().swap(wt);
   }
 };
 
@@ -229,6 +259,27 @@ int community_model(int32_t n, double deg_exp, int32_t deg_min, int32_t deg_max,
       cdf[p] = run;
     }
   }
+  // guide table of the in-weight cdf: guide[k] lies at or below the first
+  // position whose cdf exceeds k * total / n, so a draw starts its search
+  // there and walks a step or two instead of a binary search over all n
+  // (the result is exactly std::upper_bound's)
+  std::vector<int32_t> guide;
+  if (chung_lu) {
+    guide.resize(n);
+    const double tot = cdf[n - 1];
+#pragma omp parallel for schedule(static)
+    for (int32_t k = 0; k < n; ++k)
+      guide[k] = (int32_t)(std::upper_bound(cdf.begin(), cdf.end(), (double)k / n * tot) - cdf.begin());
+  }
+  auto upper = [&](double x) -> int64_t {
+    const double tot = cdf[n - 1];
+    int64_t k = (int64_t)(x / tot * n);
+    k = k < 0 ? 0 : (k >= n ? n - 1 : k);
+    int64_t t = guide[k];
+    while (t > 0 && cdf[t - 1] > x) --t;
+    while (t < n && cdf[t] <= x) ++t;
+    return t;
+  };
   Builder B(n);
   std::vector<int32_t> dpos(n);
 #pragma omp parallel for schedule(static)
@@ -261,7 +312,7 @@ int community_model(int32_t n, double deg_exp, int32_t deg_min, int32_t deg_max,
           double base = cs > 0 ? cdf[cs - 1] : 0.0;
           do {
             double x = base + r.uniform() * (cdf[ce - 1] - base);
-            t = std::upper_bound(cdf.begin() + cs, cdf.begin() + ce, x) - cdf.begin();
+            t = std::max<int64_t>(upper(x), cs);   // = upper_bound over [cs, ce): cdf[cs - 1] = base <= x
             if (t >= ce) t = ce - 1;
           } while (t == p);
         } else {
@@ -271,7 +322,7 @@ int community_model(int32_t n, double deg_exp, int32_t deg_min, int32_t deg_max,
         if (chung_lu) {
           do {
             double x = r.uniform() * total_w;
-            t = std::upper_bound(cdf.begin(), cdf.end(), x) - cdf.begin();
+            t = upper(x);
             if (t >= n) t = n - 1;
           } while (t >= cs && t < ce);
         } else {
