@@ -282,15 +282,16 @@ def config3(args):
     return out
 
 
-TRAFFIC = os.path.join(ROOT, "profiles", "sweep_flow_kernels_chunglu32m_traffic.txt")
+TRAFFIC = {"mixed": os.path.join(ROOT, "profiles", "step_kernels_chunglu32m_mixed_traffic.txt"),
+           "fp64": os.path.join(ROOT, "profiles", "step_kernels_chunglu32m_fp64_traffic.txt")}
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, precision="mixed"):
     """HBM bytes per dispatch of `kernel` on the 32M-node workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated by a plain
-    copy of known size), or None."""
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of that precision
+    (calibrated by a plain copy of known size), or None."""
     try:
-        lines = open(TRAFFIC).read().splitlines()
+        lines = open(TRAFFIC[precision]).read().splitlines()
     except OSError:
         return None
     ff = wf = None
@@ -380,20 +381,36 @@ def config4_reference(graphs, res, args):
     return out
 
 
-def roofline_of(stats, default_workload):
+def roofline_of(stats, default_workload, precision="mixed"):
+    """The roofline of the kernel with the most time in the profiled step
+    (every entry of the profiling mode is one HIP kernel), and the table of
+    every kernel with at least 1 % of the profiled kernel time: launches,
+    time, algorithmic GB/s and, on the default workload, the measured HBM
+    traffic per launch."""
     if not stats:
         return None, {}
-    table = {k: {"launches": n, "avg_ms": ms / max(n, 1), "total_ms": ms,
-                 "achieved_GBs": (b / max(n, 1)) / (ms / max(n, 1) * 1e-3) / 1e9 if ms > 0 else 0.0}
-             for k, (n, ms, b) in stats.items()}
-    single = [k for k in stats if not k.startswith("hub sequence")]   # one HIP kernel per entry
-    name = max(single, key=lambda k: stats[k][1])
-    n, ms, b = stats[name]
+    single = {k: v for k, v in stats.items() if "sequence" not in k}
+    total = sum(ms for _, ms, _ in single.values())
+    table = {}
+    for k, (n, ms, b) in single.items():
+        if ms < 0.01 * total:
+            continue
+        row = {"launches": n, "avg_ms": ms / max(n, 1), "total_ms": ms, "share": ms / total,
+               "bytes_per_launch": b / max(n, 1),
+               "achieved_GBs": (b / max(n, 1)) / (ms / max(n, 1) * 1e-3) / 1e9 if ms > 0 else 0.0}
+        tr = measured_traffic(k, precision) if default_workload else None
+        if tr:
+            row["traffic_per_launch"] = tr
+            row["traffic_over_algorithmic"] = tr / max(row["bytes_per_launch"], 1.0)
+        table[k] = row
+    name = max(single, key=lambda k: single[k][1])
+    n, ms, b = single[name]
     achieved = (b / n) / (ms / n * 1e-3) / 1e9
-    traffic = measured_traffic(name) if default_workload else None
+    traffic = measured_traffic(name, precision) if default_workload else None
     roof = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "bytes_per_launch": b / n, "avg_launch_ms": ms / n, "launches": n}
+            "bytes_per_launch": b / n, "avg_launch_ms": ms / n, "launches": n,
+            "share_of_kernel_time": round(ms / total, 4)}
     if traffic:
         roof["traffic_over_algorithmic"] = round(traffic / (b / n), 2)
     return roof, table
@@ -410,6 +427,8 @@ def main():
                     help="sub-sweeps per sweep; 0: one per 32768 units of a level, 4 to 8")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="only the FP64 path (for its kernel trace); value is then the FP64 path's")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
     ap.add_argument("--no-config4", action="store_true", help="skip the 512-instance batch measurement")
@@ -432,7 +451,11 @@ def main():
     g = synth.config5(n=args.nodes, mean_deg=args.mean_deg)
     log(f"generated Chung-Lu network: {g.n} nodes, {g.m} arcs in {time.perf_counter() - t:.1f} s")
 
-    net_m, rec_m = run_path(g, "mixed", args.steps, args.warmup, args)
+    if args.no_mixed:   # the FP64 path alone, reported in the mixed path's place
+        args.no_fp64 = True
+        net_m, rec_m = run_path(g, "fp64", args.steps, args.warmup, args)
+    else:
+        net_m, rec_m = run_path(g, "mixed", args.steps, args.warmup, args)
     p_mixed = net_m.flows()
     part_m = net_m.partition()
     net_m.close()
@@ -449,7 +472,8 @@ def main():
     step_ms = 1e3 * sum(r["step_s"] for r in rec_m) / len(rec_m)
     out = {"metric": METRIC, "value": round(gteps, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(step_ms, 2), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32 arcs + f64/int64 accumulation (mixed)", "data": "synthetic",
+           "vs_baseline": None,
+           "dtype": "f64" if args.no_mixed else "f32 arcs + f64/int64 accumulation (mixed)", "data": "synthetic",
            "config": {"workload": "directed Chung-Lu with planted communities (500M-arc configuration)",
                       "nodes": g.n, "arcs": g.m, "degree_exponent": 2.1, "degree_range": [4, 100000],
                       "community_exponent": 1.2, "community_range": [50, 50000], "mu": 0.4,
@@ -475,11 +499,19 @@ def main():
         if g.community[0] >= 0:
             out["nmi_planted"] = {"mixed": nmi(part_m, g.community), "fp64": nmi(part_d, g.community)}
     if not args.no_profile:
-        stats = profile_pass(g, "mixed", args)
-        roof, table = roofline_of(stats, args.nodes == DEFAULT_NODES and args.mean_deg == DEFAULT_MEAN_DEG)
-        out["roofline"] = roof
-        out["kernels"] = {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in
-                          sorted(table.items(), key=lambda kv: -kv[1]["total_ms"])[:12]}
+        default_workload = args.nodes == DEFAULT_NODES and args.mean_deg == DEFAULT_MEAN_DEG
+        precs = ["fp64"] if args.no_mixed else (["mixed"] + ([] if args.no_fp64 else ["fp64"]))
+        for prec in precs:
+            stats = profile_pass(g, prec, args)
+            roof, table = roofline_of(stats, default_workload, prec)
+            kern = {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in
+                    sorted(table.items(), key=lambda kv: -kv[1]["total_ms"])}
+            if prec == precs[0]:
+                out["roofline"] = roof
+                out["kernels"] = kern
+            else:
+                out["roofline_" + prec] = roof
+                out["kernels_" + prec] = kern
     del g
     if not args.no_config3:
         out["config3"] = config3(args)

@@ -182,7 +182,25 @@ struct Profiler {
   float stop(hipStream_t s);
   // stop() and account the launch under `name`
   void end(hipStream_t s, const std::string &name, double bytes);
+  // account a launch already timed with begin() / stop()
+  void add(const std::string &name, float ms, double bytes) {
+    KernelStat &k = get(name);
+    k.launches += 1;
+    k.ms += ms;
+    k.bytes += bytes;
+  }
 };
+
+// Profiling mode: one launch bracketed by events on the context's stream
+// (the byte count is evaluated only when profiling).
+#define RMX_PROF_BEGIN(c)                      \
+  do {                                         \
+    if ((c).prof.on) (c).prof.begin((c).stream); \
+  } while (0)
+#define RMX_PROF_END(c, name, bytes)                             \
+  do {                                                           \
+    if ((c).prof.on) (c).prof.end((c).stream, (name), (double)(bytes)); \
+  } while (0)
 
 struct Ctx {
   int device = 0;

@@ -287,10 +287,14 @@ void contract(Ctx &c) {
   uint32_t m_inter = 0;
   if (m > 0) {
     const bool carry = c.precision == 1 && !from_super;   // node level, mixed: the sort carries the FP32 flows
+    RMX_PROF_BEGIN(c);
     k_arc_keys<16><<<grid_for((int64_t)ns_src * 16, 256), 256, 0, s>>>(S.out_ptr.p, S.out_idx.p, ns_src, usp, bits,
                                                                         carry ? S.out_wf.p : nullptr, keys.p, arcs.p);
     RMX_LAUNCH_CHECK();
-    radix_sort_pairs_u64(keys.p, arcs.p, m, 0, 2 * bits, c.scratch, s);
+    // per arc: head id, its super node (gathered), key and value written,
+    // the FP32 flow when carried; per unit: bounds and its super node
+    RMX_PROF_END(c, "k_arc_keys<16>", (double)m * (4 + 4 + 8 + 4 + (carry ? 4 : 0)) + (double)ns_src * 12);
+    radix_sort_pairs_u64(keys.p, arcs.p, m, 0, 2 * bits, c.scratch, s, c.prof.on ? &c.prof : nullptr);
     k_count_inter<<<1, 1, 0, s>>>(keys.p, m, atot.p);
     RMX_LAUNCH_CHECK();
     RMX_HIP(hipMemcpyAsync(&m_inter, atot.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -324,10 +328,16 @@ void contract(Ctx &c) {
   twx.alloc(m_super);
 
   if (m_super > 0) {
+    RMX_PROF_BEGIN(c);
     k_reduce_segments<<<grid_for(m_super, 256), 256, 0, s>>>(
         kstart.p, m_super, m_inter, keys.p, arcs.p, S.out_w.p, from_super ? S.out_wx.p : nullptr,
         c.precision == 1 ? 1 : 0, bits, D.out_idx.p, D.out_w.p, D.out_wx.p, tkey.p, twx.p);
     RMX_LAUNCH_CHECK();
+    // per inter arc: its value and the weights it names (none when the sort
+    // carried the FP32 flow, FP64 + fixed point from a super level, FP64
+    // from the node level); per super arc: two starts, its key, five outputs
+    const double per_arc = 4.0 + ((c.precision == 1 && !from_super) ? 0.0 : (from_super ? 16.0 : 8.0));
+    RMX_PROF_END(c, "k_reduce_segments", (double)m_inter * per_arc + (double)m_super * (8 + 8 + 4 + 8 + 8 + 8 + 8));
   }
   D.out_ptr.alloc((int64_t)ns + 1);
   if (m_super > 0) {
@@ -349,10 +359,13 @@ void contract(Ctx &c) {
   D.in_wx.alloc(m_super);
   if (m_super > 0) {
     uint64_t *sk = nullptr, *sw = nullptr;
-    radix_sort_pairs_u64_u64(tkey.p, twx.p, m_super, bits, 2 * bits, c.scratch, s, &sk, &sw);
+    radix_sort_pairs_u64_u64(tkey.p, twx.p, m_super, bits, 2 * bits, c.scratch, s, &sk, &sw,
+                             c.prof.on ? &c.prof : nullptr);
+    RMX_PROF_BEGIN(c);
     k_in_from_keys<<<(unsigned)std::min<int64_t>(16384, (m_super + 255) / 256), 256, 0, s>>>(sk, sw, m_super, bits,
                                                                                             D.in_idx.p, D.in_wx.p);
     RMX_LAUNCH_CHECK();
+    RMX_PROF_END(c, "k_in_from_keys", (double)m_super * (8 + 8 + 4 + 8));
     k_in_ptr<<<(unsigned)std::min<int64_t>(16384, (m_super + 256) / 256), 256, 0, s>>>(sk, bits, m_super, (int32_t)ns,
                                                                                       D.in_ptr.p);
     RMX_LAUNCH_CHECK();

@@ -1507,6 +1507,31 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
   if ((threadIdx.x & (kWave - 1)) == 0 && moved_here) atomicAdd(my_stripe(stripes) + cMoved, moved_here);
 }
 
+// Profiling only: arcs of a batch's movers, split at big_deg: out-arcs and
+// all arcs of the movers at or below it (k_commit's), then of those above
+// it (k_commit_big's).
+__global__ __launch_bounds__(256) void k_mover_arcs(const uint32_t *__restrict__ movers,
+                                                    const uint32_t *__restrict__ counters,
+                                                    const uint32_t *__restrict__ optr,
+                                                    const uint32_t *__restrict__ iptr, uint32_t big_deg,
+                                                    unsigned long long *__restrict__ out) {
+  const uint32_t nm = counters[cMovers];
+  unsigned long long a[4] = {0, 0, 0, 0};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nm; i += gridDim.x * blockDim.x) {
+    const uint32_t u = movers[i];
+    const uint32_t od = optr[u + 1] - optr[u];
+    const uint32_t d = od + (iptr[u + 1] - iptr[u]);
+    const int hi = d > big_deg ? 2 : 0;
+    a[hi] += od;
+    a[hi + 1] += d;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const unsigned long long v = wave_sum(a[k]);
+    if ((threadIdx.x & (kWave - 1)) == 0 && v) atomicAdd(&out[k], v);
+  }
+}
+
 // Arc chunks of the deferred (high-degree) movers: prefix of chunk counts.
 constexpr int kCommitChunk = 4096;
 // smallest batch (units) whose marks may be pulled
@@ -1920,6 +1945,59 @@ static void profile_eval(Ctx &c, int bucket, hipStream_t s) {
   k.bytes += visits * (8.0 + wbytes) + units * 48.0 + cands * 32.0;
 }
 
+// Algorithmic bytes of the hub partition kernels of one batch, from its hub
+// arc visits, hubs and candidate modules (counted on the device) and its arc
+// chunks and partitions:
+//   plan    per hub: list entry, unit, bounds, three prefix entries (40)
+//   owners  per chunk and partition its hub (4), per hub its prefixes (16)
+//   count   per arc: neighbour id, its module (gathered), module list entry (12)
+//   scatter per arc: module list entry, weight, partitioned key and fixed-point flow (16 + w)
+//   eval    per arc: key and flow (12); per candidate its statistics and unit count (36);
+//           per partition: bounds, the unit's own reads and its best (8 + 48 + 28)
+//   finish  per partition its best (12), per hub its own reads and flows (64)
+static void profile_hubs(Ctx &c, const HubParts &H, const float *tk, bool f32, hipStream_t s) {
+  uint32_t before[16];
+  std::memcpy(before, c.h_cnt, sizeof(before));
+  fold_counters(c, s);
+  uint32_t tot[3] = {0, 0, 0};
+  RMX_HIP(hipMemcpyAsync(c.h_cnt, c.counters.p, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, s));
+  RMX_HIP(hipMemcpyAsync(tot, H.tot, sizeof(tot), hipMemcpyDeviceToHost, s));
+  c.sync();
+  auto u64 = [](const uint32_t *h, int i) { return (double)((uint64_t)h[i] | ((uint64_t)h[i + 1] << 32)); };
+  const double arcs = u64(c.h_cnt, cVisitsLo) - u64(before, cVisitsLo);
+  const double cands = u64(c.h_cnt, cCandsLo) - u64(before, cCandsLo);
+  const double hubs = (double)c.h_cnt[cUnits] - (double)before[cUnits];
+  const double chunks = tot[0], parts = tot[1];
+  const double w = (c.level == 0 && c.precision == 1) ? 4.0 : 8.0;
+  c.prof.add("k_hubp_plan", tk[0], hubs * 40.0);
+  c.prof.add("k_hubp_owners", tk[1], (chunks + parts) * 4.0 + hubs * 16.0);
+  c.prof.add("k_hubp_count", tk[2], arcs * 12.0);
+  c.prof.add("k_hubp_scatter", tk[3], arcs * (16.0 + w));
+  c.prof.add(f32 ? "k_hubp_eval<true>" : "k_hubp_eval<false>", tk[4], arcs * 12.0 + cands * 36.0 + parts * 84.0);
+  c.prof.add("k_hubp_finish", tk[5], parts * 12.0 + hubs * 64.0);
+}
+
+// k_commit's and k_commit_big's algorithmic bytes of one batch:
+//   per mover: id, CSR bounds, stamp, old and new module, its flows to
+//   them (32), its statistics (32) and member count, and the read-modify-
+//   write of both module rows and member counts (160): 260 B;
+//   per out-arc: the head and its stamp (8); per arc, when the marks are
+//   pushed: the neighbour id and its mark (5); per mover with pulled marks
+//   its flag (1).
+struct CommitProfile {
+  float ms = 0;
+  unsigned long long arcs[4] = {0, 0, 0, 0};   // k_mover_arcs: small out / all, big out / all
+};
+static void profile_commit_counts(Ctx &c, Level &L, uint32_t big_deg, CommitProfile &cp, hipStream_t s) {
+  DevBuf<unsigned long long> out(c.cache);
+  out.alloc(4);
+  fill_i64((long long *)out.p, 0, 4, s);
+  k_mover_arcs<<<1024, 256, 0, s>>>(c.movers.p, c.counters.p, L.out_ptr.p, L.in_ptr.p, big_deg, out.p);
+  RMX_LAUNCH_CHECK();
+  RMX_HIP(hipMemcpyAsync(cp.arcs, out.p, sizeof(cp.arcs), hipMemcpyDeviceToHost, s));
+  c.sync();
+}
+
 void ensure_sweep_buffers(Ctx &c) {
   const int32_t n = c.N;
   if (c.target.n >= n) return;
@@ -2105,6 +2183,7 @@ static uint32_t eval_batch(Ctx &c, Level &L, EvalArgs &a, const std::vector<uint
     units_b += (uint32_t)cnt;
     a.list = c.batch_list.p + seg[key];
     a.list_n = cnt;
+    bool hub_profiled = false;   // the partitioned hub kernels account themselves
     if (c.prof.on) c.prof.begin(s);
     switch (bk) {
       case 0: case 1: case 2: case 3: case 4: case 5: case 6:
@@ -2143,25 +2222,46 @@ static uint32_t eval_batch(Ctx &c, Level &L, EvalArgs &a, const std::vector<uint
         H.hubu = c.hp_hubu.p;
         H.chub = c.hp_chub.p;
         H.phub = c.hp_phub.p;
+        // profiling: each kernel of the sequence timed on its own
+        float tk[6] = {0, 0, 0, 0, 0, 0};
+        auto t0 = [&]() { if (c.prof.on) c.prof.begin(s); };
+        auto t1 = [&](int k) { if (c.prof.on) tk[k] = c.prof.stop(s); };
+        t0();
         k_hubp_plan<<<1, kPlanBlock, 0, s>>>(a, H);
+        t1(0);
+        t0();
         k_hubp_owners<<<grid_for((int64_t)cnt * kWave, 256), 256, 0, s>>>(a, H);
+        t1(1);
         fill_u32(H.bcnt, 0u, NB + 1, s);
         fill_i64(H.own, 0, 2 * (int64_t)cnt, s);
         fill_u32(H.ovf, 0u, cnt, s);
+        t0();
         k_hubp_count<<<4096, 256, 0, s>>>(a, H);
+        t1(2);
         exclusive_scan_u32(H.bcnt, H.boff, NB + 1, nullptr, c.scratch, s);
         fill_u32(H.bcnt, 0u, NB + 1, s);
+        t0();
         k_hubp_scatter<<<4096, 256, 0, s>>>(a, H);
+        t1(3);
+        t0();
         if (f32) k_hubp_eval<true><<<8192, 256, 0, s>>>(a, H);
         else k_hubp_eval<false><<<8192, 256, 0, s>>>(a, H);
+        t1(4);
+        t0();
         k_hubp_finish<<<(unsigned)std::min<int64_t>(1024, (cnt + 3) / 4), 256, 0, s>>>(a, H);
+        t1(5);
+        RMX_LAUNCH_CHECK();
+        if (c.prof.on) {
+          profile_hubs(c, H, tk, f32, s);
+          hub_profiled = true;
+        }
         hub_list = a.list;
         hub_n = cnt;
         break;
       }
     }
     RMX_LAUNCH_CHECK();
-    if (c.prof.on) profile_eval(c, bk, s);
+    if (c.prof.on && !hub_profiled) profile_eval(c, bk, s);
   }
   return units_b;
 }
@@ -2280,20 +2380,33 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     // marks by pull when at least half of a large batch moves
     const uint32_t pull_min = prioritized && units_b >= kPullMinUnits ? (units_b + 1) / 2 : 0xffffffffu;
     if (pull_min != 0xffffffffu && (quiet || n_movers >= pull_min)) maybe_pull = true;
+    CommitProfile cprof;
+    if (c.prof.on) c.prof.begin(s);
     k_commit<16><<<stride_grid((int64_t)n_movers * 16, 256), 256, 0, s>>>(
         c.movers.p, c.counters.p, L.out_ptr.p, L.out_idx.p, L.in_ptr.p, L.in_idx.p, aout, L.mod.p, c.newmod.p,
         c.stamp.p, c.stamp_now, c.agg_buf.p, L.us.p, L.nmem.p, c.ms.p, c.mNodes.p, c.mUnits.p, c.act_next.p,
         prioritized ? 1 : 0, c.counters.p + kStripeBase, big_deg, c.big_list.p, c.counters.p + cBig, pull_min,
         c.wmark.p, c.counters.p + cPull);
     RMX_LAUNCH_CHECK();
+    const bool pushed_marks = prioritized && n_movers < pull_min;
+    if (c.prof.on) {
+      cprof.ms = c.prof.stop(s);
+      profile_commit_counts(c, L, big_deg, cprof, s);
+      const double pulled = (prioritized && !pushed_marks) ? (double)n_movers : 0.0;
+      c.prof.add("k_commit<16>", cprof.ms,
+                 (double)n_movers * 260.0 + (double)cprof.arcs[0] * 8.0 +
+                     (pushed_marks ? (double)cprof.arcs[1] * 5.0 : 0.0) + pulled);
+    }
     if (n_hubs > 0) {
       // only hubs (more than big_deg arcs) are deferred: the list holds at most n_hubs
       k_commit_plan<<<1, 256, 0, s>>>(c.big_list.p, c.counters.p + cBig, L.out_ptr.p, L.in_ptr.p, c.big_plan.p);
       RMX_LAUNCH_CHECK();
+      RMX_PROF_BEGIN(c);
       k_commit_big<<<2048, 256, 0, s>>>(c.big_list.p, c.counters.p + cBig, c.big_plan.p, L.out_ptr.p, L.out_idx.p,
                                         L.in_ptr.p, L.in_idx.p, aout, L.mod.p, c.newmod.p, c.stamp.p, c.stamp_now,
                                         c.ms.p, c.act_next.p, prioritized ? 1 : 0, c.counters.p, pull_min);
       RMX_LAUNCH_CHECK();
+      RMX_PROF_END(c, "k_commit_big", (double)cprof.arcs[2] * 8.0 + (pushed_marks ? (double)cprof.arcs[3] * 5.0 : 0.0));
     }
     if (single) {
       k_qv_touched<<<stride_grid(n_movers, 256), 256, 0, s>>>(c.movers.p, c.counters.p, c.stamp.p, c.stamp_now,

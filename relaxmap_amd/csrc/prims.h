@@ -33,14 +33,19 @@ void exclusive_scan_reserve(int64_t n, Scratch &scratch);
 // Stable LSD radix sort of (key, val) pairs on bits [begin_bit, end_bit).
 // keys/vals are sorted in place; tmp buffers of n elements are taken from
 // scratch.  Stability: equal keys keep their input order.
+// With `prof`, every launch of the sort is timed and accounted under its
+// kernel's name (bytes: the keys read by the digit count, the keys and
+// values read and written by each pass).
+struct Profiler;
 void radix_sort_pairs_u32(uint32_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit,
-                          Scratch &scratch, hipStream_t s);
+                          Scratch &scratch, hipStream_t s, Profiler *prof = nullptr);
 void radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit,
-                          Scratch &scratch, hipStream_t s);
+                          Scratch &scratch, hipStream_t s, Profiler *prof = nullptr);
 // Leaves the result where the last pass wrote it (the inputs or the scratch,
 // valid until the scratch is used again): *out_keys / *out_vals.
 void radix_sort_pairs_u64_u64(uint64_t *keys, uint64_t *vals, int64_t n, int begin_bit, int end_bit,
-                              Scratch &scratch, hipStream_t s, uint64_t **out_keys, uint64_t **out_vals);
+                              Scratch &scratch, hipStream_t s, uint64_t **out_keys, uint64_t **out_vals,
+                              Profiler *prof = nullptr);
 
 void fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
 void fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s);

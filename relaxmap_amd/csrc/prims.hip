@@ -411,9 +411,16 @@ SortLayout sort_layout(int64_t n, int passes) {
   return l;
 }
 
+template <typename T>
+const char *type_name();
+template <>
+const char *type_name<uint32_t>() { return "unsigned int"; }
+template <>
+const char *type_name<uint64_t>() { return "unsigned long"; }
+
 template <typename K, typename V>
 void radix_sort_impl(K *keys, V *vals, int64_t n, int begin_bit, int end_bit, Scratch &scratch, hipStream_t s,
-                     K **out_k = nullptr, V **out_v = nullptr) {
+                     K **out_k = nullptr, V **out_v = nullptr, Profiler *prof = nullptr) {
   if (out_k) *out_k = keys;
   if (out_v) *out_v = vals;
   if (n <= 1 || begin_bit >= end_bit) return;
@@ -432,17 +439,23 @@ void radix_sort_impl(K *keys, V *vals, int64_t n, int begin_bit, int end_bit, Sc
   uint32_t *ctr = counts + passes * kRadix;
   uint32_t *err = ctr + passes;
   fill_u32(counts, 0u, n_small, s);
+  if (prof) prof->begin(s);
   k_sort_hist<K><<<(unsigned)std::min<int64_t>(n_tiles, 2048), kSortBlock, 0, s>>>(keys, n, begin_bit, passes, counts);
   RMX_LAUNCH_CHECK();
+  if (prof) prof->end(s, std::string("k_sort_hist<") + type_name<K>() + ">", (double)n * sizeof(K));
   k_sort_digit_scan<<<(unsigned)passes, kRadix, 0, s>>>(counts);
   RMX_LAUNCH_CHECK();
   K *src_k = keys, *dst_k = tk;
   V *src_v = vals, *dst_v = tv;
   for (int q = 0; q < passes; ++q) {
     fill_u32((uint32_t *)look, 0u, 2 * n_tiles * kRadix, s);
+    if (prof) prof->begin(s);
     k_sort_onesweep<K, V><<<(unsigned)n_tiles, kSortBlock, 0, s>>>(src_k, src_v, dst_k, dst_v, n, begin_bit + 8 * q,
                                                                    counts + q * kRadix, look, ctr + q, err);
     RMX_LAUNCH_CHECK();
+    if (prof)
+      prof->end(s, std::string("k_sort_onesweep<") + type_name<K>() + ", " + type_name<V>() + ">",
+                2.0 * (double)n * (sizeof(K) + sizeof(V)));
     std::swap(src_k, dst_k);
     std::swap(src_v, dst_v);
   }
@@ -501,18 +514,19 @@ size_t radix_sort_scratch_bytes(int64_t n, int key_bytes, int val_bytes, int bit
 }
 
 void radix_sort_pairs_u32(uint32_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit,
-                          Scratch &scratch, hipStream_t s) {
-  radix_sort_impl<uint32_t, uint32_t>(keys, vals, n, begin_bit, end_bit, scratch, s);
+                          Scratch &scratch, hipStream_t s, Profiler *prof) {
+  radix_sort_impl<uint32_t, uint32_t>(keys, vals, n, begin_bit, end_bit, scratch, s, nullptr, nullptr, prof);
 }
 
 void radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, int64_t n, int begin_bit, int end_bit,
-                          Scratch &scratch, hipStream_t s) {
-  radix_sort_impl<uint64_t, uint32_t>(keys, vals, n, begin_bit, end_bit, scratch, s);
+                          Scratch &scratch, hipStream_t s, Profiler *prof) {
+  radix_sort_impl<uint64_t, uint32_t>(keys, vals, n, begin_bit, end_bit, scratch, s, nullptr, nullptr, prof);
 }
 
 void radix_sort_pairs_u64_u64(uint64_t *keys, uint64_t *vals, int64_t n, int begin_bit, int end_bit,
-                              Scratch &scratch, hipStream_t s, uint64_t **out_keys, uint64_t **out_vals) {
-  radix_sort_impl<uint64_t, uint64_t>(keys, vals, n, begin_bit, end_bit, scratch, s, out_keys, out_vals);
+                              Scratch &scratch, hipStream_t s, uint64_t **out_keys, uint64_t **out_vals,
+                              Profiler *prof) {
+  radix_sort_impl<uint64_t, uint64_t>(keys, vals, n, begin_bit, end_bit, scratch, s, out_keys, out_vals, prof);
 }
 
 void fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s) {
