@@ -68,6 +68,7 @@ def lib():
             "orc_reset_active": (None, [vp, C.c_int]),
             "orc_n_active": (C.c_int, [vp]),
             "orc_run": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+            "orc_subnetwork": (vp, [vp, C.c_int, _I32P]),
             "orc_evaluate_all": (None, [vp, C.c_int, C.c_int, _I32P, _F64P]),
             "orc_candidates": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, _I32P, _F64P]),
             "orc_trace_len": (C.c_int, [vp]),
@@ -232,6 +233,30 @@ class OracleNet:
 
     def sweep(self, variant="move", sp=False, vthresh=0.0):
         return self._L.orc_sweep(self._h, VARIANTS[variant], 1 if sp else 0, float(vthresh))
+
+    def subnetwork(self, members):
+        """generate_network_from_module: the network of the given nodes (ascending
+        ids) with this network's flows, in singleton modules."""
+        members = np.ascontiguousarray(members, dtype=np.int32)
+        h = self._L.orc_subnetwork(self._h, len(members), _p(members, _I32P))
+        if not h:
+            raise ValueError("bad member list")
+        sub = OracleNet(h)
+        sub._parent = self   # names and flows were copied; the parent may go
+        return sub
+
+    def partition_module_network(self, threshold=1e-3, max_iter=10, seed=1):
+        """partition_module_network(fast) at one thread (OmpRelaxmap.cpp:436-460):
+        move() sweeps until one gains less than `threshold` bits, at most
+        max_iter; returns the partition."""
+        import math
+        self.seed(seed)
+        for _ in range(max_iter):
+            old = self.codelength
+            self.sweep("move")
+            if (old - self.codelength) / math.log(2.0) < threshold:
+                break
+        return self.partition()
 
     def evaluate_all(self, sp=False, no_singleton_swaps=False):
         """Every unit's best move against one module snapshot (nothing moves):

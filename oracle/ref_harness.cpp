@@ -20,6 +20,9 @@
 //   run        <net> <seed> <thr> <vthr> <maxIter> <prior|normal>
 //                                          main()'s super-step loop, stopping
 //                                          before the first coarse tune
+//   write      <net> <part> <dir> <name>  the reference's .tree writer
+//                                          (print_twoLevel_Cluster) on an
+//                                          installed partition: <dir>/<name>.tree
 // <part> is a text file with one 0-based module index per node.
 
 #include <cmath>
@@ -39,6 +42,7 @@
 void stochastic_greedy_partition(Network &network, int numTh, double threshold,
                                  double vThresh, int maxIter, bool prior,
                                  bool fineTune, bool fast);
+void print_twoLevel_Cluster(Network network, std::string networkName, std::string outDir);
 
 static std::stringstream g_sink;
 static std::streambuf *g_saved = 0;
@@ -317,6 +321,14 @@ int main(int argc, char **argv) {
     std::printf(", ");
     print_iarray("partition", part);
     std::printf("}\n");
+  } else if (cmd == "write" && argc >= 6) {
+    // OmpRelaxmap.cpp:785-808 on an installed partition (its codelength and
+    // module exits from updateCodeLength, its member lists ascending)
+    int n = setup(net, file, 1, 0);
+    install_partition(net, read_part(argv[3], n));
+    print_twoLevel_Cluster(net, argv[5], argv[4]);
+    std::printf("{\"codelength\": "); pd(net.CodeLength());
+    std::printf(", \"n_module\": %d}\n", net.NModule());
   } else {
     std::fprintf(stderr, "bad command\n");
     return 2;

@@ -522,6 +522,8 @@ static void calibrate(orc_net *net) {
   net->codelength = plogp(se) - 2.0 * sel + ssl - net->all_log_all;
 }
 
+static void init_modules(orc_net *net);
+
 int orc_initiate(orc_net *net) {
   level_t *L = &net->L;
   int n = net->n;
@@ -580,6 +582,15 @@ int orc_initiate(orc_net *net) {
   double all = 0.0;
   for (int i = 0; i < n; i++) all += plogp(L->size[i]);
   net->all_log_all = all;
+  init_modules(net);
+  return 0;
+}
+
+/* Singleton modules, Module::Module(int, Node*) (Module.cpp:67-99), then
+ * calibrate, the in-links and the sweep state: the end of initiate. */
+static void init_modules(orc_net *net) {
+  level_t *L = &net->L;
+  int n = net->n;
   /* singleton modules, Module::Module(int, Node*) (Module.cpp:67-99) */
   net->m_exit = (double *)xcalloc(n, sizeof(double));
   net->m_stay = (double *)xcalloc(n, sizeof(double));
@@ -623,7 +634,77 @@ int orc_initiate(orc_net *net) {
   net->active = (int32_t *)xcalloc(n, sizeof(int32_t));
   mt_seed(&net->mt, 1u);
   orc_reset_active(net, 0);
-  return 0;
+}
+
+/* generate_network_from_module (OmpRelaxmap.cpp:640-695): the network of
+ * members[0..k) (in the order given: the reference lists a module's members
+ * in ascending node id) with the parent's flows, teleport weights and
+ * dangling flags and the parent's arc flows between members, in stored
+ * order; then singleton modules over those arcs only and calibrate.  NULL
+ * for an id out of range or repeated. */
+orc_net *orc_subnetwork(const orc_net *parent, int k, const int32_t *members) {
+  const level_t *P = &parent->L;
+  if (k <= 0 || !P->size) return NULL;
+  int32_t *newid = (int32_t *)xcalloc(parent->n, sizeof(int32_t));
+  for (int i = 0; i < parent->n; i++) newid[i] = -1;
+  for (int i = 0; i < k; i++) {
+    if (members[i] < 0 || members[i] >= parent->n || newid[members[i]] >= 0) { free(newid); return NULL; }
+    newid[members[i]] = i;
+  }
+  orc_net *net = net_alloc(k);
+  level_t *L = &net->L;
+  L->n = k;
+  L->out_ptr = (int64_t *)xcalloc(k + 1, sizeof(int64_t));
+  for (int i = 0; i < k; i++) {
+    int a = members[i];
+    for (int64_t e = P->out_ptr[a]; e < P->out_ptr[a + 1]; e++)
+      if (newid[P->out_idx[e]] >= 0) L->out_ptr[i + 1]++;
+  }
+  for (int i = 0; i < k; i++) L->out_ptr[i + 1] += L->out_ptr[i];
+  int64_t m = L->out_ptr[k];
+  L->m = m;
+  net->m = m;
+  L->out_idx = (int32_t *)xcalloc(m > 0 ? m : 1, sizeof(int32_t));
+  L->out_w = (double *)xcalloc(m > 0 ? m : 1, sizeof(double));
+  net->raw_w = (double *)xcalloc(m > 0 ? m : 1, sizeof(double));
+  int64_t o = 0;
+  for (int i = 0; i < k; i++) {
+    int a = members[i];
+    for (int64_t e = P->out_ptr[a]; e < P->out_ptr[a + 1]; e++)
+      if (newid[P->out_idx[e]] >= 0) {
+        L->out_idx[o] = newid[P->out_idx[e]];
+        L->out_w[o] = P->out_w[e];
+        net->raw_w[o] = P->out_w[e];
+        o++;
+      }
+  }
+  L->tp = (double *)xcalloc(k, sizeof(double));
+  L->size = (double *)xcalloc(k, sizeof(double));
+  L->dsize = (double *)xcalloc(k, sizeof(double));
+  L->exitpr = (double *)xcalloc(k, sizeof(double));
+  L->dangling = (char *)xcalloc(k, 1);
+  L->mod = (int32_t *)xcalloc(k, sizeof(int32_t));
+  net->danglings = (int32_t *)xcalloc(k, sizeof(int32_t));
+  double tot = 0.0;
+  for (int i = 0; i < k; i++) {
+    int a = members[i];
+    free(net->names[i]);
+    net->names[i] = parent->names && parent->names[a] ? strdup(parent->names[a]) : NULL;
+    net->node_w[i] = parent->node_w[a];
+    tot += parent->node_w[a];
+    L->tp[i] = P->tp[a];
+    L->size[i] = P->size[a];
+    L->dsize[i] = P->dsize[a];
+    L->dangling[i] = P->dangling[a];
+    if (L->dangling[i]) net->danglings[net->n_dangling++] = i;
+  }
+  net->tot_node_w = tot;
+  free(newid);
+  double all = 0.0;
+  for (int i = 0; i < k; i++) all += plogp(L->size[i]);
+  net->all_log_all = all;
+  init_modules(net);
+  return net;
 }
 
 int orc_iterations(const orc_net *net) { return net->iterations; }

@@ -90,6 +90,15 @@ int orc_sweep(orc_net *net, int variant, int sp, double vthresh);
 void orc_reset_active(orc_net *net, int sp);
 int orc_n_active(const orc_net *net);
 
+/* generate_network_from_module (OmpRelaxmap.cpp:640-695): the network of
+ * the parent's nodes members[0..k) (after orc_initiate; list them ascending,
+ * as the reference does) with their flows, teleport weights and dangling
+ * flags and the flows of the arcs between them, in singleton modules
+ * (Module.cpp:67-99 over those arcs) and calibrated.  partition_module_network
+ * (OmpRelaxmap.cpp:436-460) is then orc_sweep(variant 0) until a sweep gains
+ * less than the threshold.  NULL on a bad member list. */
+orc_net *orc_subnetwork(const orc_net *parent, int k, const int32_t *members);
+
 /* The decision half of prioritize_parallelMove (Module.cpp:1355-1488) for
  * every unit of the level (sp != 0: super nodes) against the same module
  * statistics; nothing moves.  best_mod[u] is the argmin (strict <, ascending

@@ -460,10 +460,10 @@ int rmx_load_network(rmx_ctx *ctx, const char *path, int32_t *n, int64_t *m, int
     std::string err;
     int rc = read_network(path, net, err);
     if (rc != 0) return fail(ctx, rc == -2 ? RMX_EIO : RMX_EINVAL, err);
-    ctx->names = std::move(net.names);
     int r = rmx_graph_upload(ctx, net.n, (int64_t)net.idx.size(), net.ptr.data(), net.idx.data(), net.w.data(),
                              net.node_w.data());
     if (r != RMX_OK) return r;
+    ctx->names = std::move(net.names);   // after the upload, which resets the names to the default ones
     if (n) *n = net.n;
     if (m) *m = (int64_t)net.idx.size();
     if (n_self) *n_self = net.n_self;

@@ -6,6 +6,8 @@ and stores its 17-digit JSON output, gzip-compressed, one file per case.
 Run in the development container (the reference is not present elsewhere):
 
     make -C oracle ref && python tests/golden/make_golden.py
+    python tests/golden/make_golden.py writers          # .tree / .clu writer fixtures
+    python tests/golden/make_golden.py reference-runs   # the reference CLI on all threads, 5 seeds
 
 The config-1 stochastic block model input (sbm_1k_c25_s1.txt) is generated
 here with numpy and committed as data.
@@ -65,9 +67,99 @@ def sbm_config1(path, n=1000, blocks=25, mean_deg=15, mu=0.1, seed=1):
             f.write(f"{a + 1}\t{block[a] + 1}\n")
 
 
+REF_EXE = os.path.join(ROOT, "oracle", "_ref", "ompRelaxmap")
+
+
+def reference_cli(path, seed, threads, outdir):
+    """One run of the reference command line (prior, threshold 1e-3, vThresh
+    0, maxIter 10, one attempt): its printed codelengths and module count,
+    and the .tree / .clu files it wrote; None if it did not finish."""
+    name = os.path.splitext(os.path.basename(path))[0]
+    r = subprocess.run([REF_EXE, str(seed), path, str(threads), "1", "1e-3", "0", "10", outdir, "prior"],
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        return {"seed": seed, "threads": threads, "returncode": r.returncode}
+    out = r.stdout
+    num = r"([0-9.eE+-]+)"
+    import re
+    comp = re.search(r"Computed Code Length = " + num, out)
+    done = re.search(r"DONE: Code Length = " + num + r" in (\d+) modules", out)
+    res = {"seed": seed, "threads": threads, "returncode": 0,
+           "computed_codelength_bits": float(comp.group(1)), "codelength_bits": float(done.group(1)),
+           "n_module": int(done.group(2))}
+    with open(os.path.join(outdir, name + ".tree"), newline="") as f:
+        res["tree"] = f.read()
+    with open(os.path.join(outdir, name + ".clu"), newline="") as f:
+        res["clu"] = f.read()
+    return res
+
+
+def writer_fixtures(inputs, tmp):
+    """The reference's .tree writer on the installed 'swept' partitions
+    (harness 'write'), and one whole reference CLI run on one thread with the
+    .tree and .clu files it wrote."""
+    for key in ("flow_net", "lfr1k_mu03", "sbm1k"):
+        path = inputs[key]
+        part = harness("sweeps", path, 3, 3, "move")["sweeps"][-1]["partition"]
+        pf = os.path.join(tmp, "part.txt")
+        np.savetxt(pf, np.array(part), fmt="%d")
+        info = harness("write", path, pf, tmp, "w")
+        with open(os.path.join(tmp, "w.tree"), newline="") as f:
+            tree = f.read()
+        cli = reference_cli(path, 1, 1, tmp)
+        save(f"{key}.writers", {"partition": part, "tree": tree, "harness": info, "cli_one_thread": cli})
+
+
+def reference_runs(inputs, tmp, seeds=(1, 2, 3, 4, 5)):
+    """The reference command line's whole schedule (main(), coarse tunes
+    included) on all threads of this container, several seeds: the quality
+    bar of the GPU's final codelength."""
+    threads = len(os.sched_getaffinity(0))
+    sys.path.insert(0, ROOT)
+    from relaxmap_amd import synth
+    lfr100k = os.path.join(tmp, "lfr100k.net")
+    synth.config2().write(lfr100k)
+    runs = {}
+    for key, path in list(inputs.items()) + [("lfr100k", lfr100k)]:
+        runs[key] = []
+        for sd in seeds:
+            r = reference_cli(path, sd, threads, tmp)
+            r.pop("tree", None)
+            r.pop("clu", None)
+            runs[key].append(r)
+    save("reference_runs", {"threads": threads, "cpu": platform_cpu(), "settings": "prior 1e-3 0 10, 1 attempt",
+                            "runs": runs})
+
+
+def platform_cpu():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
+    if len(sys.argv) > 1 and sys.argv[1] in ("writers", "reference-runs"):
+        tmp = tempfile.mkdtemp()
+        try:
+            five_k = os.path.join(tmp, "directNet_5k_sm_mu_05.txt")
+            with gzip.open(os.path.join(DATA, "directNet_5k_sm_mu_05.txt.gz"), "rb") as fi, open(five_k, "wb") as fo:
+                shutil.copyfileobj(fi, fo)
+            inputs = {"flow_net": os.path.join(DATA, "flow.net"),
+                      "lfr1k_mu03": os.path.join(DATA, "directNet_1k_sm_mu_03.txt"),
+                      "lfr5k_mu05": five_k, "sbm1k": os.path.join(DATA, "sbm_1k_c25_s1.txt")}
+            if sys.argv[1] == "writers":
+                writer_fixtures(inputs, tmp)
+            else:
+                reference_runs(inputs, tmp)
+        finally:
+            shutil.rmtree(tmp)
+        return
     sbm = os.path.join(DATA, "sbm_1k_c25_s1.txt")
     if not os.path.exists(sbm):
         sbm_config1(sbm)

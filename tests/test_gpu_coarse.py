@@ -9,6 +9,10 @@ no trajectory can be compared. What is checked:
   sequential moves guarantee it; relaxed sub-sweeps reset such an instance)
   and some improve, with the map equation of the induced subgraph computed
   independently in numpy;
+- the sub-module partitions are as good as the reference's sequential ones:
+  over all instances, the codelength total of the GPU's relaxed sub-sweeps
+  is within 0.5 % of partition_module_network's move() sweeps on the same
+  induced networks (the oracle, mean of three seeds);
 - the super graph has the structure of the oracle's contraction of the
   sub-module partition (integer structure bit-exact, weights 1e-13 relative),
   with the parent module as each super node's module;
@@ -20,16 +24,9 @@ import numpy as np
 import pytest
 
 from conftest import INPUTS, data_path
+from submodule_eval import instance_codelengths
 
 pytestmark = pytest.mark.gpu
-
-ALPHA, BETA = 0.15, 1.0 - 0.15
-
-
-def plogp(x):
-    x = np.asarray(x, dtype=np.float64)
-    return np.where(x > 0, x * np.log(np.where(x > 0, x, 1.0)), 0.0)
-
 
 def _prepared(key, sweeps=2):
     """A network after a few node sweeps (modules with several members)."""
@@ -43,32 +40,7 @@ def _prepared(key, sweeps=2):
     return net
 
 
-def _instance_codelengths(p, tau, ptr, idx, f, part, labels):
-    """Codelength of every parent module's induced subgraph under `labels`
-    (map equation with the members' own p, tau and dangling flow; exits count
-    only arcs inside the parent)."""
-    n = len(p)
-    src = np.repeat(np.arange(n), np.diff(ptr))
-    dangling = np.diff(ptr) == 0
-    d = np.where(dangling, p, 0.0)
-    inside = part[src] == part[idx]
-    cross = inside & (labels[src] != labels[idx])
-    X = np.zeros(n)
-    np.add.at(X, labels[src[cross]], f[cross])
-    P, T, D = np.zeros(n), np.zeros(n), np.zeros(n)
-    np.add.at(P, labels, p)
-    np.add.at(T, labels, tau)
-    np.add.at(D, labels, d)
-    used = np.zeros(n, dtype=bool)
-    used[labels] = True
-    q = (ALPHA * P + BETA * D) * (1.0 - T) + BETA * X
-    out = {}
-    for I in np.unique(part):
-        mods = np.unique(labels[part == I])
-        S = q[mods].sum()
-        out[I] = float(plogp(S) - 2 * plogp(q[mods]).sum() + plogp(q[mods] + P[mods]).sum()
-                       - plogp(p[part == I]).sum())
-    return out
+_instance_codelengths = instance_codelengths
 
 
 @pytest.mark.parametrize("key", ["lfr1k_mu03", "sbm1k", "lfr5k_mu05"])
@@ -148,3 +120,36 @@ def test_coarse_tune_deterministic():
             net.generate_sub_modules(threshold=1e-3, max_iter=10, seed=4)
             out.append(net.sub_modules())
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("key", ["lfr1k_mu03", "sbm1k", "lfr5k_mu05"])
+def test_sub_module_quality_against_sequential_partitions(key):
+    """generate_sub_modules partitions each small module with the sequential
+    move() (OmpRelaxmap.cpp:534-552, partition_module_network at numTh 1,
+    436-460).  The oracle runs exactly that on every module's induced network
+    (orc_subnetwork); both partitions are scored with the same numpy map
+    equation of the induced subgraph."""
+    from oracle.pyoracle import OracleNet
+    with _prepared(key) as net:
+        part = net.partition()
+        net.generate_sub_modules(threshold=1e-3, max_iter=10, seed=3)
+        sub = net.sub_modules()
+    o = OracleNet.load(data_path(INPUTS[key])).initiate()
+    ptr, idx, f = o.out_csr()
+    p, tau = o.flows(), o.teleport()
+    gpu = instance_codelengths(p, tau, ptr, idx, f, part, sub)
+    seq_tot = []
+    for seed in (1, 2, 3):
+        labels = np.arange(o.n)
+        for I in np.unique(part):
+            members = np.flatnonzero(part == I)
+            if len(members) > 1:
+                lab = o.subnetwork(members).partition_module_network(threshold=1e-3, max_iter=10,
+                                                                     seed=seed * 7919 + int(I))
+                labels[members] = members[lab]
+        seq = instance_codelengths(p, tau, ptr, idx, f, part, labels)
+        seq_tot.append(sum(seq.values()))
+    g_tot, s_tot = sum(gpu.values()), float(np.mean(seq_tot))
+    print(f"[gpu] {key}: sub-module codelength total GPU {g_tot:.6f}, sequential {s_tot:.6f} "
+          f"(seeds {['%.6f' % x for x in seq_tot]}) nats over {len(gpu)} instances", flush=True)
+    assert g_tot <= s_tot + 0.005 * abs(s_tot)

@@ -152,3 +152,29 @@ def test_per_level_batches_quality_100k_nodes():
     print(f"[gpu] 100k per-level batches: {res}", flush=True)
     assert res[0][0] <= res[8][0] * 1.001
     assert res[0][1] >= res[8][1] - 0.005
+
+
+@pytest.mark.parametrize("key", ["flow_net", "lfr1k_mu03", "lfr5k_mu05", "sbm1k", "lfr100k"])
+def test_codelength_against_reference_cli_runs(key):
+    """The reference command line's whole schedule (coarse tunes included) on
+    every thread of the machine that wrote tests/golden/reference_runs, five
+    seeds: the GPU's final codelength, seeds 1-3, stays within 0.5 % of the
+    median of those runs (6 printed digits)."""
+    from relaxmap_amd import Network, synth
+    runs = golden("reference_runs")
+    ref = [r["computed_codelength_bits"] for r in runs["runs"][key] if r["returncode"] == 0]
+    assert len(ref) >= 3
+    med = float(np.median(ref))
+    g = synth.config2() if key == "lfr100k" else None
+    for seed in (1, 2, 3):
+        with Network() as net:
+            if g is not None:
+                g.upload_to(net)
+            else:
+                net.load(data_path(INPUTS[key]))
+            net.initiate()
+            r = net.run(threshold=1e-3, vthresh=0.0, max_iter=10, prior=True, seed=seed)
+        bits = r.computed_codelength / math.log(2)
+        print(f"[gpu] {key} seed {seed}: L={bits:.5f} bits, reference CLI median {med:.5f} "
+              f"({runs['threads']} threads, {len(ref)} seeds)", flush=True)
+        assert bits <= med * 1.005
