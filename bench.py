@@ -53,6 +53,8 @@ def nmi(a, b):
 def run_path(graph, precision, steps, warmup, args):
     from relaxmap_amd import Network
     net = Network(precision=precision, batches=args.batches)
+    if args.fp64_pull:
+        net.set_fp64_flow(False)
     t = time.perf_counter()
     graph.upload_to(net)
     log(f"{precision}: uploaded {graph.n} nodes / {graph.m} arcs in {time.perf_counter() - t:.1f} s")
@@ -79,6 +81,8 @@ def profile_pass(graph, precision, args):
     """One extra, untimed pass with per-launch HIP events (profiling mode)."""
     from relaxmap_amd import Network
     net = Network(precision=precision, batches=args.batches)
+    if args.fp64_pull:
+        net.set_fp64_flow(False)
     graph.upload_to(net)
     net.set_profiling(True)
     net.initiate()
@@ -429,6 +433,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="only the FP64 path (for its kernel trace); value is then the FP64 path's")
+    ap.add_argument("--fp64-pull", action="store_true",
+                    help="the FP64 steady state by the in-CSR pull instead of the blocked push / pull")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
     ap.add_argument("--no-config4", action="store_true", help="skip the 512-instance batch measurement")

@@ -49,6 +49,13 @@ const char *rmx_last_error(const rmx_ctx *ctx);
 int rmx_set_precision(rmx_ctx *ctx, int precision);
 int rmx_set_batches(rmx_ctx *ctx, int batches);
 
+/* How the FP64 steady state iterates (call before rmx_initiate; a change
+ * requires rmx_initiate again): 1 (default) the blocked push / pull of the
+ * mixed path with FP64 products summed exactly in two-word fixed point,
+ * 0 the in-CSR pull with FP64 gathers.  Both stop by the reference's rule
+ * and agree with it to its own rounding.  No reference counterpart. */
+int rmx_set_fp64_flow(rmx_ctx *ctx, int blocked);
+
 /* How units with more than 2048 arcs are evaluated: 0 (default) partitions
  * their arcs by neighbour module and aggregates each partition in LDS;
  * 1 uses one global-memory hash table per unit (device atomics per arc);

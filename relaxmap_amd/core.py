@@ -73,6 +73,7 @@ def load_library():
         "rmx_set_precision": (C.c_int, [vp, C.c_int]),
         "rmx_set_batches": (C.c_int, [vp, C.c_int]),
         "rmx_set_hub_path": (C.c_int, [vp, C.c_int]),
+        "rmx_set_fp64_flow": (C.c_int, [vp, C.c_int]),
         "rmx_set_graph_capture": (C.c_int, [vp, C.c_int]),
         "rmx_set_attempt_mode": (C.c_int, [vp, C.c_int]),
         "rmx_net_read": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(vp), C.c_char_p, C.c_int]),
@@ -212,6 +213,11 @@ class Network:
         self.m = 0
         self._check(L.rmx_set_precision(h, RMX_MIXED if precision == "mixed" else RMX_FP64))
         self._check(L.rmx_set_batches(h, int(batches)))
+
+    def set_fp64_flow(self, blocked):
+        """FP64 steady state: the blocked push / pull (True, the default) or
+        the in-CSR pull (False).  Call before initiate."""
+        self._check(self._L.rmx_set_fp64_flow(self._h, 1 if blocked else 0))
 
     def set_hub_path(self, mode):
         """Evaluate units with more than 2048 arcs by LDS partitions (0, the

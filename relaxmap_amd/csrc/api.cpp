@@ -147,6 +147,15 @@ int rmx_set_batches(rmx_ctx *ctx, int batches) {
   });
 }
 
+int rmx_set_fp64_flow(rmx_ctx *ctx, int blocked) {
+  return guarded(ctx, [&] {
+    if (blocked != 0 && blocked != 1) return fail(ctx, RMX_EINVAL, "FP64 flow path must be 0 or 1");
+    ctx->c.fp64_blocked = blocked;
+    ctx->c.initiated = false;   // the flows of the other path need a new initiate
+    return RMX_OK;
+  });
+}
+
 int rmx_set_hub_path(rmx_ctx *ctx, int global_tables) {
   return guarded(ctx, [&] {
     if (global_tables < 0 || global_tables > 2) return fail(ctx, RMX_EINVAL, "hub path must be 0, 1 or 2");

@@ -212,6 +212,7 @@ struct Ctx {
   int precision = 0;   // 0 FP64, 1 mixed (FP32 bulk, FP64 accumulation)
   int batches = 0;     // sub-sweeps per sweep; 0: chosen per level (sweep_batches)
   int hub_global = 0;  // 1: hubs with global-memory tables instead of LDS partitions
+  int fp64_blocked = 1;   // FP64 steady state: 1 blocked push / pull, 0 the in-CSR pull
   int use_graph = 0;   // sweeps of a batch or a sub-module partition as one captured graph
   int attempts_sequential = 0;   // 1: attempts one after another even where a batch of copies fits
   BucketCuts cuts = {{kBucketMaxDeg[0], kBucketMaxDeg[1], kBucketMaxDeg[2], kBucketMaxDeg[3], kBucketMaxDeg[4],

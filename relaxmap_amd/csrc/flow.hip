@@ -445,6 +445,128 @@ __global__ __launch_bounds__(kPullBlock) void k_pb_pull(const uint32_t *__restri
   }
 }
 
+// FP64 blocked push / pull: the same layout with FP64 products, summed
+// exactly.  Each product v (0 <= v <= 1) is split into two integers,
+// hi = rint(v 2^62) and lo = rint((v 2^62 - hi) 2^40) (the difference is
+// exact), accumulated in two 64-bit LDS words per row: the row's sum is then
+// known to 2^-102, independent of the order of the products, and converted
+// to FP64 once.  A bin holds at most kPullRowsMax64 rows (two 32 KB tables).
+constexpr int kPullRowsMax64 = 4096;
+// Row bounds the FP64 push keeps in LDS: 52 KB, 3 workgroups per CU.  At
+// 529M arcs one FP64 steady state took 1.013 s at 3 (and at 2) workgroups
+// per CU, 1.071 s at 4 (the FP32 push's 40 KB) and 1.254 s at 8; bins of
+// 2048 and 8192 rows instead of 4096 took 1.17 and 1.06 s.
+constexpr int kPushRows64 = 13312;
+constexpr double kHiScale = 4611686018427387904.0;            // 2^62
+constexpr double kLoScale = 1099511627776.0;                  // 2^40
+constexpr double kHiInv = 1.0 / 4611686018427387904.0;        // 2^-62
+constexpr double kLoInv = 1.0 / 5070602400912917605986812821504.0;   // 2^-102
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+__device__ inline f64x4 stream_d4(const double *p) { return __builtin_nontemporal_load((const f64x4 *)p); }
+
+template <int ROWS>
+__global__ __launch_bounds__(256) void k_pb_push64(const uint32_t *__restrict__ optr,
+                                                   const uint32_t *__restrict__ trow, int64_t ntiles, int64_t per_xcd,
+                                                   int64_t m, const uint32_t *__restrict__ slot,
+                                                   const double *__restrict__ w, const double *__restrict__ x,
+                                                   double *__restrict__ val) {
+  __shared__ uint32_t sp[ROWS];
+  const int64_t tile = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (tile >= ntiles) return;
+  const int64_t a0 = tile * kPushTile;
+  const int64_t a1 = std::min<int64_t>(m, a0 + kPushTile);
+  const uint32_t r0 = trow[tile], r1 = trow[tile + 1];
+  const int64_t nr = (int64_t)r1 - r0 + 1;
+  const bool in_lds = nr + 1 <= ROWS;
+  if (in_lds)
+    for (int64_t i = threadIdx.x; i <= nr; i += 256) sp[i] = optr[r0 + i];
+  __syncthreads();
+  const int64_t k0 = a0 + 8 * (int64_t)threadIdx.x;
+  if (k0 >= a1) return;
+  const uint32_t *bounds = in_lds ? sp : optr + r0;
+  int64_t lo = 0, hi = nr;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)bounds[mid] <= k0) lo = mid;
+    else hi = mid;
+  }
+  int64_t row = lo;
+  uint32_t nxt = bounds[row + 1];
+  double xr = x[r0 + row];
+  if (k0 + 8 <= a1) {
+    const u32x4 s0 = stream_u4(slot + k0), s1 = stream_u4(slot + k0 + 4);
+    const f64x4 w0 = stream_d4(w + k0), w1 = stream_d4(w + k0 + 4);
+    const uint32_t sl[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+    const double ww[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t k = (uint32_t)(k0 + j);
+      if (k >= nxt) {
+        do {
+          ++row;
+          nxt = bounds[row + 1];
+        } while (k >= nxt);
+        xr = x[r0 + row];
+      }
+      val[sl[j]] = ww[j] * xr;
+    }
+  } else {
+    for (int64_t k = k0; k < a1; ++k) {
+      if ((uint32_t)k >= nxt) {
+        do {
+          ++row;
+          nxt = bounds[row + 1];
+        } while ((uint32_t)k >= nxt);
+        xr = x[r0 + row];
+      }
+      val[slot[k]] = w[k] * xr;
+    }
+  }
+}
+
+__device__ inline void add_split(unsigned long long *hi, unsigned long long *lo, double v) {
+  const double t = v * kHiScale;
+  const double h = rint(t);
+  atomicAdd(hi, (unsigned long long)(long long)h);
+  atomicAdd(lo, (unsigned long long)(long long)rint((t - h) * kLoScale));
+}
+
+__global__ __launch_bounds__(kPullBlock) void k_pb_pull64(const uint32_t *__restrict__ bptr,
+                                                          const uint16_t *__restrict__ bdst,
+                                                          const double *__restrict__ val, int32_t R, int32_t n,
+                                                          const double *__restrict__ tau,
+                                                          const double *__restrict__ Dv,
+                                                          const uint32_t *__restrict__ rinst,
+                                                          double *__restrict__ y) {
+  __shared__ unsigned long long acc_hi[kPullRowsMax64];
+  __shared__ unsigned long long acc_lo[kPullRowsMax64];
+  for (int i = threadIdx.x; i < R; i += kPullBlock) {
+    acc_hi[i] = 0ull;
+    acc_lo[i] = 0ull;
+  }
+  __syncthreads();
+  const uint32_t b = bptr[blockIdx.x], e = bptr[blockIdx.x + 1];
+  uint32_t j = b + threadIdx.x;
+  for (; (int64_t)j + 3 * kPullBlock < (int64_t)e; j += 4 * kPullBlock) {
+    const uint16_t d0 = bdst[j], d1 = bdst[j + kPullBlock], d2 = bdst[j + 2 * kPullBlock],
+                   d3 = bdst[j + 3 * kPullBlock];
+    const double v0 = val[j], v1 = val[j + kPullBlock], v2 = val[j + 2 * kPullBlock], v3 = val[j + 3 * kPullBlock];
+    add_split(&acc_hi[d0], &acc_lo[d0], v0);
+    add_split(&acc_hi[d1], &acc_lo[d1], v1);
+    add_split(&acc_hi[d2], &acc_lo[d2], v2);
+    add_split(&acc_hi[d3], &acc_lo[d3], v3);
+  }
+  for (; j < e; j += kPullBlock) add_split(&acc_hi[bdst[j]], &acc_lo[bdst[j]], val[j]);
+  __syncthreads();
+  const int64_t row0 = (int64_t)blockIdx.x * R;
+  for (int i = threadIdx.x; i < R && row0 + i < n; i += kPullBlock) {
+    const int64_t r = row0 + i;
+    const double sum = (double)(long long)acc_hi[i] * kHiInv + (double)(long long)acc_lo[i] * kLoInv;
+    y[r] = (kAlpha + kBeta * Dv[rinst ? rinst[r] : 0]) * tau[r] + kBeta * sum;
+  }
+}
+
 // --- reductions in a fixed launch shape (deterministic) ---------------------
 __global__ __launch_bounds__(kRedBlock) void k_sum_partials(const double *__restrict__ v, int32_t n,
                                                             double *__restrict__ part) {
@@ -822,23 +944,30 @@ void spmv_all(Ctx &c, const PullCSR &L, const RowBins &rb, double *y) {
 struct PushBins {
   int32_t R = 0, nbins = 0, rshift = 0;
   int64_t ntiles = 0;
+  bool fp64 = false;   // FP64 products (val64) instead of FP32 (val)
   DevBuf<uint32_t> bptr, slot, trow;
   DevBuf<uint16_t> bdst;
   DevBuf<float> val;
-  explicit PushBins(BlockCache &bc) : bptr(bc), slot(bc), trow(bc), bdst(bc), val(bc) {}
+  DevBuf<double> val64;
+  explicit PushBins(BlockCache &bc) : bptr(bc), slot(bc), trow(bc), bdst(bc), val(bc), val64(bc) {}
+  void alloc_values(int64_t m) {
+    if (fp64) val64.alloc(m);
+    else val.alloc(m);
+  }
 };
 
-// Rows per bin: a power of two from 256 to kPullRowsMax, about n / 2048, so
-// that a large network gives thousands of bins (workgroups of the pull).
-// The slots come from the per-tile counts above when the bins fit the LDS
-// counters and the level has its in-CSR, else from a stable sort of the
-// out-arcs by bin.
+// Rows per bin: a power of two from 256 to kPullRowsMax (kPullRowsMax64 for
+// FP64 products), about n / 2048, so that a large network gives thousands of
+// bins (workgroups of the pull).  The slots come from the per-tile counts
+// above when the bins fit the LDS counters and the level has its in-CSR,
+// else from a stable sort of the out-arcs by bin.
 void build_push_bins(Ctx &c, const Level &L, PushBins &pb) {
   hipStream_t s = c.stream;
   const int32_t n = L.n;
   const int64_t m = L.m;
+  const int rows_max = pb.fp64 ? kPullRowsMax64 : kPullRowsMax;
   int rs = 8;
-  while ((1 << rs) < kPullRowsMax && ((int64_t)1 << rs) * 2048 < n) ++rs;
+  while ((1 << rs) < rows_max && ((int64_t)1 << rs) * 2048 < n) ++rs;
   pb.rshift = rs;
   pb.R = 1 << rs;
   pb.nbins = (int32_t)(((int64_t)n + pb.R - 1) >> rs);
@@ -871,7 +1000,7 @@ void build_push_bins(Ctx &c, const Level &L, PushBins &pb) {
     RMX_LAUNCH_CHECK();
     pb.slot.alloc(m);
     pb.bdst.alloc(m);
-    pb.val.alloc(m);
+    pb.alloc_values(m);
     const int64_t per_xcd = (ntl + 7) / 8;
     k_pb_place<<<(unsigned)(8 * per_xcd), kLayoutBlock, 0, s>>>(L.out_idx.p, m, tile_arcs, ntl, per_xcd, pb.nbins,
                                                                 rs, C.p, pb.slot.p, pb.bdst.p);
@@ -893,7 +1022,7 @@ void build_push_bins(Ctx &c, const Level &L, PushBins &pb) {
   RMX_LAUNCH_CHECK();
   pb.slot.alloc(m);
   pb.bdst.alloc(m);
-  pb.val.alloc(m);
+  pb.alloc_values(m);
   k_pb_layout<<<grid_for(m, 256), 256, 0, s>>>(arc.p, L.out_idx.p, m, rs, pb.slot.p, pb.bdst.p);
   RMX_LAUNCH_CHECK();
 }
@@ -904,6 +1033,22 @@ void build_push_bins(Ctx &c, const Level &L, PushBins &pb) {
 // 16-bit head (6), per row tau and writes y (16, and 4 for the instance).
 void push_pull(Ctx &c, const PushBins &pb, const Level &L, const PullCSR &F, double *y) {
   hipStream_t s = c.stream;
+  if (pb.fp64) {   // FP64 products: per arc slot, FP64 weight and product (20 B); pull 10 B per arc
+    if (pb.ntiles > 0) {
+      const int64_t per = (pb.ntiles + 7) / 8;
+      if (c.prof.on) c.prof.begin(s);
+      k_pb_push64<kPushRows64><<<(unsigned)(8 * per), 256, 0, s>>>(L.out_ptr.p, pb.trow.p, pb.ntiles, per, L.m,
+                                                                       pb.slot.p, L.out_w.p, c.x.p, pb.val64.p);
+      RMX_LAUNCH_CHECK();
+      if (c.prof.on) c.prof.end(s, "k_pb_push64", (double)L.m * 20.0 + (double)L.n * 12.0);
+    }
+    if (c.prof.on) c.prof.begin(s);
+    k_pb_pull64<<<(unsigned)pb.nbins, kPullBlock, 0, s>>>(pb.bptr.p, pb.bdst.p, pb.val64.p, pb.R, L.n, F.tau, F.Dv,
+                                                          F.rinst, y);
+    RMX_LAUNCH_CHECK();
+    if (c.prof.on) c.prof.end(s, "k_pb_pull64", (double)L.m * 10.0 + (double)L.n * (F.rinst ? 20.0 : 16.0));
+    return;
+  }
   if (pb.ntiles > 0) {
     const int64_t per = (pb.ntiles + 7) / 8;
     if (c.prof.on) c.prof.begin(s);
@@ -949,6 +1094,7 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   const int32_t n = L.n;
   const int64_t m = L.m;
   const bool mixed = c.precision == 1;
+  const bool blocked = mixed || c.fp64_blocked;   // push / pull bins (else the FP64 pull)
   L.out_w.alloc(m);
   L.in_w.alloc(m);
   L.out_wf.alloc(m);
@@ -997,7 +1143,7 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
     k_normalize<16, false><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, c.raw_w.p, L.out_w.p,
                                                                          L.out_wf.p, rowsum.p, c.dangling.p);
     RMX_LAUNCH_CHECK();
-    if (m > 0) {
+    if (m > 0 && !blocked) {   // the pull's in-arc weights
       k_in_norm<<<stride_grid(m), 256, 0, s>>>(L.in_idx.p, c.raw_in.p, m, rowsum.p, L.in_w.p);
       RMX_LAUNCH_CHECK();
     }
@@ -1011,7 +1157,8 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   PushBins pb(c.cache);
   PullCSR F;
   F.n = n;
-  const bool relabel = !batch && !mixed;
+  pb.fp64 = !mixed;
+  const bool relabel = !batch && !blocked;
   if (!relabel) {   // instances keep their node ranges
     F.ptr = L.in_ptr.p;
     F.idx = L.in_idx.p;
@@ -1020,7 +1167,7 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
     F.dangling = c.dangling.p;
     F.Dv = batch ? c.b_D.p : c.scal.p + kD;
     F.rinst = batch ? c.b_node.p : nullptr;
-    if (mixed) build_push_bins(c, L, pb);
+    if (blocked) build_push_bins(c, L, pb);
   } else {
     F.Dv = c.scal.p + kD;
     inv.alloc(n);
@@ -1056,7 +1203,7 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
     F.dangling = pdang.p;
   }
   RowBins rb;
-  if (!mixed) make_row_bins(c, F, rb);
+  if (!blocked) make_row_bins(c, F, rb);
 
   hipEvent_t e0, e1;
   RMX_HIP(hipEventCreate(&e0));
@@ -1076,7 +1223,7 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
     const double b_tol = mixed ? kMixedTol : tol;
     uint32_t running = 1;
     while (running > 0) {
-      if (mixed) push_pull(c, pb, L, F, c.y.p);
+      if (blocked) push_pull(c, pb, L, F, c.y.p);
       else spmv_all(c, F, rb, c.y.p);
       k_bnorm<<<nb, kRedBlock, 0, s>>>(c.b_off.p, c.y.p, c.dangling.p, done.p, c.x.p, c.xf.p, c.b_D.p, c.b_R.p);
       RMX_LAUNCH_CHECK();
@@ -1099,7 +1246,7 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   std::vector<double> hist;
   const int first_check = mixed ? 1 : min_iter;
   while (it < max_iter) {
-    if (mixed) push_pull(c, pb, L, F, c.y.p);
+    if (blocked) push_pull(c, pb, L, F, c.y.p);
     else spmv_all(c, F, rb, c.y.p);
     if (c.prof.on) c.prof.begin(s);
     k_sum_partials<<<kRedGrid, kRedBlock, 0, s>>>(c.y.p, n, part);

@@ -162,6 +162,40 @@ def test_batch_close_to_single_runs():
     assert max(rel) <= 0.01
 
 
+def test_instance_seeds_change_the_draws():
+    """Copies of one network in one batch, each with its own seed: after one
+    sweep their partitions differ (every instance draws its own sub-sweeps,
+    the seed mixed into each unit's draw), and over whole runs their
+    codelengths vary as much as runs of the network alone with those seeds
+    do: the median absolute deviation over the seeds (max - min is ruled by
+    single outlier runs, which either mode can draw)."""
+    from relaxmap_amd import Network
+    from relaxmap_amd.batch import run_batched, run_instances, union_of
+    from relaxmap_amd import synth
+    g = synth.lfr(20_000, 2.0, 5, 200, 1.0, 20, 500, 0.45, 0.0, 3)   # weakly modular: seeds matter
+    seeds = list(range(1, 9))
+    pairs, node_off, iseeds, ptr, idx, w = union_of([g], seeds)
+    with Network(batches=4) as net:
+        net.batch_upload(node_off, iseeds, ptr, idx, w)
+        net.initiate()
+        net.move(prioritized=False, seed=1)
+        part = net.partition()
+    local = [part[int(node_off[k]):int(node_off[k + 1])] - int(node_off[k]) for k in range(len(pairs))]
+    same = sum(np.array_equal(local[0], local[k]) for k in range(1, len(local)))
+    print(f"[gpu] {len(pairs)} copies after one sweep: {same} equal to the first", flush=True)
+    assert same == 0
+    rb, _ = run_batched([g], seeds)
+    ri, _ = run_instances([g], seeds, workers=1)
+    Lb = np.array([rb[(0, s)][0] for s in seeds])
+    Li = np.array([ri[(0, s)][0] for s in seeds])
+    mad = lambda x: float(np.median(np.abs(x - np.median(x))) / np.median(x))
+    sb, si = mad(Lb), mad(Li)
+    print(f"[gpu] codelength over {len(seeds)} seeds: batch {np.round(Lb, 6).tolist()}, alone "
+          f"{np.round(Li, 6).tolist()}; median absolute deviation {sb:.2e} vs {si:.2e}", flush=True)
+    assert len(np.unique(Lb)) >= 3
+    assert 0.2 * si <= sb <= 5.0 * si
+
+
 def test_batch_deterministic():
     from relaxmap_amd.batch import run_batched
     graphs = _graphs()[3:5]

@@ -1,5 +1,7 @@
-"""Mixed-precision steady state (blocked push / pull) on shapes that reach
-every branch of its kernels, against the oracle's FP64 flows.
+"""The blocked push / pull steady state, mixed precision (FP32 products) and
+FP64 (FP64 products in two-word fixed point), on shapes that reach every
+branch of its kernels, against the oracle's FP64 flows, and the FP64
+blocked path against the FP64 in-CSR pull.
 
 - long runs of rows without out-arcs inside one push tile (row bounds read
   from global memory instead of LDS);
@@ -62,25 +64,46 @@ def test_mixed_flows_match_oracle(name):
     ptr, idx, ww, src = _csr(n, s, d, w)
     print(f"[gpu] push/pull {name}: n={n} m={len(idx)}", flush=True)
     ref = OracleNet.from_arcs(n, src, idx, ww).initiate().flows()
-    for prec, tol in (("mixed", 1e-6), ("fp64", 1e-12)):
+    for prec, blocked, tol in (("mixed", True, 1e-6), ("fp64", True, 1e-12), ("fp64", False, 1e-12)):
         with Network(precision=prec) as net:
+            net.set_fp64_flow(blocked)
             net.upload(n, ptr, idx, ww)
             net.initiate()
             p = net.flows()
         err = np.abs(p - ref).sum()
-        print(f"[gpu]   {prec}: L1 {err:.3e}", flush=True)
+        print(f"[gpu]   {prec} {'blocked' if blocked else 'pull'}: L1 {err:.3e}", flush=True)
         assert abs(p.sum() - 1.0) <= 1e-12
         assert err <= tol, (prec, err)
 
 
-def test_mixed_flows_deterministic():
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_blocked_flows_deterministic(prec):
     from relaxmap_amd import Network
     n, s, d, w = _graphs()["hub_bins"]
     ptr, idx, ww, _ = _csr(n, s, d, w)
     runs = []
     for _ in range(2):
-        with Network(precision="mixed") as net:
+        with Network(precision=prec) as net:
             net.upload(n, ptr, idx, ww)
             net.initiate()
             runs.append(net.flows())
     assert np.array_equal(runs[0], runs[1])
+
+
+@pytest.mark.parametrize("key", ["lfr1k_mu03", "lfr5k_mu05", "sbm1k", "flow_net"])
+def test_fp64_blocked_matches_fp64_pull(key):
+    """Both FP64 iterations take the reference's stop rule and FP64 sums:
+    same iteration count (within one), flows within 1e-13 L1 of each other."""
+    from conftest import INPUTS, data_path
+    from relaxmap_amd import Network
+    out = []
+    for blocked in (True, False):
+        with Network() as net:
+            net.set_fp64_flow(blocked)
+            net.load(data_path(INPUTS[key]))
+            it, _ = net.initiate()
+            out.append((it, net.flows()))
+    err = float(np.abs(out[0][1] - out[1][1]).sum())
+    print(f"[gpu] {key}: blocked {out[0][0]} it, pull {out[1][0]} it, L1 {err:.2e}", flush=True)
+    assert abs(out[0][0] - out[1][0]) <= 1
+    assert err <= 1e-13
