@@ -1040,7 +1040,8 @@ void push_pull(Ctx &c, const PushBins &pb, const Level &L, const PullCSR &F, dou
       k_pb_push64<kPushRows64><<<(unsigned)(8 * per), 256, 0, s>>>(L.out_ptr.p, pb.trow.p, pb.ntiles, per, L.m,
                                                                        pb.slot.p, L.out_w.p, c.x.p, pb.val64.p);
       RMX_LAUNCH_CHECK();
-      if (c.prof.on) c.prof.end(s, "k_pb_push64", (double)L.m * 20.0 + (double)L.n * 12.0);
+      if (c.prof.on)
+        c.prof.end(s, "k_pb_push64<" + std::to_string(kPushRows64) + ">", (double)L.m * 20.0 + (double)L.n * 12.0);
     }
     if (c.prof.on) c.prof.begin(s);
     k_pb_pull64<<<(unsigned)pb.nbins, kPullBlock, 0, s>>>(pb.bptr.p, pb.bdst.p, pb.val64.p, pb.R, L.n, F.tau, F.Dv,
