@@ -123,6 +123,18 @@ int rmx_autotune(rmx_ctx *ctx, int32_t *max_deg);
 int rmx_batch_upload(rmx_ctx *ctx, int32_t n_inst, const int32_t *node_off, const uint64_t *seeds, int32_t n,
                      int64_t m, const int64_t *out_ptr, const int32_t *out_idx, const double *w,
                      const double *node_w);
+/* The same batch when every network runs under several seeds (the '#
+ * attempts' of OmpRelaxmap.cpp:50 as independent instances): n_graph networks
+ * as one union out-CSR (network g owns the node ids [graph_off[g],
+ * graph_off[g+1]) and no arc joins two networks), each run under n_seed
+ * seeds.  Instance k = g * n_seed + r is network g with seed seeds[k]; its
+ * node ids follow those of instance k - 1, as rmx_batch_upload would number
+ * the union of the copies.  The networks cross to the device once and are
+ * copied there; every later result equals rmx_batch_upload's on that union.
+ * No reference counterpart. */
+int rmx_batch_upload_seeds(rmx_ctx *ctx, int32_t n_graph, const int32_t *graph_off, int32_t n_seed,
+                           const uint64_t *seeds, int32_t n, int64_t m, const int64_t *out_ptr,
+                           const int32_t *out_idx, const double *w, const double *node_w);
 /* Per instance: codelength (nats) and non-empty modules of the current
  * partition, and the steady-state iterations. */
 int rmx_batch_results(rmx_ctx *ctx, double *codelength, int32_t *n_module, int32_t *iterations);

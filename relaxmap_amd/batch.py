@@ -29,25 +29,31 @@ def union_of(graphs, seeds):
     """The batch of every (graph, seed) pair, graph-major: node offsets,
     per-instance seeds and the union out-CSR (node ids shifted per instance)."""
     pairs = [(gi, s) for gi in range(len(graphs)) for s in seeds]
-    ns = np.array([graphs[gi].n for gi, _ in pairs], dtype=np.int64)
-    ms = np.array([graphs[gi].m for gi, _ in pairs], dtype=np.int64)
-    node_off = np.zeros(len(pairs) + 1, dtype=np.int64)
+    node_off, ptr, idx, w = graphs_union([graphs[gi] for gi, _ in pairs])
+    iseeds = np.array([s for _, s in pairs], dtype=np.uint64)
+    return pairs, node_off, iseeds, ptr, idx, w
+
+
+def graphs_union(graphs):
+    """The networks' union out-CSR (node ids shifted per network) and the
+    first node of every network."""
+    ns = np.array([g.n for g in graphs], dtype=np.int64)
+    ms = np.array([g.m for g in graphs], dtype=np.int64)
+    node_off = np.zeros(len(graphs) + 1, dtype=np.int64)
     node_off[1:] = np.cumsum(ns)
-    arc_off = np.zeros(len(pairs) + 1, dtype=np.int64)
+    arc_off = np.zeros(len(graphs) + 1, dtype=np.int64)
     arc_off[1:] = np.cumsum(ms)
     N, M = int(node_off[-1]), int(arc_off[-1])
     ptr = np.empty(N + 1, dtype=np.int64)
     idx = np.empty(M, dtype=np.int32)
     w = np.empty(M, dtype=np.float64)
-    for k, (gi, _) in enumerate(pairs):
-        g = graphs[gi]
+    for k, g in enumerate(graphs):
         a, b = int(node_off[k]), int(node_off[k + 1])
         ptr[a:b] = np.asarray(g.ptr[:-1], dtype=np.int64) + arc_off[k]
         idx[arc_off[k]:arc_off[k + 1]] = np.asarray(g.idx, dtype=np.int64) + a
         w[arc_off[k]:arc_off[k + 1]] = g.w
     ptr[N] = M
-    iseeds = np.array([s for _, s in pairs], dtype=np.uint64)
-    return pairs, node_off, iseeds, ptr, idx, w
+    return node_off, ptr, idx, w
 
 
 def run_batched(graphs, seeds, precision="mixed", batches=0, threshold=1e-3, vthresh=0.0, max_iter=10, prior=True,
@@ -59,10 +65,12 @@ def run_batched(graphs, seeds, precision="mixed", batches=0, threshold=1e-3, vth
     building the union, the upload, the steady states and the partition.
     """
     t0 = time.perf_counter()
-    pairs, node_off, iseeds, ptr, idx, w = union_of(graphs, seeds)
+    pairs = [(gi, s) for gi in range(len(graphs)) for s in seeds]
+    goff, ptr, idx, w = graphs_union(graphs)
+    iseeds = np.array([s for _, s in pairs], dtype=np.uint64)
     net = Network(precision=precision, batches=batches)
     try:
-        net.batch_upload(node_off, iseeds, ptr, idx, w)
+        net.batch_upload_seeds(goff, len(seeds), iseeds, ptr, idx, w)
         net.initiate()
         net.run(threshold=threshold, vthresh=vthresh, max_iter=max_iter, prior=prior, seed=1)
         L, nmod, _ = net.batch_results()

@@ -89,6 +89,8 @@ def load_library():
         "rmx_graph_upload": (C.c_int, [vp, C.c_int32, C.c_int64, _I64P, _I32P, _F64P, _F64P]),
         "rmx_batch_upload": (C.c_int, [vp, C.c_int32, _I32P, C.POINTER(C.c_uint64), C.c_int32, C.c_int64, _I64P,
                                        _I32P, _F64P, _F64P]),
+        "rmx_batch_upload_seeds": (C.c_int, [vp, C.c_int32, _I32P, C.c_int32, C.POINTER(C.c_uint64), C.c_int32,
+                                             C.c_int64, _I64P, _I32P, _F64P, _F64P]),
         "rmx_batch_results": (C.c_int, [vp, _F64P, _I32P, _I32P]),
         "rmx_set_bucket_cutoffs": (C.c_int, [vp, _I32P]),
         "rmx_get_bucket_cutoffs": (C.c_int, [vp, _I32P]),
@@ -283,6 +285,30 @@ class Network:
                                              _p(out_ptr, _I64P), _p(out_idx, _I32P), _p(w, _F64P), nwp))
         self.n, self.m = int(n), len(out_idx)
         self.n_inst = len(node_off) - 1
+        return self
+
+    def batch_upload_seeds(self, graph_off, n_seed, seeds, out_ptr, out_idx, w, node_w=None):
+        """A batch of networks under n_seed seeds each: the union out-CSR of the
+        networks (network g owning nodes [graph_off[g], graph_off[g+1])), copied
+        on the device; instance g * n_seed + r is network g with seeds[g * n_seed + r]."""
+        graph_off = np.ascontiguousarray(graph_off, dtype=np.int32)
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        out_ptr = np.ascontiguousarray(out_ptr, dtype=np.int64)
+        out_idx = np.ascontiguousarray(out_idx, dtype=np.int32)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        n_graph = len(graph_off) - 1
+        if len(seeds) != n_graph * int(n_seed):
+            raise ValueError("one seed per (network, seed) instance")
+        nwp = None
+        if node_w is not None:
+            node_w = np.ascontiguousarray(node_w, dtype=np.float64)
+            nwp = _p(node_w, _F64P)
+        n = len(out_ptr) - 1
+        self._check(self._L.rmx_batch_upload_seeds(self._h, n_graph, _p(graph_off, _I32P), int(n_seed),
+                                                   _p(seeds, C.POINTER(C.c_uint64)), int(n), len(out_idx),
+                                                   _p(out_ptr, _I64P), _p(out_idx, _I32P), _p(w, _F64P), nwp))
+        self.n, self.m = int(n) * int(n_seed), len(out_idx) * int(n_seed)
+        self.n_inst = n_graph * int(n_seed)
         return self
 
     def set_bucket_cutoffs(self, max_deg):

@@ -223,6 +223,8 @@ const char *check_csr(int32_t n, int64_t m, const int64_t *out_ptr, const int32_
   });
 }
 
+void install_batch(Ctx &c, int32_t n_inst, const int32_t *node_off, const uint64_t *seeds, const uint32_t *inst);
+
 // default names "1".."n", made when first needed (an uploaded network has
 // none of its own)
 void set_names(rmx_ctx *ctx, int32_t n) {
@@ -277,48 +279,102 @@ int rmx_batch_upload(rmx_ctx *ctx, int32_t n_inst, const int32_t *node_off, cons
     RMX_HIP(hipSetDevice(c.device));
     c.nb = 0;
     upload_graph(c, n, m, out_ptr, out_idx, w, node_w);
-    c.nb = n_inst;
-    c.h_boff.assign(node_off, node_off + n_inst + 1);
-    std::vector<uint64_t> sd(n_inst);
-    for (int32_t I = 0; I < n_inst; ++I) sd[I] = mix64(seeds[I] ^ 0x6a09e667f3bcc909ull);
-    std::vector<uint32_t> cbeg, cend, ichunk(1, 0u);
-    for (int32_t I = 0; I < n_inst; ++I) {
-      for (int64_t a = node_off[I]; a < node_off[I + 1]; a += kTotChunk) {
-        cbeg.push_back((uint32_t)a);
-        cend.push_back((uint32_t)std::min<int64_t>(a + kTotChunk, node_off[I + 1]));
+    install_batch(c, n_inst, node_off, seeds, inst.data());
+    ctx->names.clear();
+    return RMX_OK;
+  });
+}
+
+int rmx_batch_upload_seeds(rmx_ctx *ctx, int32_t n_graph, const int32_t *graph_off, int32_t n_seed,
+                           const uint64_t *seeds, int32_t n, int64_t m, const int64_t *out_ptr,
+                           const int32_t *out_idx, const double *w, const double *node_w) {
+  return guarded(ctx, [&] {
+    if (n_graph < 1 || n_seed < 1 || !graph_off || !seeds) return fail(ctx, RMX_EINVAL, "no networks, no seeds or null arrays");
+    const char *bad = check_csr(n, m, out_ptr, out_idx, w);
+    if (*bad) return fail(ctx, RMX_EINVAL, bad);
+    if (graph_off[0] != 0 || graph_off[n_graph] != n) return fail(ctx, RMX_EINVAL, "graph_off does not span the nodes");
+    for (int32_t g = 0; g < n_graph; ++g)
+      if (graph_off[g + 1] < graph_off[g]) return fail(ctx, RMX_EINVAL, "graph_off not monotone");
+    if ((int64_t)n * n_seed > 0x7fffffffLL || m * n_seed > 0x7fffffffLL)
+      return fail(ctx, RMX_EINVAL, "the copies exceed 2^31 - 1 nodes or arcs");
+    bad = rows_in_parallel(n, m, out_ptr, [&](int32_t i0, int32_t i1) -> const char * {
+      if (i0 >= i1) return "";
+      int32_t g = (int32_t)(std::upper_bound(graph_off, graph_off + n_graph + 1, i0) - graph_off) - 1;
+      for (int32_t i = i0; i < i1; ++i) {
+        while (i >= graph_off[g + 1]) ++g;
+        for (int64_t k = out_ptr[i]; k < out_ptr[i + 1]; ++k)
+          if (out_idx[k] < graph_off[g] || out_idx[k] >= graph_off[g + 1]) return "an arc joins two networks";
       }
-      ichunk.push_back((uint32_t)cbeg.size());
-    }
-    c.n_chunks = (int32_t)cbeg.size();
-    c.b_cbeg.alloc(std::max<int64_t>(1, c.n_chunks));
-    c.b_cend.alloc(std::max<int64_t>(1, c.n_chunks));
-    c.b_ichunk.alloc((int64_t)n_inst + 1);
-    c.b_cpart.alloc(4 * std::max<int64_t>(1, c.n_chunks));
-    if (c.n_chunks > 0) {
-      RMX_HIP(hipMemcpyAsync(c.b_cbeg.p, cbeg.data(), sizeof(uint32_t) * c.n_chunks, hipMemcpyHostToDevice, c.stream));
-      RMX_HIP(hipMemcpyAsync(c.b_cend.p, cend.data(), sizeof(uint32_t) * c.n_chunks, hipMemcpyHostToDevice, c.stream));
-    }
-    RMX_HIP(hipMemcpyAsync(c.b_ichunk.p, ichunk.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice,
-                           c.stream));
-    c.b_off.alloc((int64_t)n_inst + 1);
-    c.b_node.alloc(n);
-    c.b_seed.alloc(n_inst);
-    c.L0.uoff.alloc((int64_t)n_inst + 1);
-    c.L0.uinst.alloc(n);
-    RMX_HIP(hipMemcpyAsync(c.b_off.p, c.h_boff.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice,
-                           c.stream));
-    RMX_HIP(hipMemcpyAsync(c.L0.uoff.p, c.h_boff.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice,
-                           c.stream));
-    RMX_HIP(hipMemcpyAsync(c.b_node.p, inst.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
-    RMX_HIP(hipMemcpyAsync(c.L0.uinst.p, inst.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
-    RMX_HIP(hipMemcpyAsync(c.b_seed.p, sd.data(), sizeof(uint64_t) * n_inst, hipMemcpyHostToDevice, c.stream));
-    c.sync();   // host staging vectors go out of scope
+      return "";
+    });
+    if (*bad) return fail(ctx, RMX_EINVAL, bad);
+    Ctx &c = ctx->c;
+    RMX_HIP(hipSetDevice(c.device));
+    c.nb = 0;
+    upload_graph_replicated(c, n_graph, graph_off, n_seed, n, m, out_ptr, out_idx, w, node_w);
+    const int32_t n_inst = n_graph * n_seed;
+    std::vector<int32_t> node_off((size_t)n_inst + 1);
+    int64_t at = 0;
+    for (int32_t g = 0; g < n_graph; ++g)
+      for (int32_t r = 0; r < n_seed; ++r) {
+        node_off[(size_t)g * n_seed + r] = (int32_t)at;
+        at += graph_off[g + 1] - graph_off[g];
+      }
+    node_off[n_inst] = (int32_t)at;
+    install_batch(c, n_inst, node_off.data(), seeds, nullptr);
     ctx->names.clear();
     return RMX_OK;
   });
 }
 
 }  // extern "C"
+
+namespace {
+
+// The instances of a batch on level 0 (node_off[n_inst + 1], one seed each):
+// per-instance chunks of module ids, node offsets, the instance of every node
+// (inst on the host, or null when L0.uinst already holds it) and the seeds.
+void install_batch(Ctx &c, int32_t n_inst, const int32_t *node_off, const uint64_t *seeds, const uint32_t *inst) {
+  const int32_t n = c.N;
+  c.nb = n_inst;
+  c.h_boff.assign(node_off, node_off + n_inst + 1);
+  std::vector<uint64_t> sd(n_inst);
+  for (int32_t I = 0; I < n_inst; ++I) sd[I] = mix64(seeds[I] ^ 0x6a09e667f3bcc909ull);
+  std::vector<uint32_t> cbeg, cend, ichunk(1, 0u);
+  for (int32_t I = 0; I < n_inst; ++I) {
+    for (int64_t a = node_off[I]; a < node_off[I + 1]; a += kTotChunk) {
+      cbeg.push_back((uint32_t)a);
+      cend.push_back((uint32_t)std::min<int64_t>(a + kTotChunk, node_off[I + 1]));
+    }
+    ichunk.push_back((uint32_t)cbeg.size());
+  }
+  c.n_chunks = (int32_t)cbeg.size();
+  c.b_cbeg.alloc(std::max<int64_t>(1, c.n_chunks));
+  c.b_cend.alloc(std::max<int64_t>(1, c.n_chunks));
+  c.b_ichunk.alloc((int64_t)n_inst + 1);
+  c.b_cpart.alloc(4 * std::max<int64_t>(1, c.n_chunks));
+  if (c.n_chunks > 0) {
+    RMX_HIP(hipMemcpyAsync(c.b_cbeg.p, cbeg.data(), sizeof(uint32_t) * c.n_chunks, hipMemcpyHostToDevice, c.stream));
+    RMX_HIP(hipMemcpyAsync(c.b_cend.p, cend.data(), sizeof(uint32_t) * c.n_chunks, hipMemcpyHostToDevice, c.stream));
+  }
+  RMX_HIP(hipMemcpyAsync(c.b_ichunk.p, ichunk.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice,
+                         c.stream));
+  c.b_off.alloc((int64_t)n_inst + 1);
+  c.b_node.alloc(n);
+  c.b_seed.alloc(n_inst);
+  c.L0.uoff.alloc((int64_t)n_inst + 1);
+  c.L0.uinst.alloc(n);   // already filled when inst is null
+  RMX_HIP(hipMemcpyAsync(c.b_off.p, c.h_boff.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice,
+                         c.stream));
+  RMX_HIP(hipMemcpyAsync(c.L0.uoff.p, c.h_boff.data(), sizeof(uint32_t) * (n_inst + 1), hipMemcpyHostToDevice,
+                         c.stream));
+  if (inst) RMX_HIP(hipMemcpyAsync(c.L0.uinst.p, inst, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
+  RMX_HIP(hipMemcpyAsync(c.b_node.p, c.L0.uinst.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, c.stream));
+  RMX_HIP(hipMemcpyAsync(c.b_seed.p, sd.data(), sizeof(uint64_t) * n_inst, hipMemcpyHostToDevice, c.stream));
+  c.sync();   // host staging vectors go out of scope
+}
+
+}  // namespace
 
 namespace {
 
@@ -904,26 +960,12 @@ int batched_attempts(rmx_ctx *ctx, const rmx_partition_opts &o, rmx_partition_re
   }
   RMX_HIP(hipMemcpyAsync(nw.data(), c.node_w.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
   c.sync();
-  const int64_t N = (int64_t)A * n, M = (int64_t)A * m;
-  std::vector<int64_t> uptr((size_t)N + 1);
-  std::vector<int32_t> uidx((size_t)M), off((size_t)A + 1);
-  std::vector<double> uw((size_t)M), unw((size_t)N);
+  const int64_t N = (int64_t)A * n;
+  std::vector<int64_t> ptr((size_t)n + 1);
+  for (int32_t i = 0; i <= n; ++i) ptr[i] = p32[i];
+  const int32_t goff[2] = {0, n};
   std::vector<uint64_t> seeds((size_t)A);
-  for (int a = 0; a < A; ++a) {
-    off[a] = (int32_t)((int64_t)a * n);
-    seeds[a] = o.seed + (uint64_t)a;
-    const int64_t bn = (int64_t)a * n, bm = (int64_t)a * m;
-    for (int32_t i = 0; i < n; ++i) {
-      uptr[bn + i] = bm + p32[i];
-      unw[bn + i] = nw[i];
-    }
-    for (int64_t k = 0; k < m; ++k) {
-      uidx[bm + k] = (int32_t)(idx[k] + bn);
-      uw[bm + k] = w[k];
-    }
-  }
-  uptr[N] = M;
-  off[A] = (int32_t)N;
+  for (int a = 0; a < A; ++a) seeds[a] = o.seed + (uint64_t)a;
   rmx_ctx *t = nullptr;
   if (rmx_create(&t, c.device) != RMX_OK) return fail(ctx, RMX_EDEVICE, "attempts: no second context");
   struct Drop {
@@ -938,8 +980,8 @@ int batched_attempts(rmx_ctx *ctx, const rmx_partition_opts &o, rmx_partition_re
   auto check = [&](int rc, const char *what) {
     if (rc != RMX_OK) throw HipError(std::string("attempts: ") + what + ": " + rmx_last_error(t));
   };
-  check(rmx_batch_upload(t, A, off.data(), seeds.data(), (int32_t)N, M, uptr.data(), uidx.data(), uw.data(),
-                         unw.data()),
+  check(rmx_batch_upload_seeds(t, 1, goff, A, seeds.data(), n, m, ptr.data(), (const int32_t *)idx.data(), w.data(),
+                               nw.data()),
         "upload");
   check(rmx_initiate(t, nullptr, nullptr, nullptr), "initiate");
   rmx_partition_opts o1 = o;

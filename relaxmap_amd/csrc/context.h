@@ -326,6 +326,13 @@ struct Ctx {
 // ----- graph.hip
 void upload_graph(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx,
                   const double *w, const double *node_w);
+// A batch of G networks (a union out-CSR, network g owning rows goff[g]..)
+// under R seeds each: the networks are uploaded once and copied on the
+// device into the union of the G * R instances (instance g * R + r is a copy
+// of network g), level 0 then equal to what upload_graph builds from that
+// union; L0.uinst holds the instance of every node.
+void upload_graph_replicated(Ctx &c, int32_t G, const int32_t *goff, int32_t R, int32_t n, int64_t m,
+                             const int64_t *out_ptr, const int32_t *out_idx, const double *w, const double *node_w);
 void build_in_csr(Ctx &c, Level &L, const double *raw, DevBuf<double> *raw_in);
 void super_in_weights(Ctx &c, const Level &L, double *in_w);   // FP64 in-weights of a contracted level
 void build_buckets(Ctx &c, Level &L);

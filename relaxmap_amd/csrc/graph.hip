@@ -7,6 +7,7 @@
 // reference's push_back loop produces it (OmpRelaxmap.cpp:158-162).
 #include "context.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace rmx {
@@ -149,8 +150,11 @@ unsigned gs_grid(int64_t n) {
 
 }  // namespace
 
-void upload_graph(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx,
-                  const double *w, const double *node_w) {
+namespace {
+
+// the out-CSR, raw weights and node weights to the device, and the in-CSR
+void upload_csr(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx, const double *w,
+                const double *node_w) {
   if (n <= 0) throw HipError("network has no nodes");
   if (m >= (int64_t)0xffffffffLL) throw HipError("more than 2^32-2 arcs");
   hipStream_t s = c.stream;
@@ -177,6 +181,140 @@ void upload_graph(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const in
   RMX_HIP(hipMemcpyAsync(c.node_w.p, nw.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
   c.sync();   // host staging vectors go out of scope
   build_in_csr(c, L, c.raw_w.p, &c.raw_in);
+}
+
+// Copies of the networks, one workgroup row per instance k (gridDim.y
+// strides over the instances): instance k = g * R + r is network g, its
+// nodes at NO[k], its arcs at AO[k].  Network g's rows are GO[g].. and its
+// arcs GA[g].. in the uploaded union; every arc of it stays inside it, and
+// its in-arcs occupy the same range GA[g].. of the in-CSR.
+__global__ void k_rep_nodes(int32_t K, int32_t R, const uint32_t *__restrict__ NO, const uint32_t *__restrict__ AO,
+                            const uint32_t *__restrict__ GO, const uint32_t *__restrict__ GA,
+                            const uint32_t *__restrict__ gop, const uint32_t *__restrict__ gip,
+                            const double *__restrict__ gnw, uint32_t *__restrict__ op, uint32_t *__restrict__ ip,
+                            double *__restrict__ nw, uint32_t *__restrict__ inst) {
+  for (int32_t k = blockIdx.y; k < K; k += gridDim.y) {
+    const int32_t g = k / R;
+    const uint32_t n0 = NO[k], nk = NO[k + 1] - n0, g0 = GO[g];
+    const uint32_t base = AO[k] - GA[g];   // arc offset of the copy (mod 2^32)
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nk; i += gridDim.x * blockDim.x) {
+      op[n0 + i] = base + gop[g0 + i];
+      ip[n0 + i] = base + gip[g0 + i];
+      nw[n0 + i] = gnw[g0 + i];
+      inst[n0 + i] = (uint32_t)k;
+    }
+  }
+}
+
+__global__ void k_rep_arcs(int32_t K, int32_t R, const uint32_t *__restrict__ NO, const uint32_t *__restrict__ AO,
+                           const uint32_t *__restrict__ GO, const uint32_t *__restrict__ GA,
+                           const uint32_t *__restrict__ goi, const uint32_t *__restrict__ gii,
+                           const double *__restrict__ gw, const double *__restrict__ gwi, uint32_t *__restrict__ oi,
+                           uint32_t *__restrict__ ii, double *__restrict__ w, double *__restrict__ wi) {
+  for (int32_t k = blockIdx.y; k < K; k += gridDim.y) {
+    const int32_t g = k / R;
+    const uint32_t a0 = AO[k], mk = AO[k + 1] - a0, ga = GA[g];
+    const uint32_t shift = NO[k] - GO[g];   // node offset of the copy (mod 2^32)
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < mk; j += gridDim.x * blockDim.x) {
+      oi[a0 + j] = goi[ga + j] + shift;
+      ii[a0 + j] = gii[ga + j] + shift;
+      w[a0 + j] = gw[ga + j];
+      wi[a0 + j] = gwi[ga + j];
+    }
+  }
+}
+
+__global__ void k_rep_end(uint32_t N, uint32_t M, uint32_t *__restrict__ op, uint32_t *__restrict__ ip) {
+  op[N] = M;
+  ip[N] = M;
+}
+
+}  // namespace
+
+void upload_graph(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const int32_t *out_idx,
+                  const double *w, const double *node_w) {
+  upload_csr(c, n, m, out_ptr, out_idx, w, node_w);
+  build_buckets(c, c.L0);
+  c.initiated = false;
+}
+
+void upload_graph_replicated(Ctx &c, int32_t G, const int32_t *goff, int32_t R, int32_t n, int64_t m,
+                             const int64_t *out_ptr, const int32_t *out_idx, const double *w,
+                             const double *node_w) {
+  const int64_t K = (int64_t)G * R, N = (int64_t)n * R, M = m * R;
+  if (G < 1 || R < 1) throw HipError("no networks or no seeds");
+  if (N > 0x7fffffffLL || M > 0x7fffffffLL) throw HipError("the copies exceed 2^31 - 1 nodes or arcs");
+  upload_csr(c, n, m, out_ptr, out_idx, w, node_w);   // the networks once, with their in-CSR
+  hipStream_t s = c.stream;
+  Level &L = c.L0;
+  // instance tables: NO, AO (K + 1 each), then GO, GA (G + 1 each)
+  std::vector<uint32_t> tab((size_t)(2 * (K + 1) + 2 * (G + 1)));
+  uint32_t *NO = tab.data(), *AO = NO + (K + 1), *GO = AO + (K + 1), *GA = GO + (G + 1);
+  int64_t nn = 0, mm = 0;
+  for (int32_t g = 0; g < G; ++g) {
+    GO[g] = (uint32_t)goff[g];
+    GA[g] = (uint32_t)out_ptr[goff[g]];
+    const int64_t ng = goff[g + 1] - goff[g], mg = out_ptr[goff[g + 1]] - out_ptr[goff[g]];
+    for (int32_t r = 0; r < R; ++r) {
+      NO[(int64_t)g * R + r] = (uint32_t)nn;
+      AO[(int64_t)g * R + r] = (uint32_t)mm;
+      nn += ng;
+      mm += mg;
+    }
+  }
+  NO[K] = (uint32_t)N;
+  AO[K] = (uint32_t)M;
+  GO[G] = (uint32_t)n;
+  GA[G] = (uint32_t)m;
+  DevBuf<uint32_t> dtab(c.cache), op(c.cache), ip(c.cache), oi(c.cache), ii(c.cache), inst(c.cache);
+  DevBuf<double> rw(c.cache), rwi(c.cache), nw(c.cache);
+  dtab.alloc((int64_t)tab.size());
+  RMX_HIP(hipMemcpyAsync(dtab.p, tab.data(), sizeof(uint32_t) * tab.size(), hipMemcpyHostToDevice, s));
+  const uint32_t *dNO = dtab.p, *dAO = dNO + (K + 1), *dGO = dAO + (K + 1), *dGA = dGO + (G + 1);
+  op.alloc(N + 1);
+  ip.alloc(N + 1);
+  oi.alloc(M);
+  ii.alloc(M);
+  rw.alloc(M);
+  rwi.alloc(M);
+  nw.alloc(N);
+  inst.alloc(N);
+  const int64_t nmax = [&] {
+    int64_t x = 1;
+    for (int32_t g = 0; g < G; ++g) x = std::max<int64_t>(x, goff[g + 1] - goff[g]);
+    return x;
+  }();
+  const int64_t mmax = [&] {
+    int64_t x = 1;
+    for (int32_t g = 0; g < G; ++g) x = std::max<int64_t>(x, out_ptr[goff[g + 1]] - out_ptr[goff[g]]);
+    return x;
+  }();
+  const unsigned gy = (unsigned)std::min<int64_t>(K, 65535);
+  const dim3 gn((unsigned)std::min<int64_t>((nmax + 255) / 256, 64), gy);
+  k_rep_nodes<<<gn, 256, 0, s>>>((int32_t)K, R, dNO, dAO, dGO, dGA, L.out_ptr.p, L.in_ptr.p, c.node_w.p, op.p, ip.p,
+                                 nw.p, inst.p);
+  RMX_LAUNCH_CHECK();
+  k_rep_end<<<1, 1, 0, s>>>((uint32_t)N, (uint32_t)M, op.p, ip.p);
+  RMX_LAUNCH_CHECK();
+  if (M > 0) {
+    const dim3 ga((unsigned)std::min<int64_t>((mmax + 255) / 256, 256), gy);
+    k_rep_arcs<<<ga, 256, 0, s>>>((int32_t)K, R, dNO, dAO, dGO, dGA, L.out_idx.p, L.in_idx.p, c.raw_w.p,
+                                  c.raw_in.p, oi.p, ii.p, rw.p, rwi.p);
+    RMX_LAUNCH_CHECK();
+  }
+  L.out_ptr.swap(op);
+  L.in_ptr.swap(ip);
+  L.out_idx.swap(oi);
+  L.in_idx.swap(ii);
+  c.raw_w.swap(rw);
+  c.raw_in.swap(rwi);
+  c.node_w.swap(nw);
+  L.uinst.swap(inst);
+  L.n = (int32_t)N;
+  L.m = M;
+  c.N = (int32_t)N;
+  c.E = M;
+  c.sync();   // the networks' buffers and the host tables go out of scope
   build_buckets(c, L);
   c.initiated = false;
 }

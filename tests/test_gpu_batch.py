@@ -92,6 +92,41 @@ def test_attempts_as_one_batch():
     assert np.array_equal(part, bpart[best * g.n:(best + 1) * g.n] - best * g.n)
 
 
+def test_seeded_copies_equal_the_host_union():
+    """rmx_batch_upload_seeds (each network sent once and copied on the
+    device, one copy per seed) against rmx_batch_upload of the same union
+    built on the host: identical flows, per-instance codelengths, module
+    counts, iterations and partitions, in both precisions.  One network has
+    no arcs (copies of an arc-less node range)."""
+    from types import SimpleNamespace
+    from relaxmap_amd import Network
+    from relaxmap_amd.batch import graphs_union, union_of
+    g = _graphs()
+    empty = SimpleNamespace(n=5, m=0, ptr=np.zeros(6, np.int64), idx=np.zeros(0, np.int32), w=np.zeros(0))
+    graphs = [g[0], empty, g[2], g[1]]
+    seeds = [3, 1, 4]
+    _, node_off, iseeds, ptr, idx, w = union_of(graphs, seeds)
+    goff, gptr, gidx, gw = graphs_union(graphs)
+    for prec in ("mixed", "fp64"):
+        out = []
+        for seeded in (False, True):
+            with Network(precision=prec) as net:
+                if seeded:
+                    net.batch_upload_seeds(goff, len(seeds), iseeds, gptr, gidx, gw)
+                else:
+                    net.batch_upload(node_off, iseeds, ptr, idx, w)
+                net.initiate()
+                p = net.flows()
+                net.run(seed=1)
+                out.append((p, net.batch_results(), net.partition()))
+        (p0, r0, q0), (p1, r1, q1) = out
+        print(f"[gpu] {prec}: {len(r0[0])} instances, L[0] {r0[0][0]:.9f} / {r1[0][0]:.9f}", flush=True)
+        assert np.array_equal(p0, p1)
+        for a, b in zip(r0, r1):
+            assert np.array_equal(a, b)
+        assert np.array_equal(q0, q1)
+
+
 # ---------------------------------------------------------------- one batch
 def _oracle_codelength(g, part):
     from oracle.pyoracle import OracleNet

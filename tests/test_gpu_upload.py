@@ -1,5 +1,5 @@
-"""Input checks of rmx_graph_upload / rmx_batch_upload (api.cpp check_csr):
-each broken rule is reported, on a small network (one host thread) and on
+"""Input checks of rmx_graph_upload / rmx_batch_upload / rmx_batch_upload_seeds
+(api.cpp check_csr): each broken rule is reported, on a small network (one host thread) and on
 one with more than 2^20 arcs (row ranges checked by several threads), with
 the defect in a late row."""
 import numpy as np
@@ -62,6 +62,28 @@ def test_batch_upload_rejects_cross_instance_arcs(n):
     with Network() as net:
         with pytest.raises(RuntimeError, match="joins two instances"):
             net.batch_upload(node_off, [1, 2], ptr, idx, w)
+
+
+def test_batch_upload_seeds_rejects():
+    """rmx_batch_upload_seeds: an arc between two networks, offsets that do
+    not span the nodes, and a seed count that does not match are refused; the
+    context then still takes a valid batch, whose copies of one network have
+    identical flows summing to 1 each."""
+    from relaxmap_amd import Network
+    n = 1000
+    ptr, idx, w = _net(n)              # the ring joins the two halves
+    with Network() as net:
+        with pytest.raises(RuntimeError, match="joins two networks"):
+            net.batch_upload_seeds([0, n // 2, n], 2, [1, 2, 3, 4], ptr, idx, w)
+        with pytest.raises(RuntimeError, match="does not span"):
+            net.batch_upload_seeds([0, n - 1], 2, [1, 2], ptr, idx, w)
+        with pytest.raises(ValueError, match="one seed per"):
+            net.batch_upload_seeds([0, n], 2, [1], ptr, idx, w)
+        net.batch_upload_seeds([0, n], 2, [1, 2], ptr, idx, w)
+        net.initiate()
+        p = net.flows()
+        assert np.array_equal(p[:n], p[n:])
+        assert abs(p[:n].sum() - 1.0) <= 1e-12
 
 
 def test_precision_change_needs_initiate():
