@@ -53,17 +53,15 @@ __global__ void k_intra_ptr(const uint32_t *__restrict__ optr, int32_t n, int64_
 
 // the flagged arcs, compacted; a null weight destination is left out
 __global__ void k_intra_copy(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ apos, int64_t m,
-                             const uint32_t *__restrict__ oidx, const double *__restrict__ w,
-                             const float *__restrict__ wf, uint32_t *__restrict__ sidx, double *__restrict__ sw,
-                             float *__restrict__ swf) {
+                             const uint32_t *__restrict__ oidx, const long long *__restrict__ wx,
+                             uint32_t *__restrict__ sidx, long long *__restrict__ swx) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; k < m; k += stride) {
     if (!flag[k]) continue;
     const uint32_t j = apos[k];
     sidx[j] = oidx[k];
-    if (sw) sw[j] = w[k];
-    if (swf) swf[j] = wf[k];
+    swx[j] = wx[k];
   }
 }
 
@@ -299,15 +297,13 @@ int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t se
     S.m = m_in;
     S.out_ptr.alloc((int64_t)n + 1);
     S.out_idx.alloc(m_in);
-    S.out_w.alloc(m_in);
-    S.out_wf.alloc(m_in);
-    S.in_w.alloc(m_in);
-    S.in_wf.alloc(m_in);
+    S.out_wx.alloc(m_in);   // the sweeps read only the fixed-point weights
+    S.in_wx.alloc(m_in);
     k_intra_ptr<<<grid_for((int64_t)n + 1, 256), 256, 0, s>>>(L0.out_ptr.p, n, m, apos.p, m_in, S.out_ptr.p);
     RMX_LAUNCH_CHECK();
     if (m_in > 0) {
       k_intra_copy<<<(unsigned)std::min<int64_t>(16384, (m + 255) / 256), 256, 0, s>>>(
-          flag.p, apos.p, m, L0.out_idx.p, L0.out_w.p, L0.out_wf.p, S.out_idx.p, S.out_w.p, S.out_wf.p);
+          flag.p, apos.p, m, L0.out_idx.p, L0.out_wx.p, S.out_idx.p, S.out_wx.p);
       RMX_LAUNCH_CHECK();
     }
     c.sync();
@@ -334,8 +330,7 @@ int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t se
     RMX_LAUNCH_CHECK();
     if (m_in > 0) {
       k_intra_copy<<<(unsigned)std::min<int64_t>(16384, (m + 255) / 256), 256, 0, s>>>(
-          flag.p, apos.p, m, L0.in_idx.p, L0.in_w.p, L0.in_wf.p, S.in_idx.p,
-          c.precision == 1 ? nullptr : S.in_w.p, c.precision == 1 ? S.in_wf.p : nullptr);   // the weights the sweeps read
+          flag.p, apos.p, m, L0.in_idx.p, L0.in_wx.p, S.in_idx.p, S.in_wx.p);
       RMX_LAUNCH_CHECK();
     }
     c.sync();
@@ -344,11 +339,8 @@ int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t se
   S.us.alloc(n);
   S.nmem.alloc(n);
   S.mod.alloc(n);
-  {   // the node level's weight mode (FP64 or FP32 flows) over the intra arcs
-    ArcW aw = arcw_out(c, L0);
-    aw.w = S.out_w.p;
-    aw.wf = S.out_wf.p;
-    k_sub_units<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(S.out_ptr.p, n, aw, L0.us.p, S.us.p);
+  {   // out-flow of every unit over its intra arcs
+    k_sub_units<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(S.out_ptr.p, n, arcw_out(c, S), L0.us.p, S.us.p);
     RMX_LAUNCH_CHECK();
   }
   // singleton sub-modules (Module(i, node), OmpRelaxmap.cpp:686-690)

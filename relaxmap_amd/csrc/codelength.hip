@@ -347,23 +347,9 @@ __global__ __launch_bounds__(kRedBlock) void k_p5_partials(const double *__restr
 
 }  // namespace
 
-ArcW arcw_out(const Ctx &c, const Level &L) {
-  ArcW a;
-  a.w = L.out_w.p;
-  a.wf = L.out_wf.p;
-  a.wx = L.out_wx.p;
-  a.mode = (&L == &c.L1) ? 2 : (c.precision == 1 ? 1 : 0);
-  return a;
-}
+ArcW arcw_out(const Ctx &, const Level &L) { return ArcW{L.out_wx.p}; }
 
-ArcW arcw_in(const Ctx &c, const Level &L) {
-  ArcW a;
-  a.w = L.in_w.p;
-  a.wf = L.in_wf.p;
-  a.wx = L.in_wx.p;
-  a.mode = (&L == &c.L1) ? 2 : (c.precision == 1 ? 1 : 0);
-  return a;
-}
+ArcW arcw_in(const Ctx &, const Level &L) { return ArcW{L.in_wx.p}; }
 
 void module_totals_async(Ctx &c) {
   if (c.nb > 0) {   // per instance: S for the sweeps, L for the driver

@@ -5,16 +5,12 @@
 
 namespace rmx {
 
-// Arc weight in fixed point: level 0 converts the FP64 (or, in mixed
-// precision, FP32) flow of the arc; level 1 stores exact fixed-point sums.
+// Arc weights in fixed point, one array per direction and level: at the node
+// level the FP64 (or, in mixed precision, FP32) flow of each arc converted
+// once by the steady state, at a contracted level the exact fixed-point sums.
 struct ArcW {
-  const double *w;
-  const float *wf;
   const long long *wx;
-  int mode;   // 0: FP64 flows, 1: FP32 flows, 2: fixed-point sums
-  __device__ long long fxw(uint32_t k) const {
-    return mode == 2 ? wx[k] : (mode == 1 ? fx_of((double)wf[k]) : fx_of(w[k]));
-  }
+  __device__ long long fxw(uint32_t k) const { return wx[k]; }
 };
 
 ArcW arcw_out(const Ctx &c, const Level &L);

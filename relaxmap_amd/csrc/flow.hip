@@ -104,12 +104,14 @@ __global__ void k_pack_ps(const double *__restrict__ p, const double *__restrict
 }
 
 __global__ void k_in_flows(const uint32_t *__restrict__ iidx, const double *__restrict__ raw_in, int64_t m,
-                           const double2 *__restrict__ ps, double *__restrict__ in_w, float *__restrict__ in_wf) {
+                           const double2 *__restrict__ ps, int mixed, double *__restrict__ in_w,
+                           float *__restrict__ in_wf, long long *__restrict__ in_wx) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
     const double2 t = ps[iidx[j]];
     const double f = t.x * (raw_in[j] / t.y);
     if (in_w) in_w[j] = f;
     if (in_wf) in_wf[j] = (float)f;
+    in_wx[j] = fx_of(mixed ? (double)(float)f : f);
   }
 }
 
@@ -746,7 +748,7 @@ __global__ __launch_bounds__(kRedBlock) void k_bconst(const uint32_t *__restrict
 // --- flows on arcs and node statistics (Module.cpp:267-325) ---------------
 template <int G>
 __global__ void k_flows(const uint32_t *__restrict__ optr, int32_t n, double *__restrict__ out_w,
-                        float *__restrict__ out_wf, const double *__restrict__ x,
+                        float *__restrict__ out_wf, long long *__restrict__ out_wx, const double *__restrict__ x,
                         const double *__restrict__ tau, const uint8_t *__restrict__ dang, int mixed,
                         double *__restrict__ p_out, Stat4 *__restrict__ us, double *__restrict__ qlit) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -761,9 +763,11 @@ __global__ void k_flows(const uint32_t *__restrict__ optr, int32_t n, double *__
     double f = pr * out_w[k];
     float ff = (float)f;
     out_w[k] = f;
-    if (mixed) out_wf[k] = ff;   // the FP64 sweeps read out_w only
+    if (mixed) out_wf[k] = ff;   // the contraction's weights; the sweeps read out_wx
     sf += f;
-    sx += fx_of(mixed ? (double)ff : f);
+    const long long fx = fx_of(mixed ? (double)ff : f);
+    out_wx[k] = fx;
+    sx += fx;
   }
   sf = group_sum<G>(sf);
   sx = group_sum_ll<G>(sx);
@@ -1287,16 +1291,21 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   c.sync();   // the relabelled copy goes back to the cache
 
   // flows, node statistics and the literal node exits
-  k_flows<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, L.out_w.p, L.out_wf.p, c.x.p, L.tau.p,
-                                                              c.dangling.p, mixed ? 1 : 0, L.p.p, L.us.p, c.node_q.p);
+  // the sweeps' fixed-point arc weights: the same conversion fxw made of
+  // out_wf / in_wf (mixed) or out_w / in_w (FP64), done once here
+  L.out_wx.alloc(m);
+  L.in_wx.alloc(m);
+  k_flows<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L.out_ptr.p, n, L.out_w.p, L.out_wf.p, L.out_wx.p, c.x.p,
+                                                              L.tau.p, c.dangling.p, mixed ? 1 : 0, L.p.p, L.us.p,
+                                                              c.node_q.p);
   RMX_LAUNCH_CHECK();
   if (m > 0) {
     DevBuf<double2> ps(c.cache);
     ps.alloc(n);
     k_pack_ps<<<grid_for(n, 256), 256, 0, s>>>(c.x.p, rowsum.p, n, ps.p);
     RMX_LAUNCH_CHECK();
-    k_in_flows<<<stride_grid(m), 256, 0, s>>>(L.in_idx.p, c.raw_in.p, m, ps.p, mixed ? nullptr : L.in_w.p,
-                                               mixed ? L.in_wf.p : nullptr);
+    k_in_flows<<<stride_grid(m), 256, 0, s>>>(L.in_idx.p, c.raw_in.p, m, ps.p, mixed ? 1 : 0,
+                                               mixed ? nullptr : L.in_w.p, mixed ? L.in_wf.p : nullptr, L.in_wx.p);
     RMX_LAUNCH_CHECK();
     c.sync();   // ps goes back to the cache
   }

@@ -366,12 +366,9 @@ __device__ inline int lds_find(const uint32_t *keys, int H, uint32_t m) {
   }
 }
 
-// Fixed-point weight of out-arc (out) or in-arc k; both directions share the
-// weight mode of the level.
+// Fixed-point weight of out-arc (out) or in-arc k.
 __device__ inline long long arc_fxw(const EvalArgs &a, bool out, uint32_t k) {
-  if (a.aout.mode == 2) return *((out ? a.aout.wx : a.ain.wx) + k);
-  if (a.aout.mode == 1) return fx_of((double)*((out ? a.aout.wf : a.ain.wf) + k));
-  return fx_of(*((out ? a.aout.w : a.ain.w) + k));
+  return *((out ? a.aout.wx : a.ain.wx) + k);
 }
 
 // Aggregates arcs j = first, first + STEP, ... of a unit's out-list followed
@@ -1938,7 +1935,7 @@ static void profile_eval(Ctx &c, int bucket, hipStream_t s) {
   const double visits = u64(c.h_cnt, cVisitsLo) - u64(before, cVisitsLo);
   const double cands = u64(c.h_cnt, cCandsLo) - u64(before, cCandsLo);
   const double units = (double)c.h_cnt[cUnits] - (double)before[cUnits];
-  const double wbytes = (c.level == 0 && c.precision == 1) ? 4.0 : 8.0;
+  const double wbytes = 8.0;   // fixed-point arc weights at every level
   KernelStat &k = c.prof.get(names[c.precision == 1 ? 1 : 0][bucket]);
   k.launches += 1;
   k.ms += ms;
@@ -1968,7 +1965,7 @@ static void profile_hubs(Ctx &c, const HubParts &H, const float *tk, bool f32, h
   const double cands = u64(c.h_cnt, cCandsLo) - u64(before, cCandsLo);
   const double hubs = (double)c.h_cnt[cUnits] - (double)before[cUnits];
   const double chunks = tot[0], parts = tot[1];
-  const double w = (c.level == 0 && c.precision == 1) ? 4.0 : 8.0;
+  const double w = 8.0;   // fixed-point arc weight
   c.prof.add("k_hubp_plan", tk[0], hubs * 40.0);
   c.prof.add("k_hubp_owners", tk[1], (chunks + parts) * 4.0 + hubs * 16.0);
   c.prof.add("k_hubp_count", tk[2], arcs * 12.0);
