@@ -341,9 +341,19 @@ __device__ inline uint32_t ld_key(const uint32_t *keys, uint32_t h) {
   return keys[h];
 }
 
+// LDS tables try the compare-and-swap first: one round trip both when the
+// key is new and when it is already there (global tables read first, since
+// an atomic there costs more than a load).
 template <bool GLOBAL = false>
 __device__ inline uint32_t lds_insert(uint32_t *keys, int H, uint32_t m) {
   uint32_t h = hash_u32(m) & (uint32_t)(H - 1);
+  if (!GLOBAL) {
+    while (true) {
+      const uint32_t old = atomicCAS(&keys[h], kEmptyKey, m);
+      if (old == kEmptyKey || old == m) return h;
+      h = (h + 1) & (uint32_t)(H - 1);
+    }
+  }
   while (true) {
     uint32_t k = ld_key<GLOBAL>(keys, h);
     if (k == m) return h;
@@ -970,12 +980,8 @@ __device__ inline uint32_t part_of(uint32_t m, uint32_t np) {
 __device__ inline int lds_insert_bounded(uint32_t *keys, int H, uint32_t m) {
   uint32_t h = hash_u32(m) & (uint32_t)(H - 1);
   for (int probe = 0; probe < H; ++probe) {
-    const uint32_t k = keys[h];
-    if (k == m) return (int)h;
-    if (k == kEmptyKey) {
-      const uint32_t old = atomicCAS(&keys[h], kEmptyKey, m);
-      if (old == kEmptyKey || old == m) return (int)h;
-    }
+    const uint32_t old = atomicCAS(&keys[h], kEmptyKey, m);
+    if (old == kEmptyKey || old == m) return (int)h;
     h = (h + 1) & (uint32_t)(H - 1);
   }
   return -1;   // table full
