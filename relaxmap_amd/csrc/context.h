@@ -231,6 +231,10 @@ struct Ctx {
   DevBuf<Stat4> ms;                     // fixed-point sums: flow, teleport, dangling, exit arcs
   DevBuf<double> qv;                    // exit probability of every module (0 when empty), during a sweep
   DevBuf<int32_t> mNodes, mUnits, mStale;  // original nodes, units at the current level, stale count
+  // one bit per module: it holds exactly one unit (mUnits == 1), rebuilt at
+  // the start of every sweep and kept by the commits' apply pass; 4 MB at
+  // 33.5M modules, so the singleton rule's reads hit the cache
+  DevBuf<uint32_t> mOne;
 
   // A batch of independent instances (rmx_batch_upload): instance I owns the
   // node ids [b_off[I], b_off[I + 1]) and no arc joins two instances.

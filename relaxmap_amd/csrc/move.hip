@@ -78,6 +78,7 @@ struct EvalArgs {
   const Stat4 *us;
   const Stat4 *ms;
   const int32_t *mUnits, *mStale;
+  const uint32_t *mOne;   // bit m: module m holds one unit
   const uint8_t *act;
   int all_active;
   uint64_t seed;
@@ -249,9 +250,11 @@ __device__ inline void unit_ctx(const EvalArgs &a, uint32_t u, long long fo_o, l
   unit_ctx(a, load_unit(a, u), fo_o, fi_o, U);
 }
 
+__device__ inline bool one_unit(const EvalArgs &a, uint32_t m) { return (a.mOne[m >> 5] >> (m & 31u)) & 1u; }
+
 __device__ inline bool allowed(const EvalArgs &a, const UnitCtx &U, uint32_t m) {
   // a singleton may join another singleton only if that module's id is smaller
-  return !(m > U.o && U.single && a.mUnits[m] == 1);
+  return !(m > U.o && U.single && one_unit(a, m));
 }
 
 __device__ inline void record_best(const EvalArgs &a, uint32_t u, const Best &b) {
@@ -432,17 +435,17 @@ __device__ inline void eval_cands(const EvalArgs &a, const UnitCtx &U, const uin
       mm[q] = j < n ? keys[sl[q]] : U.o;
     }
     Stat4 M[UC];
-    int32_t mu[UC];
+    bool mu[UC];
 #pragma unroll
     for (int q = 0; q < UC; ++q) {
       const bool c = mm[q] != U.o;
       M[q] = c ? a.ms[mm[q]] : Stat4{0, 0, 0, 0};
-      mu[q] = (c && U.single && mm[q] > U.o) ? a.mUnits[mm[q]] : 0;
+      mu[q] = c && U.single && mm[q] > U.o && one_unit(a, mm[q]);
     }
 #pragma unroll
     for (int q = 0; q < UC; ++q) {
       // a singleton may join another singleton only if that module's id is smaller
-      if (mm[q] == U.o || mu[q] == 1) continue;
+      if (mm[q] == U.o || mu[q]) continue;
       const double dl = cand_delta<F32>(U, M[q], fo[sl[q]], fi[sl[q]]);
       if (better(dl, mm[q], b) && dl < 0.0) {
         b.d = dl;
@@ -1693,14 +1696,37 @@ __global__ void k_qv_touched(const uint32_t *__restrict__ movers, const uint32_t
   }
 }
 
+__device__ inline void set_one(uint32_t *one, const int32_t *mUnits, uint32_t m) {
+  const uint32_t bit = 1u << (m & 31u);
+  if (mUnits[m] == 1) atomicOr(&one[m >> 5], bit);
+  else atomicAnd(&one[m >> 5], ~bit);
+}
+
+// New labels of the batch's movers, and the one-unit bits of the modules the
+// commits changed (every mover's old and new module; the commits are done,
+// so each bit is set from its module's final count)
 __global__ void k_apply(const uint32_t *__restrict__ movers, const uint32_t *__restrict__ counters,
                         const uint32_t *__restrict__ newmod, const uint32_t *__restrict__ stamp, uint32_t cur,
-                        uint32_t *__restrict__ mod) {
+                        uint32_t *__restrict__ mod, const int32_t *__restrict__ mUnits, uint32_t *one) {
   const uint32_t nm = counters[cMovers];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nm; i += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t u = movers[i];
-    if (stamp[u] == cur) mod[u] = newmod[u];
+    if (stamp[u] != cur) continue;
+    const uint32_t o = mod[u], n = newmod[u];
+    mod[u] = n;
+    set_one(one, mUnits, o);
+    set_one(one, mUnits, n);
   }
+}
+
+// one-unit bits of modules [0, n): 32 modules per thread
+__global__ void k_one_bits(const int32_t *__restrict__ mUnits, int32_t n, uint32_t *__restrict__ one) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w * 32 >= n) return;
+  uint32_t bits = 0;
+  const int64_t m0 = w * 32;
+  for (int b = 0; b < 32 && m0 + b < n; ++b) bits |= (mUnits[m0 + b] == 1 ? 1u : 0u) << b;
+  one[w] = bits;
 }
 
 // sort key of every bucket position: (bucket, batch) or (bucket, inactive)
@@ -1859,6 +1885,13 @@ EvalArgs sweep_args(Ctx &c, Level &L, bool prioritized, double vthresh, uint64_t
   a.ms = c.ms.p;
   a.mUnits = c.mUnits.p;
   a.mStale = c.mStale.p;
+  {   // the one-unit bits from the module counts as the sweep starts
+    const int64_t words = ((int64_t)c.N + 31) / 32;
+    c.mOne.alloc(words);
+    k_one_bits<<<grid_for(words, 256), 256, 0, c.stream>>>(c.mUnits.p, c.N, c.mOne.p);
+    RMX_LAUNCH_CHECK();
+    a.mOne = c.mOne.p;
+  }
   a.act = c.act.p;
   a.all_active = prioritized ? 0 : 1;
   a.seed = sweep_seed;
@@ -1955,7 +1988,7 @@ static void profile_eval(Ctx &c, int bucket, hipStream_t s) {
 //   owners  per chunk and partition its hub (4), per hub its prefixes (16)
 //   count   per arc: neighbour id, its module (gathered), module list entry (12)
 //   scatter per arc: module list entry, weight, partitioned key and fixed-point flow (16 + w)
-//   eval    per arc: key and flow (12); per candidate its statistics and unit count (36);
+//   eval    per arc: key and flow (12); per candidate its statistics (32; its one-unit bit is cached);
 //           per partition: bounds, the unit's own reads and its best (8 + 48 + 28)
 //   finish  per partition its best (12), per hub its own reads and flows (64)
 static void profile_hubs(Ctx &c, const HubParts &H, const float *tk, bool f32, hipStream_t s) {
@@ -1976,7 +2009,7 @@ static void profile_hubs(Ctx &c, const HubParts &H, const float *tk, bool f32, h
   c.prof.add("k_hubp_owners", tk[1], (chunks + parts) * 4.0 + hubs * 16.0);
   c.prof.add("k_hubp_count", tk[2], arcs * 12.0);
   c.prof.add("k_hubp_scatter", tk[3], arcs * (16.0 + w));
-  c.prof.add(f32 ? "k_hubp_eval<true>" : "k_hubp_eval<false>", tk[4], arcs * 12.0 + cands * 36.0 + parts * 84.0);
+  c.prof.add(f32 ? "k_hubp_eval<true>" : "k_hubp_eval<false>", tk[4], arcs * 12.0 + cands * 32.0 + parts * 84.0);
   c.prof.add("k_hubp_finish", tk[5], parts * 12.0 + hubs * 64.0);
 }
 
@@ -2417,7 +2450,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
       RMX_LAUNCH_CHECK();
     }
     k_apply<<<stride_grid(n_movers, 256), 256, 0, s>>>(c.movers.p, c.counters.p, c.newmod.p, c.stamp.p,
-                                                        c.stamp_now, L.mod.p);
+                                                        c.stamp_now, L.mod.p, c.mUnits.p, c.mOne.p);
     RMX_LAUNCH_CHECK();
     // S for the next batch
     if (c.sub_mode) instance_totals_async(c);   // of every instance
