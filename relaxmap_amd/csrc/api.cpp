@@ -62,7 +62,7 @@ void free_ctx_device(Ctx &c) {
   c.L2.free_all();
   c.Lsub.free_all();
   c.node_w.free(); c.raw_w.free(); c.dangling.free(); c.raw_in.free(); c.node_sp.free();
-  c.ms.free(); c.qv.free(); c.mNodes.free(); c.mUnits.free(); c.mStale.free(); c.mOne.free();
+  c.ms.free(); c.qv.free(); c.mNodes.free(); c.mUnits.free(); c.mStale.free(); c.mOne.free(); c.mMulti.free();
   c.b_off.free(); c.b_node.free(); c.b_seed.free(); c.b_S.free(); c.b_L.free(); c.b_C.free();
   c.b_Lold.free(); c.b_Lstep.free(); c.b_Louter.free(); c.b_nmod.free(); c.b_D.free(); c.b_R.free();
   c.b_iters.free(); c.b_step.free(); c.b_outer.free(); c.b_inner.free();

@@ -235,6 +235,7 @@ struct Ctx {
   // the start of every sweep and kept by the commits' apply pass; 4 MB at
   // 33.5M modules, so the singleton rule's reads hit the cache
   DevBuf<uint32_t> mOne;
+  DevBuf<uint32_t> mMulti;   // node level: the module held more than one node (mStale > 1) as the sweep started
 
   // A batch of independent instances (rmx_batch_upload): instance I owns the
   // node ids [b_off[I], b_off[I + 1]) and no arc joins two instances.

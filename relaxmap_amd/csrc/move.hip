@@ -77,8 +77,8 @@ struct EvalArgs {
   const uint32_t *mod;
   const Stat4 *us;
   const Stat4 *ms;
-  const int32_t *mUnits, *mStale;
-  const uint32_t *mOne;   // bit m: module m holds one unit
+  const uint32_t *mOne;     // bit m: module m holds one unit
+  const uint32_t *mMulti;   // node level, bit m: module m held more than one node as the sweep started
   const uint8_t *act;
   int all_active;
   uint64_t seed;
@@ -116,6 +116,9 @@ __device__ inline uint32_t batch_of(uint64_t seed, int batches, uint32_t u) {
   if (batches <= 1) return 0;
   return (uint32_t)(mix64(seed ^ ((uint64_t)u * 0x9e3779b97f4a7c15ull)) % (uint64_t)batches);
 }
+
+// module m holds exactly one unit
+__device__ inline bool one_unit(const EvalArgs &a, uint32_t m) { return (a.mOne[m >> 5] >> (m & 31u)) & 1u; }
 
 __device__ inline void put_mover(const EvalArgs &a, uint32_t slot, uint32_t u) {
   if (slot < a.mv_cap) a.movers_st[(size_t)(blockIdx.x & (uint32_t)(kStripes - 1)) * a.mv_cap + slot] = u;
@@ -205,7 +208,8 @@ __device__ inline double delta_empty(const UnitCtx &U) {
 struct UnitLoads {
   uint32_t o;
   Stat4 su, so;
-  int32_t units_o, stale_o;
+  bool single_o;      // the unit's module holds it alone
+  bool multi_o;       // node level: the module held more than one node when the sweep started
   double S;
   bool empty_left;   // an empty module is still available to the unit's instance
 };
@@ -218,8 +222,8 @@ __device__ inline UnitLoads load_unit(const EvalArgs &a, uint32_t u) {
   l.S = a.S_inst ? a.S_inst[I] : a.scal[kS];
   l.empty_left = a.e_cnt ? a.e_cur[I] < a.e_cnt[I] : a.empty_avail != 0;
   l.so = a.ms[l.o];
-  l.units_o = a.mUnits[l.o];
-  l.stale_o = a.node_level ? a.mStale[l.o] : 0;
+  l.single_o = one_unit(a, l.o);
+  l.multi_o = a.node_level && ((a.mMulti[l.o >> 5] >> (l.o & 31u)) & 1u);
   return l;
 }
 
@@ -242,15 +246,14 @@ __device__ inline void unit_ctx(const EvalArgs &a, const UnitLoads &l, long long
   U.S = S;
   U.phiS = plogp(S);
   U.d1 = q1 - qo;
-  U.single = l.units_o == 1;
-  U.offer_empty = l.empty_left && (a.node_level ? l.stale_o > 1 : l.units_o > 1);
+  U.single = l.single_o;
+  // (the module holds the unit itself, so more than one unit = not alone)
+  U.offer_empty = l.empty_left && (a.node_level ? l.multi_o : !l.single_o);
 }
 
 __device__ inline void unit_ctx(const EvalArgs &a, uint32_t u, long long fo_o, long long fi_o, UnitCtx &U) {
   unit_ctx(a, load_unit(a, u), fo_o, fi_o, U);
 }
-
-__device__ inline bool one_unit(const EvalArgs &a, uint32_t m) { return (a.mOne[m >> 5] >> (m & 31u)) & 1u; }
 
 __device__ inline bool allowed(const EvalArgs &a, const UnitCtx &U, uint32_t m) {
   // a singleton may join another singleton only if that module's id is smaller
@@ -1719,14 +1722,20 @@ __global__ void k_apply(const uint32_t *__restrict__ movers, const uint32_t *__r
   }
 }
 
-// one-unit bits of modules [0, n): 32 modules per thread
-__global__ void k_one_bits(const int32_t *__restrict__ mUnits, int32_t n, uint32_t *__restrict__ one) {
+// one-unit bits of modules [0, n), and with stale counts (node level) their
+// more-than-one-node bits: 32 modules per thread
+__global__ void k_one_bits(const int32_t *__restrict__ mUnits, const int32_t *__restrict__ mStale, int32_t n,
+                           uint32_t *__restrict__ one, uint32_t *__restrict__ multi) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w * 32 >= n) return;
-  uint32_t bits = 0;
+  uint32_t bits = 0, mb = 0;
   const int64_t m0 = w * 32;
-  for (int b = 0; b < 32 && m0 + b < n; ++b) bits |= (mUnits[m0 + b] == 1 ? 1u : 0u) << b;
+  for (int b = 0; b < 32 && m0 + b < n; ++b) {
+    bits |= (mUnits[m0 + b] == 1 ? 1u : 0u) << b;
+    if (mStale) mb |= (mStale[m0 + b] > 1 ? 1u : 0u) << b;
+  }
   one[w] = bits;
+  if (mStale) multi[w] = mb;
 }
 
 // sort key of every bucket position: (bucket, batch) or (bucket, inactive)
@@ -1883,14 +1892,18 @@ EvalArgs sweep_args(Ctx &c, Level &L, bool prioritized, double vthresh, uint64_t
   a.mod = L.mod.p;
   a.us = L.us.p;
   a.ms = c.ms.p;
-  a.mUnits = c.mUnits.p;
-  a.mStale = c.mStale.p;
-  {   // the one-unit bits from the module counts as the sweep starts
+  {   // the one-unit bits from the module counts as the sweep starts, and
+      // at the node level the more-than-one-node bits (the stale counts do
+      // not change during a sweep)
     const int64_t words = ((int64_t)c.N + 31) / 32;
+    const bool node = c.level == 0;
     c.mOne.alloc(words);
-    k_one_bits<<<grid_for(words, 256), 256, 0, c.stream>>>(c.mUnits.p, c.N, c.mOne.p);
+    if (node) c.mMulti.alloc(words);
+    k_one_bits<<<grid_for(words, 256), 256, 0, c.stream>>>(c.mUnits.p, node ? c.mStale.p : nullptr, c.N, c.mOne.p,
+                                                           node ? c.mMulti.p : nullptr);
     RMX_LAUNCH_CHECK();
     a.mOne = c.mOne.p;
+    a.mMulti = node ? c.mMulti.p : nullptr;
   }
   a.act = c.act.p;
   a.all_active = prioritized ? 0 : 1;
