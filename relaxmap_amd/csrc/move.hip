@@ -437,18 +437,19 @@ __device__ inline void eval_cands(const EvalArgs &a, const UnitCtx &U, const uin
       sl[q] = slot(j < n ? j : first);
       mm[q] = j < n ? keys[sl[q]] : U.o;
     }
+    // a singleton may join another singleton only if that module's id is
+    // smaller: the rule is decided from the cached one-unit bits before any
+    // module row is read, so excluded candidates cost no memory request
+    bool skip[UC];
+#pragma unroll
+    for (int q = 0; q < UC; ++q)
+      skip[q] = mm[q] == U.o || (U.single && mm[q] > U.o && one_unit(a, mm[q]));
     Stat4 M[UC];
-    bool mu[UC];
+#pragma unroll
+    for (int q = 0; q < UC; ++q) M[q] = skip[q] ? Stat4{0, 0, 0, 0} : a.ms[mm[q]];
 #pragma unroll
     for (int q = 0; q < UC; ++q) {
-      const bool c = mm[q] != U.o;
-      M[q] = c ? a.ms[mm[q]] : Stat4{0, 0, 0, 0};
-      mu[q] = c && U.single && mm[q] > U.o && one_unit(a, mm[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < UC; ++q) {
-      // a singleton may join another singleton only if that module's id is smaller
-      if (mm[q] == U.o || mu[q]) continue;
+      if (skip[q]) continue;
       const double dl = cand_delta<F32>(U, M[q], fo[sl[q]], fi[sl[q]]);
       if (better(dl, mm[q], b) && dl < 0.0) {
         b.d = dl;
