@@ -77,9 +77,18 @@ __global__ void k_qv_all(const Stat4 *__restrict__ ms, const int32_t *__restrict
 __global__ __launch_bounds__(kRedBlock) void k_sum_qv(const double *__restrict__ qv, int32_t n,
                                                       double *__restrict__ part) {
   __shared__ double lds[kRedBlock / kWave];
+  constexpr int64_t kStride = (int64_t)kRedGrid * kRedBlock;
   double s = 0;
-  for (int64_t m = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; m < n; m += (int64_t)kRedGrid * kRedBlock)
-    s += qv[m];
+  // four loads in flight per lane; the additions keep the one-at-a-time order
+  int64_t m = (int64_t)blockIdx.x * kRedBlock + threadIdx.x;
+  for (; m + 3 * kStride < n; m += 4 * kStride) {
+    const double v0 = qv[m], v1 = qv[m + kStride], v2 = qv[m + 2 * kStride], v3 = qv[m + 3 * kStride];
+    s += v0;
+    s += v1;
+    s += v2;
+    s += v3;
+  }
+  for (; m < n; m += kStride) s += qv[m];
   s = block_sum<double, kRedBlock>(s, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }

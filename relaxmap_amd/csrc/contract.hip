@@ -88,15 +88,88 @@ __global__ void k_count_inter(const uint64_t *__restrict__ keys, int64_t m, uint
   *out = (uint32_t)lo;
 }
 
-__global__ void k_key_heads(const uint64_t *__restrict__ keys, int64_t m, uint32_t *__restrict__ flag) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+// Segments of the sorted inter keys (one per super arc), in two reads of the
+// keys: k_seg_count counts each tile's heads (positions whose key differs
+// from the one before), and after a scan of the tile counts k_seg_starts
+// writes every segment's first position and, where the source super node
+// changes between two segments, the out-pointers of the sources in between.
+// A tile is 16 rounds of 256 consecutive keys; a head's rank inside its tile
+// follows (round, wavefront, lane) order, which is position order.
+constexpr int kSegItems = 16;
+constexpr int kSegTile = 256 * kSegItems;
+
+__device__ inline bool seg_head(const uint64_t *__restrict__ keys, int64_t j, uint64_t &k, uint64_t &kp) {
+  k = keys[j];
+  kp = j > 0 ? keys[j - 1] : ~k;
+  return k != kp;
 }
 
-__global__ void k_key_starts(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ pos, int64_t m,
-                             uint32_t *__restrict__ start) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m && flag[i]) start[pos[i]] = (uint32_t)i;
+__global__ __launch_bounds__(256) void k_seg_count(const uint64_t *__restrict__ keys, int64_t m,
+                                                   uint32_t *__restrict__ tile_cnt) {
+  __shared__ uint32_t lds[256 / kWave];
+  const int64_t base = (int64_t)blockIdx.x * kSegTile;
+  uint32_t c = 0;
+#pragma unroll 4
+  for (int i = 0; i < kSegItems; ++i) {
+    const int64_t j = base + (int64_t)i * 256 + threadIdx.x;
+    uint64_t k, kp;
+    if (j < m && seg_head(keys, j, k, kp)) ++c;
+  }
+  c = block_sum<uint32_t, 256>(c, lds);
+  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = c;
+}
+
+// tile_base: exclusive scan of the tile counts; *n_seg: their total
+__global__ __launch_bounds__(256) void k_seg_starts(const uint64_t *__restrict__ keys, int64_t m, int bits,
+                                                    int32_t ns, const uint32_t *__restrict__ tile_base,
+                                                    const uint32_t *__restrict__ n_seg, uint32_t *__restrict__ start,
+                                                    uint32_t *__restrict__ out_ptr) {
+  constexpr int NW = 256 / kWave;
+  static_assert(kSegItems * NW == kWave, "one lane per (round, wavefront) count");
+  __shared__ uint32_t cnt[kSegItems * NW];   // heads of (round, wavefront), then their exclusive prefix
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  const int64_t base = (int64_t)blockIdx.x * kSegTile;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+  uint64_t key[kSegItems], prev[kSegItems];
+  uint32_t rank[kSegItems];
+  uint32_t heads = 0;   // bit i: this lane's element of round i is a head
+#pragma unroll
+  for (int i = 0; i < kSegItems; ++i) {
+    const int64_t j = base + (int64_t)i * 256 + t;
+    const bool h = j < m && seg_head(keys, j, key[i], prev[i]);
+    const uint64_t bal = __ballot(h);
+    if (lane == 0) cnt[i * NW + w] = (uint32_t)__popcll(bal);
+    rank[i] = (uint32_t)__popcll(bal & below);
+    heads |= (h ? 1u : 0u) << i;
+  }
+  __syncthreads();
+  if (w == 0) {   // exclusive scan of the 64 (round, wavefront) counts, one per lane
+    const uint32_t v = cnt[lane];
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off, kWave);
+      if (lane >= off) x += y;
+    }
+    cnt[lane] = x - v;
+  }
+  __syncthreads();
+  const uint32_t tb = tile_base[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kSegItems; ++i) {
+    if (!((heads >> i) & 1u)) continue;
+    const int64_t j = base + (int64_t)i * 256 + t;
+    const uint32_t s = tb + cnt[i * NW + w] + rank[i];
+    start[s] = (uint32_t)j;
+    const int64_t src = (int64_t)(key[i] >> bits);
+    const int64_t src_prev = j > 0 ? (int64_t)(prev[i] >> bits) : -1;
+    for (int64_t d = src_prev + 1; d <= src; ++d) out_ptr[d] = s;   // empty when the source is unchanged
+  }
+  if (m > 0 && base <= m - 1 && m - 1 < base + kSegTile && (m - 1 - base) % 256 == t) {
+    // the last key: super nodes after its source have no out-arcs
+    const uint32_t total = *n_seg;
+    for (int64_t d = (int64_t)(keys[m - 1] >> bits) + 1; d <= ns; ++d) out_ptr[d] = total;
+  }
 }
 
 // one thread per (src, dst) super arc: sequential FP64 sum in arc order (in
@@ -151,20 +224,6 @@ __global__ void k_in_ptr(const uint64_t *__restrict__ key, int bits, int64_t m, 
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= m; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t lo = i == 0 ? 0 : (int64_t)(key[i - 1] >> bits) + 1;
     const int64_t hi = i == m ? (int64_t)n : (int64_t)(key[i] >> bits);
-    for (int64_t d = lo; d <= hi; ++d) ptr[d] = (uint32_t)i;
-  }
-}
-
-// out_ptr of the super level from the segments' sources (ascending): segment
-// i fills every super node in (src[i-1], src[i]], and position n_seg the tail
-__global__ void k_super_out_ptr(const uint32_t *__restrict__ start, int64_t n_seg,
-                                const uint64_t *__restrict__ keys, int bits, int32_t ns,
-                                uint32_t *__restrict__ ptr) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i <= n_seg; i += stride) {
-    const int64_t lo = i == 0 ? 0 : (int64_t)(keys[start[i - 1]] >> bits) + 1;
-    const int64_t hi = i == n_seg ? (int64_t)ns : (int64_t)(keys[start[i]] >> bits);
     for (int64_t d = lo; d <= hi; ++d) ptr[d] = (uint32_t)i;
   }
 }
@@ -301,23 +360,31 @@ void contract(Ctx &c) {
   }
   c.sync();
 
-  // one super arc per distinct key
+  // one super arc per distinct key: its first position, and the out-pointers
   uint32_t m_super = 0;
-  DevBuf<uint32_t> kflag(c.cache), kpos(c.cache), kstart(c.cache);
+  DevBuf<uint32_t> tcnt(c.cache), kstart(c.cache);
+  D.out_ptr.alloc((int64_t)ns + 1);
   if (m_inter > 0) {
-    kflag.alloc(m_inter);
-    kpos.alloc(m_inter);
-    k_key_heads<<<grid_for(m_inter, 256), 256, 0, s>>>(keys.p, m_inter, kflag.p);
+    const int64_t nt = ((int64_t)m_inter + kSegTile - 1) / kSegTile;
+    tcnt.alloc(nt);
+    kstart.alloc(m_inter);   // capacity: at most one segment per key
+    RMX_PROF_BEGIN(c);
+    k_seg_count<<<(unsigned)nt, 256, 0, s>>>(keys.p, m_inter, tcnt.p);
     RMX_LAUNCH_CHECK();
-    exclusive_scan_u32(kflag.p, kpos.p, m_inter, atot.p, c.scratch, s);
+    RMX_PROF_END(c, "k_seg_count", (double)m_inter * 8 + (double)nt * 4);
+    exclusive_scan_u32(tcnt.p, tcnt.p, nt, atot.p, c.scratch, s);
+    RMX_PROF_BEGIN(c);
+    k_seg_starts<<<(unsigned)nt, 256, 0, s>>>(keys.p, m_inter, bits, (int32_t)ns, tcnt.p, atot.p, kstart.p,
+                                               D.out_ptr.p);
+    RMX_LAUNCH_CHECK();
+    const float ms_starts = c.prof.on ? c.prof.stop(s) : 0.f;
     RMX_HIP(hipMemcpyAsync(&m_super, atot.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     c.sync();
-    kstart.alloc(m_super);
-    k_key_starts<<<grid_for(m_inter, 256), 256, 0, s>>>(kflag.p, kpos.p, m_inter, kstart.p);
-    RMX_LAUNCH_CHECK();
-    c.sync();
-    kflag.free();
-    kpos.free();
+    // per key: the key read; per segment: its start written; per super node: its out-pointer
+    if (c.prof.on) c.prof.add("k_seg_starts", ms_starts, (double)m_inter * 8 + (double)m_super * 4 + (double)ns * 4);
+    tcnt.free();
+  } else {
+    fill_u32(D.out_ptr.p, 0u, (int64_t)ns + 1, s);
   }
   D.m = m_super;
   D.out_idx.alloc(m_super);
@@ -338,14 +405,6 @@ void contract(Ctx &c) {
     // from the node level); per super arc: two starts, its key, five outputs
     const double per_arc = 4.0 + ((c.precision == 1 && !from_super) ? 0.0 : (from_super ? 16.0 : 8.0));
     RMX_PROF_END(c, "k_reduce_segments", (double)m_inter * per_arc + (double)m_super * (8 + 8 + 4 + 8 + 8 + 8 + 8));
-  }
-  D.out_ptr.alloc((int64_t)ns + 1);
-  if (m_super > 0) {
-    k_super_out_ptr<<<(unsigned)std::min<int64_t>(16384, (m_super + 256) / 256), 256, 0, s>>>(
-        kstart.p, m_super, keys.p, bits, (int32_t)ns, D.out_ptr.p);
-    RMX_LAUNCH_CHECK();
-  } else {
-    fill_u32(D.out_ptr.p, 0u, (int64_t)ns + 1, s);
   }
   c.sync();
   keys.free();

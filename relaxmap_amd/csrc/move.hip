@@ -1725,18 +1725,23 @@ __global__ void k_apply(const uint32_t *__restrict__ movers, const uint32_t *__r
 
 // one-unit bits of modules [0, n), and with stale counts (node level) their
 // more-than-one-node bits: 32 modules per thread
-__global__ void k_one_bits(const int32_t *__restrict__ mUnits, const int32_t *__restrict__ mStale, int32_t n,
-                           uint32_t *__restrict__ one, uint32_t *__restrict__ multi) {
-  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (w * 32 >= n) return;
-  uint32_t bits = 0, mb = 0;
-  const int64_t m0 = w * 32;
-  for (int b = 0; b < 32 && m0 + b < n; ++b) {
-    bits |= (mUnits[m0 + b] == 1 ? 1u : 0u) << b;
-    if (mStale) mb |= (mStale[m0 + b] > 1 ? 1u : 0u) << b;
+// One module per lane (coalesced reads); a wavefront's 64 bits are its
+// ballot, written as two words by lanes 0 and 1.  Launched with 256-lane
+// blocks, so every wavefront starts at a multiple of 64 modules.
+__global__ __launch_bounds__(256) void k_one_bits(const int32_t *__restrict__ mUnits,
+                                                  const int32_t *__restrict__ mStale, int32_t n,
+                                                  uint32_t *__restrict__ one, uint32_t *__restrict__ multi) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t w0 = (m - lane) / 32;   // first word of this wavefront's modules
+  if (w0 * 32 >= n) return;             // the whole wavefront lies past the end
+  const uint64_t b1 = __ballot(m < n && mUnits[m] == 1);
+  const uint64_t b2 = mStale ? __ballot(m < n && mStale[m] > 1) : 0ull;
+  const int64_t words = ((int64_t)n + 31) / 32;
+  if (lane < 2 && w0 + lane < words) {
+    one[w0 + lane] = (uint32_t)(b1 >> (32 * lane));
+    if (mStale) multi[w0 + lane] = (uint32_t)(b2 >> (32 * lane));
   }
-  one[w] = bits;
-  if (mStale) multi[w] = mb;
 }
 
 // sort key of every bucket position: (bucket, batch) or (bucket, inactive)
@@ -1900,8 +1905,8 @@ EvalArgs sweep_args(Ctx &c, Level &L, bool prioritized, double vthresh, uint64_t
     const bool node = c.level == 0;
     c.mOne.alloc(words);
     if (node) c.mMulti.alloc(words);
-    k_one_bits<<<grid_for(words, 256), 256, 0, c.stream>>>(c.mUnits.p, node ? c.mStale.p : nullptr, c.N, c.mOne.p,
-                                                           node ? c.mMulti.p : nullptr);
+    k_one_bits<<<grid_for(c.N, 256), 256, 0, c.stream>>>(c.mUnits.p, node ? c.mStale.p : nullptr, c.N, c.mOne.p,
+                                                         node ? c.mMulti.p : nullptr);
     RMX_LAUNCH_CHECK();
     a.mOne = c.mOne.p;
     a.mMulti = node ? c.mMulti.p : nullptr;

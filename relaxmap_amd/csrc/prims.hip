@@ -178,19 +178,30 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K *__restrict__ 
   for (int q = 0; q < kMaxPasses; ++q) h[q][threadIdx.x] = 0;
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
-  for (int64_t j0 = (int64_t)blockIdx.x * kSortBlock; j0 < n; j0 += (int64_t)gridDim.x * kSortBlock) {
-    const int64_t j = j0 + threadIdx.x;
-    const bool valid = j < n;   // the valid lanes of a wavefront are a prefix
-    const K k = valid ? keys[j] : K(0);
-    const uint64_t live = __ballot(valid);
-    if (!live) continue;
-    for (int q = 0; q < passes; ++q) {
-      const uint32_t d = (uint32_t)(k >> (begin_bit + 8 * q)) & 0xffu;
-      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-      if (__ballot(valid && d == d0) == live) {
-        if (lane == 0) atomicAdd(&h[q][d0], (uint32_t)__popcll(live));
-      } else if (valid) {
-        atomicAdd(&h[q][d], 1u);
+  constexpr int kHistLoads = 8;   // keys per lane in flight
+  for (int64_t j0 = (int64_t)blockIdx.x * kSortBlock * kHistLoads; j0 < n;
+       j0 += (int64_t)gridDim.x * kSortBlock * kHistLoads) {
+    K kk[kHistLoads];
+#pragma unroll
+    for (int u = 0; u < kHistLoads; ++u) {
+      const int64_t j = j0 + u * kSortBlock + threadIdx.x;
+      kk[u] = j < n ? keys[j] : K(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kHistLoads; ++u) {
+      const int64_t j = j0 + u * kSortBlock + threadIdx.x;
+      const bool valid = j < n;   // the valid lanes of a wavefront are a prefix
+      const uint64_t live = __ballot(valid);
+      if (!live) continue;
+      const K k = kk[u];
+      for (int q = 0; q < passes; ++q) {
+        const uint32_t d = (uint32_t)(k >> (begin_bit + 8 * q)) & 0xffu;
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        if (__ballot(valid && d == d0) == live) {
+          if (lane == 0) atomicAdd(&h[q][d0], (uint32_t)__popcll(live));
+        } else if (valid) {
+          atomicAdd(&h[q][d], 1u);
+        }
       }
     }
   }
