@@ -134,6 +134,48 @@ __global__ void k_starts(const uint32_t *__restrict__ flag, const uint32_t *__re
   if (i < n && flag[i]) start[pos[i]] = (uint32_t)i;
 }
 
+// Instances of more than kBigInst nodes are reduced by several workgroups,
+// kBigInst nodes each, and their partial sums combined in chunk order; the
+// others by one wavefront each.  The order of every sum is fixed by the
+// instance's node list, so the results are reproducible.  The plan (which
+// instances are big and where their chunks' partial sums go) is made once
+// per coarse tune, since the instances do not change during its sweeps.
+constexpr uint32_t kBigInst = 16384;
+constexpr int kBigParts = 3;   // partial sums kept per chunk
+
+__device__ inline void inst_range(const uint32_t *__restrict__ start, int32_t n_seg, int32_t n, int64_t s,
+                                  uint32_t &b, uint32_t &e) {
+  b = start[s];
+  e = s + 1 < n_seg ? start[s + 1] : (uint32_t)n;
+}
+
+// one thread per instance: big ones get a list entry, consecutive chunk slots
+// and the owner of every slot (ctr[0]: big instances, ctr[1]: chunk slots)
+__global__ void k_big_plan(const uint32_t *__restrict__ start, int32_t n_seg, int32_t n, uint32_t *__restrict__ ctr,
+                           uint32_t *__restrict__ list, uint32_t *__restrict__ base, uint32_t *__restrict__ owner) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_seg) return;
+  uint32_t b, e;
+  inst_range(start, n_seg, n, s, b, e);
+  if (e - b <= kBigInst) return;
+  const uint32_t nch = (e - b + kBigInst - 1) / kBigInst;
+  const uint32_t li = atomicAdd(&ctr[0], 1u);
+  const uint32_t sl = atomicAdd(&ctr[1], nch);
+  list[li] = (uint32_t)s;
+  base[li] = sl;
+  for (uint32_t k = 0; k < nch; ++k) owner[sl + k] = li;
+}
+
+// the node range of chunk slot `slot`
+__device__ inline void big_chunk(const BigInst &B, const uint32_t *__restrict__ start, int32_t n_seg, int32_t n,
+                                 uint32_t slot, uint32_t &b, uint32_t &e) {
+  const uint32_t li = B.owner[slot];
+  uint32_t sb, se;
+  inst_range(start, n_seg, n, B.list[li], sb, se);
+  b = sb + (slot - B.base[li]) * kBigInst;
+  e = b + kBigInst < se ? b + kBigInst : se;
+}
+
 // wavefront per instance: sum of phi(p) over its nodes (constant)
 __global__ void k_inst_const(const uint32_t *__restrict__ start, int32_t n_seg, int32_t n,
                              const uint32_t *__restrict__ nodes, const uint32_t *__restrict__ par,
@@ -141,15 +183,55 @@ __global__ void k_inst_const(const uint32_t *__restrict__ start, int32_t n_seg, 
   const int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (s >= n_seg) return;
-  const uint32_t b = start[s], e = s + 1 < n_seg ? start[s + 1] : (uint32_t)n;
+  uint32_t b, e;
+  inst_range(start, n_seg, n, s, b, e);
+  if (e - b > kBigInst) return;   // k_inst_const_chunks / k_inst_const_big
   double a = 0.0;
   for (uint32_t i = b + lane; i < e; i += kWave) a += plogp(fx_to(us[nodes[i]].p));
   a = wave_sum(a);
   if (lane == 0) C[par[nodes[b]]] = a;
 }
 
+__global__ __launch_bounds__(256) void k_inst_const_chunks(BigInst B, uint32_t n_slots,
+                                                           const uint32_t *__restrict__ start, int32_t n_seg,
+                                                           int32_t n, const uint32_t *__restrict__ nodes,
+                                                           const Stat4 *__restrict__ us) {
+  __shared__ double lds[256 / kWave];
+  for (uint32_t slot = blockIdx.x; slot < n_slots; slot += gridDim.x) {
+    uint32_t b, e;
+    big_chunk(B, start, n_seg, n, slot, b, e);
+    double a = 0.0;
+    for (uint32_t i = b + threadIdx.x; i < e; i += 256) a += plogp(fx_to(us[nodes[i]].p));
+    a = block_sum<double, 256>(a, lds);
+    if (threadIdx.x == 0) B.part[(size_t)slot * kBigParts] = a;
+  }
+}
+
+__global__ void k_inst_const_big(BigInst B, uint32_t n_big, const uint32_t *__restrict__ start, int32_t n_seg,
+                                 int32_t n, const uint32_t *__restrict__ nodes, const uint32_t *__restrict__ par,
+                                 double *__restrict__ C) {
+  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= n_big) return;
+  uint32_t b, e;
+  inst_range(start, n_seg, n, B.list[li], b, e);
+  const uint32_t nch = (e - b + kBigInst - 1) / kBigInst;
+  double a = 0.0;
+  for (uint32_t k = 0; k < nch; ++k) a += B.part[(size_t)(B.base[li] + k) * kBigParts];
+  C[par[nodes[b]]] = a;
+}
+
 // wavefront per instance: S = sum of its sub-module exits and its codelength
 // L = phi(S) - 2 sum phi(q) + sum phi(q + P) - sum phi(p)  (Module.cpp:474-501)
+__device__ inline void inst_terms(const Stat4 *__restrict__ ms, const int32_t *__restrict__ mnodes, uint32_t m,
+                                  double &sq, double &pq, double &pqp) {
+  if (mnodes[m] <= 0) return;
+  const Stat4 st = ms[m];
+  const double q = module_exit(st.p, st.t, st.d, st.x);
+  sq += q;
+  pq += plogp(q);
+  pqp += plogp(q + fx_to(st.p));
+}
+
 __global__ void k_inst_totals(const uint32_t *__restrict__ start, int32_t n_seg, int32_t n,
                               const uint32_t *__restrict__ nodes, const uint32_t *__restrict__ par,
                               const Stat4 *__restrict__ ms, const int32_t *__restrict__ mnodes,
@@ -157,17 +239,11 @@ __global__ void k_inst_totals(const uint32_t *__restrict__ start, int32_t n_seg,
   const int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (s >= n_seg) return;
-  const uint32_t b = start[s], e = s + 1 < n_seg ? start[s + 1] : (uint32_t)n;
+  uint32_t b, e;
+  inst_range(start, n_seg, n, s, b, e);
+  if (e - b > kBigInst) return;   // k_inst_totals_chunks / k_inst_totals_big
   double sq = 0.0, pq = 0.0, pqp = 0.0;
-  for (uint32_t i = b + lane; i < e; i += kWave) {
-    const uint32_t m = nodes[i];   // sub-module ids are node ids of the instance
-    if (mnodes[m] <= 0) continue;
-    const Stat4 st = ms[m];
-    const double q = module_exit(st.p, st.t, st.d, st.x);
-    sq += q;
-    pq += plogp(q);
-    pqp += plogp(q + fx_to(st.p));
-  }
+  for (uint32_t i = b + lane; i < e; i += kWave) inst_terms(ms, mnodes, nodes[i], sq, pq, pqp);   // sub-module ids are node ids of the instance
   sq = wave_sum(sq);
   pq = wave_sum(pq);
   pqp = wave_sum(pqp);
@@ -176,6 +252,50 @@ __global__ void k_inst_totals(const uint32_t *__restrict__ start, int32_t n_seg,
     S_out[I] = sq;
     L_out[I] = plogp(sq) - 2.0 * pq + pqp - C[I];
   }
+}
+
+__global__ __launch_bounds__(256) void k_inst_totals_chunks(BigInst B, uint32_t n_slots,
+                                                            const uint32_t *__restrict__ start, int32_t n_seg,
+                                                            int32_t n, const uint32_t *__restrict__ nodes,
+                                                            const Stat4 *__restrict__ ms,
+                                                            const int32_t *__restrict__ mnodes) {
+  __shared__ double lds[256 / kWave];
+  for (uint32_t slot = blockIdx.x; slot < n_slots; slot += gridDim.x) {
+    uint32_t b, e;
+    big_chunk(B, start, n_seg, n, slot, b, e);
+    double sq = 0.0, pq = 0.0, pqp = 0.0;
+    for (uint32_t i = b + threadIdx.x; i < e; i += 256) inst_terms(ms, mnodes, nodes[i], sq, pq, pqp);
+    sq = block_sum<double, 256>(sq, lds);
+    pq = block_sum<double, 256>(pq, lds);
+    pqp = block_sum<double, 256>(pqp, lds);
+    if (threadIdx.x == 0) {
+      double *o = B.part + (size_t)slot * kBigParts;
+      o[0] = sq;
+      o[1] = pq;
+      o[2] = pqp;
+    }
+  }
+}
+
+__global__ void k_inst_totals_big(BigInst B, uint32_t n_big, const uint32_t *__restrict__ start, int32_t n_seg,
+                                  int32_t n, const uint32_t *__restrict__ nodes, const uint32_t *__restrict__ par,
+                                  const double *__restrict__ C, double *__restrict__ S_out,
+                                  double *__restrict__ L_out) {
+  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= n_big) return;
+  uint32_t b, e;
+  inst_range(start, n_seg, n, B.list[li], b, e);
+  const uint32_t nch = (e - b + kBigInst - 1) / kBigInst;
+  double sq = 0.0, pq = 0.0, pqp = 0.0;
+  for (uint32_t k = 0; k < nch; ++k) {
+    const double *o = B.part + (size_t)(B.base[li] + k) * kBigParts;
+    sq += o[0];
+    pq += o[1];
+    pqp += o[2];
+  }
+  const uint32_t I = par[nodes[b]];
+  S_out[I] = sq;
+  L_out[I] = plogp(sq) - 2.0 * pq + pqp - C[I];
 }
 
 // instances with more than one member start active (in a batch of
@@ -257,12 +377,52 @@ void swap_in_sub(Ctx &c) {
 
 }  // namespace
 
+namespace {
+
+BigInst big_view(Ctx &c) { return BigInst{c.ibig_list.p, c.ibig_base.p, c.ibig_owner.p, c.ibig_part.p}; }
+
+unsigned big_grid(uint32_t slots) { return (unsigned)std::min<uint32_t>(slots, 2048u); }
+
+// the chunk plan of the current instances (c.inst_start, c.n_inst)
+void plan_big_instances(Ctx &c) {
+  hipStream_t s = c.stream;
+  const int64_t cap = (int64_t)c.N / kBigInst + 1;   // big instances; their chunks fit in 2 cap slots
+  c.ibig_ctr.alloc(2);
+  c.ibig_list.alloc(cap);
+  c.ibig_base.alloc(cap);
+  c.ibig_owner.alloc(2 * cap);
+  c.ibig_part.alloc(2 * cap * kBigParts);
+  fill_u32(c.ibig_ctr.p, 0u, 2, s);
+  c.n_ibig = c.n_ibig_slots = 0;
+  if (c.n_inst <= 0) return;
+  k_big_plan<<<grid_for(c.n_inst, 256), 256, 0, s>>>(c.inst_start.p, c.n_inst, c.N, c.ibig_ctr.p, c.ibig_list.p,
+                                                     c.ibig_base.p, c.ibig_owner.p);
+  RMX_LAUNCH_CHECK();
+  uint32_t h[2] = {0, 0};
+  RMX_HIP(hipMemcpyAsync(h, c.ibig_ctr.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  c.sync();
+  c.n_ibig = h[0];
+  c.n_ibig_slots = h[1];
+}
+
+}  // namespace
+
 void instance_totals_async(Ctx &c) {
   if (c.n_inst <= 0) return;
   k_inst_totals<<<grid_for((int64_t)c.n_inst * kWave, 256), 256, 0, c.stream>>>(
       c.inst_start.p, c.n_inst, c.N, c.inst_nodes.p, c.sub_par.p, c.ms.p, c.mNodes.p, c.inst_C.p, c.inst_S.p,
       c.inst_L.p);
   RMX_LAUNCH_CHECK();
+  if (c.n_ibig > 0) {
+    k_inst_totals_chunks<<<big_grid(c.n_ibig_slots), 256, 0, c.stream>>>(big_view(c), c.n_ibig_slots, c.inst_start.p,
+                                                                         c.n_inst, c.N, c.inst_nodes.p, c.ms.p,
+                                                                         c.mNodes.p);
+    RMX_LAUNCH_CHECK();
+    k_inst_totals_big<<<grid_for(c.n_ibig, 256), 256, 0, c.stream>>>(big_view(c), c.n_ibig, c.inst_start.p, c.n_inst,
+                                                                     c.N, c.inst_nodes.p, c.sub_par.p, c.inst_C.p,
+                                                                     c.inst_S.p, c.inst_L.p);
+    RMX_LAUNCH_CHECK();
+  }
 }
 
 int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t seed) {
@@ -382,8 +542,17 @@ int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t se
   c.inst_C.alloc(n);
   c.inst_act.alloc(n);
   const unsigned gw = grid_for((int64_t)c.n_inst * kWave, 256), gs = grid_for(c.n_inst, 256);
+  plan_big_instances(c);
   k_inst_const<<<gw, 256, 0, s>>>(c.inst_start.p, c.n_inst, n, c.inst_nodes.p, c.sub_par.p, S.us.p, c.inst_C.p);
   RMX_LAUNCH_CHECK();
+  if (c.n_ibig > 0) {
+    k_inst_const_chunks<<<big_grid(c.n_ibig_slots), 256, 0, s>>>(big_view(c), c.n_ibig_slots, c.inst_start.p, c.n_inst,
+                                                                 n, c.inst_nodes.p, S.us.p);
+    RMX_LAUNCH_CHECK();
+    k_inst_const_big<<<grid_for(c.n_ibig, 256), 256, 0, s>>>(big_view(c), c.n_ibig, c.inst_start.p, c.n_inst, n,
+                                                             c.inst_nodes.p, c.sub_par.p, c.inst_C.p);
+    RMX_LAUNCH_CHECK();
+  }
   fill_u8(c.inst_act.p, 0, n, s);
   k_inst_init<<<gs, 256, 0, s>>>(c.inst_start.p, c.n_inst, n, c.inst_nodes.p, c.sub_par.p,
                                  c.nb > 0 ? c.b_step.p : nullptr, c.b_node.p, c.inst_act.p);

@@ -9,9 +9,10 @@
 //
 // From-scratch form (Network::calculateCodeLength, Module.cpp:2966-3002):
 // every node's exit-arc flow, then nodes stably sorted by module and one
-// wavefront per module summing its members in FP64 (a fixed tree order, so
-// the result is reproducible; it differs from the reference's sequential
-// order by rounding only).
+// wavefront per module summing its members in FP64 (a module of more than
+// 16384 members: one workgroup per chunk of them, the chunks added in order).
+// The tree order is fixed, so the result is reproducible; it differs from the
+// reference's sequential order by rounding only.
 #include "devutil.h"
 
 namespace rmx {
@@ -302,7 +303,25 @@ __global__ void k_p5_ext(int32_t n, const uint32_t *__restrict__ optr, const uin
 
 // one wavefront per module (its members are a segment of the module-sorted
 // node list): FP64 sums of flow, teleport, dangling flow and exit-arc flow,
-// then the module exit of Module.cpp:2978-2991 in the reference's form
+// then the module exit of Module.cpp:2978-2991 in the reference's form.
+// Modules of more than kP5Big members are summed in chunks of kP5Big members
+// by k_p5_chunks (one workgroup per chunk) and combined in chunk order by
+// k_p5_big, so that one large module does not hold a single wavefront for
+// the whole launch; every sum keeps an order fixed by the member list.
+constexpr uint32_t kP5Big = 16384;
+
+__device__ inline void p5_seg(const uint32_t *__restrict__ start, uint32_t nseg, int32_t n, uint32_t s,
+                              uint32_t &b, uint32_t &e) {
+  b = start[s];
+  e = (s + 1 < nseg) ? start[s + 1] : (uint32_t)n;
+}
+
+__device__ inline void p5_exit(double P, double T, double D, double X, double &q, double &qp) {
+  q = kAlpha * (1.0 - T) * P;
+  q += kBeta * (X + (1.0 - T) * D);
+  qp = q + P;
+}
+
 __global__ void k_p5_modules(const uint32_t *__restrict__ start, const uint32_t *__restrict__ nseg_p, int32_t n,
                              const uint32_t *__restrict__ node, const double *__restrict__ p,
                              const double *__restrict__ tau, const uint8_t *__restrict__ dang,
@@ -311,8 +330,9 @@ __global__ void k_p5_modules(const uint32_t *__restrict__ start, const uint32_t 
   const int64_t sidx = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (sidx >= nseg) return;
-  const uint32_t b = start[sidx];
-  const uint32_t e = (sidx + 1 < nseg) ? start[sidx + 1] : (uint32_t)n;
+  uint32_t b, e;
+  p5_seg(start, nseg, n, (uint32_t)sidx, b, e);
+  if (e - b > kP5Big) return;   // k_p5_chunks / k_p5_big
   double P = 0.0, T = 0.0, D = 0.0, X = 0.0;
   for (uint32_t i = b + lane; i < e; i += kWave) {
     const uint32_t a = node[i];
@@ -325,12 +345,83 @@ __global__ void k_p5_modules(const uint32_t *__restrict__ start, const uint32_t 
   T = wave_sum(T);
   D = wave_sum(D);
   X = wave_sum(X);
-  if (lane == 0) {
-    double q = kAlpha * (1.0 - T) * P;
-    q += kBeta * (X + (1.0 - T) * D);
-    segq[sidx] = q;
-    segqp[sidx] = q + P;
+  if (lane == 0) p5_exit(P, T, D, X, segq[sidx], segqp[sidx]);
+}
+
+// Chunk plan of the large modules: ctr[0] large modules, ctr[1] chunk slots;
+// list / base per large module, owner per slot.
+__global__ void k_p5_plan(const uint32_t *__restrict__ start, const uint32_t *__restrict__ nseg_p, int32_t n,
+                          uint32_t *__restrict__ ctr, uint32_t *__restrict__ list, uint32_t *__restrict__ base,
+                          uint32_t *__restrict__ owner) {
+  const uint32_t nseg = *nseg_p;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  uint32_t b, e;
+  p5_seg(start, nseg, n, (uint32_t)s, b, e);
+  if (e - b <= kP5Big) return;
+  const uint32_t nch = (e - b + kP5Big - 1) / kP5Big;
+  const uint32_t li = atomicAdd(&ctr[0], 1u);
+  const uint32_t sl = atomicAdd(&ctr[1], nch);
+  list[li] = (uint32_t)s;
+  base[li] = sl;
+  for (uint32_t k = 0; k < nch; ++k) owner[sl + k] = li;
+}
+
+__global__ __launch_bounds__(256) void k_p5_chunks(const uint32_t *__restrict__ ctr, const uint32_t *__restrict__ list,
+                                                   const uint32_t *__restrict__ base,
+                                                   const uint32_t *__restrict__ owner, const uint32_t *__restrict__ start,
+                                                   const uint32_t *__restrict__ nseg_p, int32_t n,
+                                                   const uint32_t *__restrict__ node, const double *__restrict__ p,
+                                                   const double *__restrict__ tau, const uint8_t *__restrict__ dang,
+                                                   const double *__restrict__ ext, double *__restrict__ part) {
+  __shared__ double lds[256 / kWave];
+  const uint32_t nseg = *nseg_p, slots = ctr[1];
+  for (uint32_t slot = blockIdx.x; slot < slots; slot += gridDim.x) {
+    const uint32_t li = owner[slot];
+    uint32_t sb, se;
+    p5_seg(start, nseg, n, list[li], sb, se);
+    const uint32_t b = sb + (slot - base[li]) * kP5Big, e = b + kP5Big < se ? b + kP5Big : se;
+    double P = 0.0, T = 0.0, D = 0.0, X = 0.0;
+    for (uint32_t i = b + threadIdx.x; i < e; i += 256) {
+      const uint32_t a = node[i];
+      P += p[a];
+      T += tau[a];
+      if (dang[a]) D += p[a];
+      X += ext[a];
+    }
+    P = block_sum<double, 256>(P, lds);
+    T = block_sum<double, 256>(T, lds);
+    D = block_sum<double, 256>(D, lds);
+    X = block_sum<double, 256>(X, lds);
+    if (threadIdx.x == 0) {
+      double *o = part + 4 * (size_t)slot;
+      o[0] = P;
+      o[1] = T;
+      o[2] = D;
+      o[3] = X;
+    }
   }
+}
+
+__global__ void k_p5_big(const uint32_t *__restrict__ ctr, const uint32_t *__restrict__ list,
+                         const uint32_t *__restrict__ base, const uint32_t *__restrict__ start,
+                         const uint32_t *__restrict__ nseg_p, int32_t n, const double *__restrict__ part,
+                         double *__restrict__ segq, double *__restrict__ segqp) {
+  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= ctr[0]) return;
+  const uint32_t s = list[li];
+  uint32_t b, e;
+  p5_seg(start, *nseg_p, n, s, b, e);
+  const uint32_t nch = (e - b + kP5Big - 1) / kP5Big;
+  double P = 0.0, T = 0.0, D = 0.0, X = 0.0;
+  for (uint32_t k = 0; k < nch; ++k) {
+    const double *o = part + 4 * (size_t)(base[li] + k);
+    P += o[0];
+    T += o[1];
+    D += o[2];
+    X += o[3];
+  }
+  p5_exit(P, T, D, X, segq[s], segqp[s]);
 }
 
 __global__ __launch_bounds__(kRedBlock) void k_p5_partials(const double *__restrict__ segq,
@@ -451,6 +542,23 @@ void p5_launch(int32_t n, const uint32_t *optr, const uint32_t *oidx, const doub
   k_p5_modules<<<grid_for((int64_t)n * kWave, 256), 256, 0, s>>>(w.start, w.total, n, w.node, p, tau, dang, w.ext,
                                                                  w.segq, w.segqp);
   RMX_LAUNCH_CHECK();
+  if ((uint32_t)n > kP5Big) {
+    // modules of more than kP5Big members: the chunk plan lives in the
+    // node-sized temporaries that the segmentation no longer needs (flag:
+    // counters, list, base, owner; pos: partial sums), at most n / kP5Big + 1
+    // large modules and twice that many chunks
+    const int64_t cap = (int64_t)n / kP5Big + 1;
+    uint32_t *ctr = w.flag, *list = ctr + 2, *base = list + cap, *owner = base + cap;
+    double *part = (double *)w.pos;
+    fill_u32(ctr, 0u, 2, s);
+    k_p5_plan<<<g, 256, 0, s>>>(w.start, w.total, n, ctr, list, base, owner);
+    RMX_LAUNCH_CHECK();
+    k_p5_chunks<<<(unsigned)std::min<int64_t>(2 * cap, 2048), 256, 0, s>>>(ctr, list, base, owner, w.start, w.total,
+                                                                            n, w.node, p, tau, dang, w.ext, part);
+    RMX_LAUNCH_CHECK();
+    k_p5_big<<<grid_for(cap, 256), 256, 0, s>>>(ctr, list, base, w.start, w.total, n, part, w.segq, w.segqp);
+    RMX_LAUNCH_CHECK();
+  }
   k_p5_partials<<<kRedGrid, kRedBlock, 0, s>>>(w.segq, w.segqp, w.total, w.partial);
   RMX_LAUNCH_CHECK();
   if (device_log_all) {

@@ -166,6 +166,14 @@ struct KernelStat {
   double ms = 0, bytes = 0;
 };
 
+// Device view of the chunk plan of large instances (coarse.hip)
+struct BigInst {
+  const uint32_t *list;    // big instance -> its segment
+  const uint32_t *base;    // big instance -> its first chunk slot
+  const uint32_t *owner;   // chunk slot -> big instance
+  double *part;            // chunk slot -> partial sums
+};
+
 struct Profiler {
   bool on = false;
   float last_ms = 0;
@@ -298,6 +306,10 @@ struct Ctx {
   int32_t n_inst = 0;                // instances (non-empty parent modules)
   DevBuf<double> inst_S, inst_L, inst_Lprev, inst_C;   // per parent module id
   DevBuf<uint8_t> inst_act;
+  // instances too large for one wavefront: reduced in chunks (coarse.hip)
+  DevBuf<uint32_t> ibig_ctr, ibig_list, ibig_base, ibig_owner;
+  DevBuf<double> ibig_part;
+  uint32_t n_ibig = 0, n_ibig_slots = 0;
   int32_t n_sub = 0;                 // sub-modules after the last partition
 
   // steady-state buffers
