@@ -140,7 +140,7 @@ __global__ void k_starts(const uint32_t *__restrict__ flag, const uint32_t *__re
 // instance's node list, so the results are reproducible.  The plan (which
 // instances are big and where their chunks' partial sums go) is made once
 // per coarse tune, since the instances do not change during its sweeps.
-constexpr uint32_t kBigInst = 16384;
+constexpr uint32_t kBigInst = 2048;
 constexpr int kBigParts = 3;   // partial sums kept per chunk
 
 __device__ inline void inst_range(const uint32_t *__restrict__ start, int32_t n_seg, int32_t n, int64_t s,

@@ -10,7 +10,7 @@
 // From-scratch form (Network::calculateCodeLength, Module.cpp:2966-3002):
 // every node's exit-arc flow, then nodes stably sorted by module and one
 // wavefront per module summing its members in FP64 (a module of more than
-// 16384 members: one workgroup per chunk of them, the chunks added in order).
+// 2048 members: one workgroup per chunk of them, the chunks added in order).
 // The tree order is fixed, so the result is reproducible; it differs from the
 // reference's sequential order by rounding only.
 #include "devutil.h"
@@ -308,7 +308,7 @@ __global__ void k_p5_ext(int32_t n, const uint32_t *__restrict__ optr, const uin
 // by k_p5_chunks (one workgroup per chunk) and combined in chunk order by
 // k_p5_big, so that one large module does not hold a single wavefront for
 // the whole launch; every sum keeps an order fixed by the member list.
-constexpr uint32_t kP5Big = 16384;
+constexpr uint32_t kP5Big = 2048;
 
 __device__ inline void p5_seg(const uint32_t *__restrict__ start, uint32_t nseg, int32_t n, uint32_t s,
                               uint32_t &b, uint32_t &e) {

@@ -1458,60 +1458,100 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
                                                 int32_t *mUnits, uint8_t *act_next, int mark, uint32_t *stripes,
                                                 uint32_t big_deg, uint32_t *big_list, uint32_t *big_cnt,
                                                 uint32_t pull_min, uint8_t *wmark, uint32_t *pull_flag) {
+  constexpr int GPB = 256 / G;   // movers per workgroup and round
+  // Each round takes GPB movers; their module-row updates (old and new module
+  // of each) are merged by module in LDS before the atomics, so that movers
+  // of one round joining the same module update its row once.
+  __shared__ uint32_t s_mod[2 * GPB];
+  __shared__ long long s_dv[2 * GPB][4];   // flow, teleport, dangling flow, exit flow
+  __shared__ int32_t s_dn[2 * GPB];        // member nodes (units: +-1)
   const uint32_t nm = counters[cMovers];
   const bool pull = mark && nm >= pull_min;
   if (pull && blockIdx.x == 0 && threadIdx.x == 0) *pull_flag = 1u;
   const int lane = threadIdx.x % G;
-  const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-  const int64_t gstride = (int64_t)gridDim.x * blockDim.x / G;
+  const int gl = threadIdx.x / G;
   uint32_t moved_here = 0;
-  for (int64_t i = g0; i < nm; i += gstride) {
-    const uint32_t u = movers[i];
-    const uint32_t ob = optr[u], oe = optr[u + 1];
-    const bool big = (oe - ob) + (iptr[u + 1] - iptr[u]) > big_deg;
-    if (pull && lane == 0) wmark[u] = 1;
-    if (big) {
-      if (lane == 0) big_list[atomicAdd(big_cnt, 1u)] = u;
-    } else if (mark && !pull) {
-      for (uint32_t k = ob + lane; k < oe; k += G) act_next[oidx[k]] = 1;
-      for (uint32_t k = iptr[u] + lane; k < iptr[u + 1]; k += G) act_next[iidx[k]] = 1;
-    }
-    if (stamp[u] != cur) continue;
-    const uint32_t o = mod[u], n = newmod[u];
-    long long dxo = 0, dxn = 0;
-    for (uint32_t k = ob + lane; k < (big ? ob : oe); k += G) {
-      const uint32_t v = oidx[k];
-      if (stamp[v] != cur) continue;
-      const long long w = aout.fxw(k);
-      const uint32_t ov = mod[v], nv = newmod[v];
-      dxo += w * ((long long)(o != ov) - (long long)(o != nv));
-      dxn += w * ((long long)(n != nv) - (long long)(n != ov));
-    }
-    dxo = group_sum_ll<G>(dxo);
-    dxn = group_sum_ll<G>(dxn);
-    if (lane == 0) {
-      const long long *g = agg + 4 * (int64_t)u;
-      const Stat4 su = us[u];
-      const long long W = su.x;
-      dxo += -(W - g[0]) + g[1];
-      dxn += (W - g[2]) - g[3];
-      atomicAdd((unsigned long long *)&ms[o].p, (unsigned long long)(-su.p));
-      atomicAdd((unsigned long long *)&ms[n].p, (unsigned long long)su.p);
-      atomicAdd((unsigned long long *)&ms[o].t, (unsigned long long)(-su.t));
-      atomicAdd((unsigned long long *)&ms[n].t, (unsigned long long)su.t);
-      if (su.d) {
-        atomicAdd((unsigned long long *)&ms[o].d, (unsigned long long)(-su.d));
-        atomicAdd((unsigned long long *)&ms[n].d, (unsigned long long)su.d);
+  for (int64_t r0 = (int64_t)blockIdx.x * GPB; r0 < nm; r0 += (int64_t)gridDim.x * GPB) {
+    const int64_t i = r0 + gl;
+    bool live = i < nm;   // this group's mover commits a row update this round
+    if (live) {
+      const uint32_t u = movers[i];
+      const uint32_t ob = optr[u], oe = optr[u + 1];
+      const bool big = (oe - ob) + (iptr[u + 1] - iptr[u]) > big_deg;
+      if (pull && lane == 0) wmark[u] = 1;
+      if (big) {
+        if (lane == 0) big_list[atomicAdd(big_cnt, 1u)] = u;
+      } else if (mark && !pull) {
+        for (uint32_t k = ob + lane; k < oe; k += G) act_next[oidx[k]] = 1;
+        for (uint32_t k = iptr[u] + lane; k < iptr[u + 1]; k += G) act_next[iidx[k]] = 1;
       }
-      if (dxo) atomicAdd((unsigned long long *)&ms[o].x, (unsigned long long)dxo);
-      if (dxn) atomicAdd((unsigned long long *)&ms[n].x, (unsigned long long)dxn);
-      const int32_t k = nmem[u];
-      atomicSub(&mNodes[o], k);
-      atomicAdd(&mNodes[n], k);
-      atomicSub(&mUnits[o], 1);
-      atomicAdd(&mUnits[n], 1);
-      moved_here += (uint32_t)k;
+      live = stamp[u] == cur;
+      if (live) {
+        const uint32_t o = mod[u], n = newmod[u];
+        long long dxo = 0, dxn = 0;
+        for (uint32_t k = ob + lane; k < (big ? ob : oe); k += G) {
+          const uint32_t v = oidx[k];
+          if (stamp[v] != cur) continue;
+          const long long w = aout.fxw(k);
+          const uint32_t ov = mod[v], nv = newmod[v];
+          dxo += w * ((long long)(o != ov) - (long long)(o != nv));
+          dxn += w * ((long long)(n != nv) - (long long)(n != ov));
+        }
+        dxo = group_sum_ll<G>(dxo);
+        dxn = group_sum_ll<G>(dxn);
+        if (lane == 0) {
+          const long long *g = agg + 4 * (int64_t)u;
+          const Stat4 su = us[u];
+          const long long W = su.x;
+          dxo += -(W - g[0]) + g[1];
+          dxn += (W - g[2]) - g[3];
+          const int32_t k = nmem[u];
+          s_mod[2 * gl] = o;
+          s_dv[2 * gl][0] = -su.p;
+          s_dv[2 * gl][1] = -su.t;
+          s_dv[2 * gl][2] = -su.d;
+          s_dv[2 * gl][3] = dxo;
+          s_dn[2 * gl] = -k;
+          s_mod[2 * gl + 1] = n;
+          s_dv[2 * gl + 1][0] = su.p;
+          s_dv[2 * gl + 1][1] = su.t;
+          s_dv[2 * gl + 1][2] = su.d;
+          s_dv[2 * gl + 1][3] = dxn;
+          s_dn[2 * gl + 1] = k;
+          moved_here += (uint32_t)k;
+        }
+      }
     }
+    if (!live && lane == 0) {
+      s_mod[2 * gl] = kNone;
+      s_mod[2 * gl + 1] = kNone;
+    }
+    __syncthreads();
+    // entry t: the first entry of its module adds every entry of that module
+    const int t = threadIdx.x;
+    if (t < 2 * GPB && s_mod[t] != kNone) {
+      const uint32_t m = s_mod[t];
+      bool first = true;
+      for (int j = 0; j < t; ++j) first = first && s_mod[j] != m;
+      if (first) {
+        long long dv[4] = {s_dv[t][0], s_dv[t][1], s_dv[t][2], s_dv[t][3]};
+        int32_t dn = s_dn[t], du = (t & 1) ? 1 : -1;
+        for (int j = t + 1; j < 2 * GPB; ++j) {
+          if (s_mod[j] != m) continue;
+#pragma unroll
+          for (int f = 0; f < 4; ++f) dv[f] += s_dv[j][f];
+          dn += s_dn[j];
+          du += (j & 1) ? 1 : -1;
+        }
+        unsigned long long *row = (unsigned long long *)&ms[m];
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+          if (dv[f]) atomicAdd(row + f, (unsigned long long)dv[f]);
+        if (dn) atomicAdd(&mNodes[m], dn);
+        if (du) atomicAdd(&mUnits[m], du);
+      }
+    }
+    __syncthreads();
   }
   moved_here = wave_sum(moved_here);
   if ((threadIdx.x & (kWave - 1)) == 0 && moved_here) atomicAdd(my_stripe(stripes) + cMoved, moved_here);

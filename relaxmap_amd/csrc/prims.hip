@@ -167,9 +167,10 @@ struct SortShape {
 constexpr int kMaxPasses = 8;
 
 // Digit counts of every pass at once: one read of the keys.  Keys often
-// arrive grouped by their high bits (arcs in source order), so a wavefront's
-// digit of a high pass is usually one value: that case is found with one
-// ballot and added by a single lane, instead of 64 atomics on one slot.
+// arrive grouped by their high bits (arcs in source order), so neighbouring
+// lanes of a wavefront often share a digit: each run of equal digits is
+// found with one shuffle and one ballot and added by its first lane, instead
+// of one LDS atomic per lane on the same slot.
 template <typename K>
 __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K *__restrict__ keys, int64_t n, int begin_bit,
                                                           int passes, uint32_t *__restrict__ counts) {
@@ -194,13 +195,18 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K *__restrict__ 
       const uint64_t live = __ballot(valid);
       if (!live) continue;
       const K k = kk[u];
+      const int n_live = __popcll(live);
       for (int q = 0; q < passes; ++q) {
+        // a run of lanes with one digit adds its length with one atomic (its
+        // first lane's): keys grouped by their high bits repeat a digit
         const uint32_t d = (uint32_t)(k >> (begin_bit + 8 * q)) & 0xffu;
-        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-        if (__ballot(valid && d == d0) == live) {
-          if (lane == 0) atomicAdd(&h[q][d0], (uint32_t)__popcll(live));
-        } else if (valid) {
-          atomicAdd(&h[q][d], 1u);
+        const uint32_t dp = __shfl_up(d, 1, kWave);
+        const bool head = valid && (lane == 0 || d != dp);
+        const uint64_t heads = __ballot(head);
+        if (head) {
+          const uint64_t above = lane == kWave - 1 ? 0ull : heads >> (lane + 1);
+          const int next = above ? lane + 1 + __builtin_ctzll(above) : n_live;
+          atomicAdd(&h[q][d], (uint32_t)(next - lane));
         }
       }
     }
