@@ -40,8 +40,21 @@ __global__ void k_intra_flags(const uint32_t *__restrict__ optr, const uint32_t 
   const int64_t a = t / G;
   const int lane = (int)(t % G);
   if (a >= n) return;
+  if (optr[a + 1] - optr[a] > kBigRow) return;   // big rows: k_intra_flags_big
   const uint32_t pa = par[a];
   for (uint32_t k = optr[a] + lane; k < optr[a + 1]; k += G) flag[k] = par[oidx[k]] == pa ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_intra_flags_big(BigRows B, const uint32_t *__restrict__ optr,
+                                                         const uint32_t *__restrict__ oidx,
+                                                         const uint32_t *__restrict__ par, uint32_t *__restrict__ flag) {
+  const uint32_t nch = *B.ctr;
+  for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+    const uint32_t a = B.crow[c], b = B.cbeg[c];
+    const uint32_t e = b + kBigRow < optr[a + 1] ? b + kBigRow : optr[a + 1];
+    const uint32_t pa = par[a];
+    for (uint32_t k = b + threadIdx.x; k < e; k += 256) flag[k] = par[oidx[k]] == pa ? 1u : 0u;
+  }
 }
 
 __global__ void k_intra_ptr(const uint32_t *__restrict__ optr, int32_t n, int64_t m, const uint32_t *__restrict__ apos,
@@ -150,14 +163,21 @@ __device__ inline void inst_range(const uint32_t *__restrict__ start, int32_t n_
 }
 
 // one thread per instance: big ones get a list entry, consecutive chunk slots
-// and the owner of every slot (ctr[0]: big instances, ctr[1]: chunk slots)
+// and the owner of every slot (ctr[0]: big instances, ctr[1]: chunk slots);
+// the other instances of more than one node are listed in `multi` (ctr[2]):
+// after the first totals only these and the big ones can change, since a
+// one-node instance has no move to make
 __global__ void k_big_plan(const uint32_t *__restrict__ start, int32_t n_seg, int32_t n, uint32_t *__restrict__ ctr,
-                           uint32_t *__restrict__ list, uint32_t *__restrict__ base, uint32_t *__restrict__ owner) {
+                           uint32_t *__restrict__ list, uint32_t *__restrict__ base, uint32_t *__restrict__ owner,
+                           uint32_t *__restrict__ multi) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_seg) return;
   uint32_t b, e;
   inst_range(start, n_seg, n, s, b, e);
-  if (e - b <= kBigInst) return;
+  if (e - b <= kBigInst) {
+    if (e - b > 1) multi[atomicAdd(&ctr[2], 1u)] = (uint32_t)s;
+    return;
+  }
   const uint32_t nch = (e - b + kBigInst - 1) / kBigInst;
   const uint32_t li = atomicAdd(&ctr[0], 1u);
   const uint32_t sl = atomicAdd(&ctr[1], nch);
@@ -232,13 +252,16 @@ __device__ inline void inst_terms(const Stat4 *__restrict__ ms, const int32_t *_
   pqp += plogp(q + fx_to(st.p));
 }
 
+// (one wavefront per listed instance, or per instance without a list)
 __global__ void k_inst_totals(const uint32_t *__restrict__ start, int32_t n_seg, int32_t n,
+                              const uint32_t *__restrict__ list, int64_t n_list,
                               const uint32_t *__restrict__ nodes, const uint32_t *__restrict__ par,
                               const Stat4 *__restrict__ ms, const int32_t *__restrict__ mnodes,
                               const double *__restrict__ C, double *__restrict__ S_out, double *__restrict__ L_out) {
-  const int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
-  if (s >= n_seg) return;
+  if (wv >= n_list) return;
+  const int64_t s = list ? (int64_t)list[wv] : wv;
   uint32_t b, e;
   inst_range(start, n_seg, n, s, b, e);
   if (e - b > kBigInst) return;   // k_inst_totals_chunks / k_inst_totals_big
@@ -387,32 +410,37 @@ unsigned big_grid(uint32_t slots) { return (unsigned)std::min<uint32_t>(slots, 2
 void plan_big_instances(Ctx &c) {
   hipStream_t s = c.stream;
   const int64_t cap = (int64_t)c.N / kBigInst + 1;   // big instances; their chunks fit in 2 cap slots
-  c.ibig_ctr.alloc(2);
+  c.ibig_ctr.alloc(3);
   c.ibig_list.alloc(cap);
   c.ibig_base.alloc(cap);
   c.ibig_owner.alloc(2 * cap);
   c.ibig_part.alloc(2 * cap * kBigParts);
-  fill_u32(c.ibig_ctr.p, 0u, 2, s);
-  c.n_ibig = c.n_ibig_slots = 0;
+  c.inst_multi.alloc(std::max<int64_t>(c.n_inst, 1));
+  fill_u32(c.ibig_ctr.p, 0u, 3, s);
+  c.n_ibig = c.n_ibig_slots = c.n_multi = 0;
   if (c.n_inst <= 0) return;
   k_big_plan<<<grid_for(c.n_inst, 256), 256, 0, s>>>(c.inst_start.p, c.n_inst, c.N, c.ibig_ctr.p, c.ibig_list.p,
-                                                     c.ibig_base.p, c.ibig_owner.p);
+                                                     c.ibig_base.p, c.ibig_owner.p, c.inst_multi.p);
   RMX_LAUNCH_CHECK();
-  uint32_t h[2] = {0, 0};
+  uint32_t h[3] = {0, 0, 0};
   RMX_HIP(hipMemcpyAsync(h, c.ibig_ctr.p, sizeof(h), hipMemcpyDeviceToHost, s));
   c.sync();
   c.n_ibig = h[0];
   c.n_ibig_slots = h[1];
+  c.n_multi = h[2];
 }
 
 }  // namespace
 
-void instance_totals_async(Ctx &c) {
+void instance_totals_async(Ctx &c, bool all) {
   if (c.n_inst <= 0) return;
-  k_inst_totals<<<grid_for((int64_t)c.n_inst * kWave, 256), 256, 0, c.stream>>>(
-      c.inst_start.p, c.n_inst, c.N, c.inst_nodes.p, c.sub_par.p, c.ms.p, c.mNodes.p, c.inst_C.p, c.inst_S.p,
-      c.inst_L.p);
-  RMX_LAUNCH_CHECK();
+  const int64_t nw = all ? c.n_inst : c.n_multi;   // wavefronts
+  if (nw > 0) {
+    k_inst_totals<<<grid_for(nw * kWave, 256), 256, 0, c.stream>>>(
+        c.inst_start.p, c.n_inst, c.N, all ? nullptr : c.inst_multi.p, nw, c.inst_nodes.p, c.sub_par.p, c.ms.p,
+        c.mNodes.p, c.inst_C.p, c.inst_S.p, c.inst_L.p);
+    RMX_LAUNCH_CHECK();
+  }
   if (c.n_ibig > 0) {
     k_inst_totals_chunks<<<big_grid(c.n_ibig_slots), 256, 0, c.stream>>>(big_view(c), c.n_ibig_slots, c.inst_start.p,
                                                                          c.n_inst, c.N, c.inst_nodes.p, c.ms.p,
@@ -424,6 +452,24 @@ void instance_totals_async(Ctx &c) {
     RMX_LAUNCH_CHECK();
   }
 }
+
+namespace {
+
+// flags of the arcs of rows of more than kBigRow arcs (left out by k_intra_flags)
+void intra_flags_big(Ctx &c, const uint32_t *ptr, const uint32_t *idx, int32_t n, int64_t m, uint32_t *flag) {
+  DevBuf<uint32_t> bctr(c.cache), bcrow(c.cache), bcbeg(c.cache);
+  DevBuf<long long> barcs(c.cache);
+  bctr.alloc(1);
+  barcs.alloc(1);
+  bcrow.alloc(big_rows_cap(m));
+  bcbeg.alloc(big_rows_cap(m));
+  const BigRows big{bctr.p, bcrow.p, bcbeg.p, (unsigned long long *)barcs.p};
+  plan_big_rows(ptr, n, big, c.stream);
+  k_intra_flags_big<<<big_rows_grid(m), 256, 0, c.stream>>>(big, ptr, idx, c.sub_par.p, flag);
+  RMX_LAUNCH_CHECK();
+}   // the buffers go back to the cache: later work on the stream runs after the kernel
+
+}  // namespace
 
 int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t seed) {
   hipStream_t s = c.stream;
@@ -450,6 +496,7 @@ int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t se
       k_intra_flags<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L0.out_ptr.p, L0.out_idx.p, n, c.sub_par.p,
                                                                         flag.p);
       RMX_LAUNCH_CHECK();
+      intra_flags_big(c, L0.out_ptr.p, L0.out_idx.p, n, m, flag.p);
       exclusive_scan_u32(flag.p, apos.p, m, tot.p, c.scratch, s);
       RMX_HIP(hipMemcpyAsync(&m_in, tot.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
       c.sync();
@@ -479,6 +526,7 @@ int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t se
       k_intra_flags<16><<<grid_for((int64_t)n * 16, 256), 256, 0, s>>>(L0.in_ptr.p, L0.in_idx.p, n, c.sub_par.p,
                                                                         flag.p);
       RMX_LAUNCH_CHECK();
+      intra_flags_big(c, L0.in_ptr.p, L0.in_idx.p, n, m, flag.p);
       exclusive_scan_u32(flag.p, apos.p, m, tot.p, c.scratch, s);
       RMX_HIP(hipMemcpyAsync(&m_in2, tot.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
       c.sync();
@@ -566,7 +614,7 @@ int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t se
   DevBuf<double> L_single(c.cache);   // each instance's codelength as singletons
   L_single.alloc(n);
   try {
-    instance_totals_async(c);
+    instance_totals_async(c, true);
     RMX_HIP(hipMemcpyAsync(L_single.p, c.inst_L.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     k_node_act<<<grid_for(n, 256), 256, 0, s>>>(n, c.sub_par.p, c.inst_act.p, c.act.p);
     RMX_LAUNCH_CHECK();

@@ -309,7 +309,8 @@ struct Ctx {
   // instances too large for one wavefront: reduced in chunks (coarse.hip)
   DevBuf<uint32_t> ibig_ctr, ibig_list, ibig_base, ibig_owner;
   DevBuf<double> ibig_part;
-  uint32_t n_ibig = 0, n_ibig_slots = 0;
+  DevBuf<uint32_t> inst_multi;       // the other instances of more than one node
+  uint32_t n_ibig = 0, n_ibig_slots = 0, n_multi = 0;
   int32_t n_sub = 0;                 // sub-modules after the last partition
 
   // steady-state buffers
@@ -417,7 +418,9 @@ inline int sweep_batches(const Ctx &c, const Level &L) {
 int32_t generate_sub_modules(Ctx &c, double threshold, int max_iter, uint64_t seed);
 // Network::generateSuperNodesFromSubModules (Module.cpp:3083-3158)
 void contract_sub_modules(Ctx &c);
-void instance_totals_async(Ctx &c);   // per-instance S and L of the sub-module partition
+// per-instance S and L of the sub-module partition: of every instance (all),
+// else of those with more than one node (the others cannot change)
+void instance_totals_async(Ctx &c, bool all = false);
 
 // ----- contract.hip
 void update_members(Ctx &c);

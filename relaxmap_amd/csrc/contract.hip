@@ -68,12 +68,47 @@ __global__ void k_arc_keys(const uint32_t *__restrict__ optr, const uint32_t *__
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t a = t / G;
   const int lane = (int)(t % G);
-  if (a >= n) return;
+  if (a >= n || optr[a + 1] - optr[a] > kBigRow) return;   // big rows: k_arc_keys_big
   const uint32_t sa = node_sp[a];
   for (uint32_t k = optr[a] + lane; k < optr[a + 1]; k += G) {
     const uint32_t sb = node_sp[oidx[k]];
     keys[k] = sb != sa ? (((uint64_t)sa << bits) | (uint64_t)sb) : kIntra;
     vals[k] = carry_wf ? __float_as_uint(carry_wf[k]) : k;
+  }
+}
+
+// chunks of the rows of more than kBigRow arcs (the order of the chunks is
+// that of the atomic counter; every chunk's work is independent)
+__global__ void k_big_rows(const uint32_t *__restrict__ ptr, int32_t n, BigRows B) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t b = ptr[r], d = ptr[r + 1] - b;
+  if (d <= kBigRow) return;
+  const uint32_t nch = (d + kBigRow - 1) / kBigRow;
+  const uint32_t at = atomicAdd(B.ctr, nch);
+  atomicAdd(B.arcs, (unsigned long long)d);
+  for (uint32_t k = 0; k < nch; ++k) {
+    B.crow[at + k] = (uint32_t)r;
+    B.cbeg[at + k] = b + k * kBigRow;
+  }
+}
+
+// the arcs of rows of more than kBigRow arcs, left out by k_arc_keys
+__global__ __launch_bounds__(256) void k_arc_keys_big(BigRows B, const uint32_t *__restrict__ optr,
+                                                      const uint32_t *__restrict__ oidx,
+                                                      const uint32_t *__restrict__ node_sp, int bits,
+                                                      const float *__restrict__ carry_wf,
+                                                      uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  const uint32_t nch = *B.ctr;
+  for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+    const uint32_t a = B.crow[c], b = B.cbeg[c];
+    const uint32_t e = b + kBigRow < optr[a + 1] ? b + kBigRow : optr[a + 1];
+    const uint32_t sa = node_sp[a];
+    for (uint32_t k = b + threadIdx.x; k < e; k += 256) {
+      const uint32_t sb = node_sp[oidx[k]];
+      keys[k] = sb != sa ? (((uint64_t)sa << bits) | (uint64_t)sb) : kIntra;
+      vals[k] = carry_wf ? __float_as_uint(carry_wf[k]) : k;
+    }
   }
 }
 
@@ -274,6 +309,14 @@ int bits_for(int64_t n) {
 
 }  // namespace
 
+void plan_big_rows(const uint32_t *ptr, int32_t n, const BigRows &B, hipStream_t s) {
+  fill_u32(B.ctr, 0u, 1, s);
+  fill_i64((long long *)B.arcs, 0, 1, s);
+  if (n <= 0) return;
+  k_big_rows<<<grid_for(n, 256), 256, 0, s>>>(ptr, n, B);
+  RMX_LAUNCH_CHECK();
+}
+
 void update_members(Ctx &c) {
   k_copy_i32<<<grid_for(c.N, 256), 256, 0, c.stream>>>(c.mNodes.p, c.mStale.p, c.N);
   RMX_LAUNCH_CHECK();
@@ -346,13 +389,32 @@ void contract(Ctx &c) {
   uint32_t m_inter = 0;
   if (m > 0) {
     const bool carry = c.precision == 1 && !from_super;   // node level, mixed: the sort carries the FP32 flows
+    DevBuf<uint32_t> bctr(c.cache), bcrow(c.cache), bcbeg(c.cache);
+    DevBuf<long long> barcs(c.cache);
+    bctr.alloc(1);
+    barcs.alloc(1);
+    bcrow.alloc(big_rows_cap(m));
+    bcbeg.alloc(big_rows_cap(m));
+    const BigRows big{bctr.p, bcrow.p, bcbeg.p, (unsigned long long *)barcs.p};
+    plan_big_rows(S.out_ptr.p, ns_src, big, s);
+    long long m_big = 0;   // arcs of the big rows (profiling mode)
+    if (c.prof.on) {
+      RMX_HIP(hipMemcpyAsync(&m_big, barcs.p, sizeof(long long), hipMemcpyDeviceToHost, s));
+      c.sync();
+    }
+    // per arc: head id, its super node (gathered), key and value written,
+    // the FP32 flow when carried; per unit: bounds and its super node
+    const double per_arc = 4 + 4 + 8 + 4 + (carry ? 4 : 0);
     RMX_PROF_BEGIN(c);
     k_arc_keys<16><<<grid_for((int64_t)ns_src * 16, 256), 256, 0, s>>>(S.out_ptr.p, S.out_idx.p, ns_src, usp, bits,
                                                                         carry ? S.out_wf.p : nullptr, keys.p, arcs.p);
     RMX_LAUNCH_CHECK();
-    // per arc: head id, its super node (gathered), key and value written,
-    // the FP32 flow when carried; per unit: bounds and its super node
-    RMX_PROF_END(c, "k_arc_keys<16>", (double)m * (4 + 4 + 8 + 4 + (carry ? 4 : 0)) + (double)ns_src * 12);
+    RMX_PROF_END(c, "k_arc_keys<16>", (double)(m - m_big) * per_arc + (double)ns_src * 12);
+    RMX_PROF_BEGIN(c);
+    k_arc_keys_big<<<big_rows_grid(m), 256, 0, s>>>(big, S.out_ptr.p, S.out_idx.p, usp, bits,
+                                                     carry ? S.out_wf.p : nullptr, keys.p, arcs.p);
+    RMX_LAUNCH_CHECK();
+    RMX_PROF_END(c, "k_arc_keys_big", (double)m_big * per_arc);
     radix_sort_pairs_u64(keys.p, arcs.p, m, 0, 2 * bits, c.scratch, s, c.prof.on ? &c.prof : nullptr);
     k_count_inter<<<1, 1, 0, s>>>(keys.p, m, atot.p);
     RMX_LAUNCH_CHECK();

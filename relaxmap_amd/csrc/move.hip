@@ -1740,9 +1740,14 @@ __global__ void k_qv_touched(const uint32_t *__restrict__ movers, const uint32_t
   }
 }
 
+// Every writer of bit m sets it to the same value (from m's final count), so
+// a bit that already holds it needs no atomic: movers joining one popular
+// module then do not all update the same word.
 __device__ inline void set_one(uint32_t *one, const int32_t *mUnits, uint32_t m) {
   const uint32_t bit = 1u << (m & 31u);
-  if (mUnits[m] == 1) atomicOr(&one[m >> 5], bit);
+  const bool want = mUnits[m] == 1;
+  if (((one[m >> 5] & bit) != 0) == want) return;
+  if (want) atomicOr(&one[m >> 5], bit);
   else atomicAnd(&one[m >> 5], ~bit);
 }
 
@@ -1764,10 +1769,10 @@ __global__ void k_apply(const uint32_t *__restrict__ movers, const uint32_t *__r
 }
 
 // one-unit bits of modules [0, n), and with stale counts (node level) their
-// more-than-one-node bits: 32 modules per thread
-// One module per lane (coalesced reads); a wavefront's 64 bits are its
-// ballot, written as two words by lanes 0 and 1.  Launched with 256-lane
-// blocks, so every wavefront starts at a multiple of 64 modules.
+// more-than-one-node bits.  One module per lane (coalesced reads); a
+// wavefront's 64 bits are its ballot, written as two words by lanes 0 and 1.
+// Launched with 256-lane blocks, so every wavefront starts at a multiple of
+// 64 modules.
 __global__ __launch_bounds__(256) void k_one_bits(const int32_t *__restrict__ mUnits,
                                                   const int32_t *__restrict__ mStale, int32_t n,
                                                   uint32_t *__restrict__ one, uint32_t *__restrict__ multi) {
