@@ -124,8 +124,8 @@ __global__ void k_count_inter(const uint64_t *__restrict__ keys, int64_t m, uint
 }
 
 // Segments of the sorted inter keys (one per super arc), in two reads of the
-// keys: k_seg_count counts each tile's heads (positions whose key differs
-// from the one before), and after a scan of the tile counts k_seg_starts
+// keys: k_arc_seg_count counts each tile's heads (positions whose key differs
+// from the one before), and after a scan of the tile counts k_arc_seg_starts
 // writes every segment's first position and, where the source super node
 // changes between two segments, the out-pointers of the sources in between.
 // A tile is 16 rounds of 256 consecutive keys; a head's rank inside its tile
@@ -133,13 +133,13 @@ __global__ void k_count_inter(const uint64_t *__restrict__ keys, int64_t m, uint
 constexpr int kSegItems = 16;
 constexpr int kSegTile = 256 * kSegItems;
 
-__device__ inline bool seg_head(const uint64_t *__restrict__ keys, int64_t j, uint64_t &k, uint64_t &kp) {
+__device__ inline bool arc_seg_head(const uint64_t *__restrict__ keys, int64_t j, uint64_t &k, uint64_t &kp) {
   k = keys[j];
   kp = j > 0 ? keys[j - 1] : ~k;
   return k != kp;
 }
 
-__global__ __launch_bounds__(256) void k_seg_count(const uint64_t *__restrict__ keys, int64_t m,
+__global__ __launch_bounds__(256) void k_arc_seg_count(const uint64_t *__restrict__ keys, int64_t m,
                                                    uint32_t *__restrict__ tile_cnt) {
   __shared__ uint32_t lds[256 / kWave];
   const int64_t base = (int64_t)blockIdx.x * kSegTile;
@@ -148,14 +148,14 @@ __global__ __launch_bounds__(256) void k_seg_count(const uint64_t *__restrict__ 
   for (int i = 0; i < kSegItems; ++i) {
     const int64_t j = base + (int64_t)i * 256 + threadIdx.x;
     uint64_t k, kp;
-    if (j < m && seg_head(keys, j, k, kp)) ++c;
+    if (j < m && arc_seg_head(keys, j, k, kp)) ++c;
   }
   c = block_sum<uint32_t, 256>(c, lds);
   if (threadIdx.x == 0) tile_cnt[blockIdx.x] = c;
 }
 
 // tile_base: exclusive scan of the tile counts; *n_seg: their total
-__global__ __launch_bounds__(256) void k_seg_starts(const uint64_t *__restrict__ keys, int64_t m, int bits,
+__global__ __launch_bounds__(256) void k_arc_seg_starts(const uint64_t *__restrict__ keys, int64_t m, int bits,
                                                     int32_t ns, const uint32_t *__restrict__ tile_base,
                                                     const uint32_t *__restrict__ n_seg, uint32_t *__restrict__ start,
                                                     uint32_t *__restrict__ out_ptr) {
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void k_seg_starts(const uint64_t *__restrict__
 #pragma unroll
   for (int i = 0; i < kSegItems; ++i) {
     const int64_t j = base + (int64_t)i * 256 + t;
-    const bool h = j < m && seg_head(keys, j, key[i], prev[i]);
+    const bool h = j < m && arc_seg_head(keys, j, key[i], prev[i]);
     const uint64_t bal = __ballot(h);
     if (lane == 0) cnt[i * NW + w] = (uint32_t)__popcll(bal);
     rank[i] = (uint32_t)__popcll(bal & below);
@@ -431,19 +431,19 @@ void contract(Ctx &c) {
     tcnt.alloc(nt);
     kstart.alloc(m_inter);   // capacity: at most one segment per key
     RMX_PROF_BEGIN(c);
-    k_seg_count<<<(unsigned)nt, 256, 0, s>>>(keys.p, m_inter, tcnt.p);
+    k_arc_seg_count<<<(unsigned)nt, 256, 0, s>>>(keys.p, m_inter, tcnt.p);
     RMX_LAUNCH_CHECK();
-    RMX_PROF_END(c, "k_seg_count", (double)m_inter * 8 + (double)nt * 4);
+    RMX_PROF_END(c, "k_arc_seg_count", (double)m_inter * 8 + (double)nt * 4);
     exclusive_scan_u32(tcnt.p, tcnt.p, nt, atot.p, c.scratch, s);
     RMX_PROF_BEGIN(c);
-    k_seg_starts<<<(unsigned)nt, 256, 0, s>>>(keys.p, m_inter, bits, (int32_t)ns, tcnt.p, atot.p, kstart.p,
+    k_arc_seg_starts<<<(unsigned)nt, 256, 0, s>>>(keys.p, m_inter, bits, (int32_t)ns, tcnt.p, atot.p, kstart.p,
                                                D.out_ptr.p);
     RMX_LAUNCH_CHECK();
     const float ms_starts = c.prof.on ? c.prof.stop(s) : 0.f;
     RMX_HIP(hipMemcpyAsync(&m_super, atot.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     c.sync();
     // per key: the key read; per segment: its start written; per super node: its out-pointer
-    if (c.prof.on) c.prof.add("k_seg_starts", ms_starts, (double)m_inter * 8 + (double)m_super * 4 + (double)ns * 4);
+    if (c.prof.on) c.prof.add("k_arc_seg_starts", ms_starts, (double)m_inter * 8 + (double)m_super * 4 + (double)ns * 4);
     tcnt.free();
   } else {
     fill_u32(D.out_ptr.p, 0u, (int64_t)ns + 1, s);
