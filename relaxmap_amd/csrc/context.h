@@ -210,6 +210,25 @@ struct Profiler {
     if ((c).prof.on) (c).prof.end((c).stream, (name), (double)(bytes)); \
   } while (0)
 
+// The bin-major slot layout of the blocked push / pull steady state
+// (flow.hip): bins of R consecutive rows, every out-arc's slot in its bin and
+// every slot's head relative to the bin, plus the products' array.  It depends
+// only on the network's structure and the product type, so it is kept from
+// one steady state to the next (Ctx::pb) until the next upload.
+struct PushBins {
+  int32_t R = 0, nbins = 0, rshift = 0;
+  int64_t ntiles = 0;
+  bool fp64 = false;   // FP64 products (val64) instead of FP32 (val)
+  DevBuf<uint32_t> bptr, slot, trow;
+  DevBuf<uint16_t> bdst;
+  DevBuf<float> val;
+  DevBuf<double> val64;
+  void alloc_values(int64_t m) {
+    if (fp64) val64.alloc(m);
+    else val.alloc(m);
+  }
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -329,6 +348,9 @@ struct Ctx {
 
   Scratch scratch, scratch2;
   BlockCache cache;   // temporaries of contraction, sorting and codelength
+  uint64_t graph_version = 0;   // bumped by every upload
+  PushBins pb;                  // the steady state's layout, for graph version pb_graph
+  uint64_t pb_graph = 0;
   Timing timing;
   Profiler prof;
   uint64_t sweep_counter = 0;

@@ -943,23 +943,6 @@ void spmv_all(Ctx &c, const PullCSR &L, const RowBins &rb, double *y) {
   launch_spmv_bin<0>(5, rb, L, c, y);
 }
 
-// The bin-major layout of the push / pull (mixed precision), built once per
-// steady state from the out-CSR.
-struct PushBins {
-  int32_t R = 0, nbins = 0, rshift = 0;
-  int64_t ntiles = 0;
-  bool fp64 = false;   // FP64 products (val64) instead of FP32 (val)
-  DevBuf<uint32_t> bptr, slot, trow;
-  DevBuf<uint16_t> bdst;
-  DevBuf<float> val;
-  DevBuf<double> val64;
-  explicit PushBins(BlockCache &bc) : bptr(bc), slot(bc), trow(bc), bdst(bc), val(bc), val64(bc) {}
-  void alloc_values(int64_t m) {
-    if (fp64) val64.alloc(m);
-    else val.alloc(m);
-  }
-};
-
 // Rows per bin: a power of two from 256 to kPullRowsMax (kPullRowsMax64 for
 // FP64 products), about n / 2048, so that a large network gives thousands of
 // bins (workgroups of the pull).  The slots come from the per-tile counts
@@ -1159,10 +1142,9 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
   DevBuf<double> pw(c.cache), ptau(c.cache);
   DevBuf<float> pwf(c.cache);
   DevBuf<uint8_t> pdang(c.cache);
-  PushBins pb(c.cache);
+  PushBins &pb = c.pb;
   PullCSR F;
   F.n = n;
-  pb.fp64 = !mixed;
   const bool relabel = !batch && !blocked;
   if (!relabel) {   // instances keep their node ranges
     F.ptr = L.in_ptr.p;
@@ -1172,7 +1154,13 @@ void steady_state(Ctx &c, int max_iter, int min_iter, double tol) {
     F.dangling = c.dangling.p;
     F.Dv = batch ? c.b_D.p : c.scal.p + kD;
     F.rinst = batch ? c.b_node.p : nullptr;
-    if (blocked) build_push_bins(c, L, pb);
+    // the layout depends on the network's structure and the product type
+    // only: built by the first steady state after an upload, then reused
+    if (blocked && (c.pb_graph != c.graph_version || pb.fp64 != !mixed)) {
+      pb.fp64 = !mixed;
+      build_push_bins(c, L, pb);
+      c.pb_graph = c.graph_version;
+    }
   } else {
     F.Dv = c.scal.p + kD;
     inv.alloc(n);

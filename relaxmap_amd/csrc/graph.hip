@@ -236,6 +236,7 @@ void upload_graph(Ctx &c, int32_t n, int64_t m, const int64_t *out_ptr, const in
   upload_csr(c, n, m, out_ptr, out_idx, w, node_w);
   build_buckets(c, c.L0);
   c.initiated = false;
+  ++c.graph_version;
 }
 
 void upload_graph_replicated(Ctx &c, int32_t G, const int32_t *goff, int32_t R, int32_t n, int64_t m,
@@ -317,6 +318,7 @@ void upload_graph_replicated(Ctx &c, int32_t G, const int32_t *goff, int32_t R, 
   c.sync();   // the networks' buffers and the host tables go out of scope
   build_buckets(c, L);
   c.initiated = false;
+  ++c.graph_version;
 }
 
 namespace {
