@@ -97,7 +97,7 @@ struct EvalArgs {
   uint32_t mv_cap;
   uint32_t *counters;
   uint32_t *stripes;     // kStripes copies of the per-launch counters
-  uint32_t *want;
+  uint32_t *want;        // empty-module requests of the batch: one bit per unit
   // hub tables: region [hub_off[g], hub_off[g+1]) of the pool for hub g
   uint32_t *hkeys;
   long long *hout, *hin;
@@ -280,7 +280,7 @@ __device__ inline void emit(const EvalArgs &a, uint32_t u, const Best &b, long l
   uint32_t *st = my_stripe(a.stripes);
   put_mover(a, atomicAdd(&st[cMovers], 1u), u);
   if (b.m == a.empty_key) {
-    a.want[u] = 1u;
+    atomicOr(&a.want[u >> 5], 1u << (u & 31u));
     atomicAdd(&st[cWant], 1u);
   }
 }
@@ -314,7 +314,7 @@ __device__ inline void emit_wave(const EvalArgs &a, bool active, uint32_t u, con
   g[3] = fi_n;
   put_mover(a, base + (uint32_t)__popcll(bal & lanes_below()), u);
   if (b.m == a.empty_key) {
-    a.want[u] = 1u;
+    atomicOr(&a.want[u >> 5], 1u << (u & 31u));
     atomicAdd(&st[cWant], 1u);
   }
 }
@@ -1349,19 +1349,34 @@ __global__ __launch_bounds__(256) void k_hubp_finish(EvalArgs a, HubParts H) {
 // Final target of every mover: a neighbour module, an empty module by rank
 // among this batch's empty requests (ascending unit id), or invalid when no
 // empty module is left (Module.cpp:1138-1152).
+// Empty-module requests of a batch: one bit per unit (set by the evaluation,
+// cleared once the batch is resolved).  A request's rank among the batch's
+// requests, in unit order, is the requests of the words before its own plus
+// those of the lower bits of its word: want_pre holds the exclusive scan of
+// the words' bit counts, 32 times shorter than a scan over the units.
+__device__ inline uint32_t want_rank(const uint32_t *__restrict__ want, const uint32_t *__restrict__ want_pre,
+                                     uint32_t u) {
+  return want_pre[u >> 5] + (uint32_t)__popc(want[u >> 5] & ((1u << (u & 31u)) - 1u));
+}
+
+__global__ void k_popc_words(const uint32_t *__restrict__ w, int64_t nw, uint32_t *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nw) out[i] = (uint32_t)__popc(w[i]);
+}
+
 __global__ void k_resolve(const uint32_t *__restrict__ movers, const uint32_t *__restrict__ counters,
-                          const uint32_t *__restrict__ target, const uint32_t *__restrict__ rank,
+                          const uint32_t *__restrict__ target, const uint32_t *__restrict__ want,
+                          const uint32_t *__restrict__ want_pre,
                           const uint32_t *__restrict__ empty_list, uint32_t empty_cursor, uint32_t n_empty,
                           uint32_t empty_key, uint32_t *__restrict__ newmod, uint32_t *__restrict__ stamp,
-                          uint32_t *__restrict__ want, uint32_t cur) {
+                          uint32_t cur) {
   const uint32_t nm = counters[cMovers];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nm; i += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t u = movers[i];
     uint32_t t = target[u];
     if (t == empty_key) {
-      const uint32_t r = empty_cursor + rank[u];
+      const uint32_t r = empty_cursor + want_rank(want, want_pre, u);
       t = r < n_empty ? empty_list[r] : kInvalid;
-      want[u] = 0u;
     }
     newmod[u] = t;
     if (t != kInvalid) stamp[u] = cur;
@@ -1369,17 +1384,18 @@ __global__ void k_resolve(const uint32_t *__restrict__ movers, const uint32_t *_
 }
 
 // The same for a batch: an empty-module request is served from the
-// requester's instance, by its rank among that instance's requests.  rank is
-// the exclusive scan of the requests over the level's units (grouped by
-// instance), *total its sum.
+// requester's instance, by its rank among that instance's requests: its rank
+// among the level's requests (units grouped by instance) less that of the
+// instance's first unit; *total counts all requests.
 __global__ void k_resolve_batch(const uint32_t *__restrict__ movers, const uint32_t *__restrict__ counters,
-                                const uint32_t *__restrict__ target, const uint32_t *__restrict__ rank,
+                                const uint32_t *__restrict__ target, const uint32_t *__restrict__ want,
+                                const uint32_t *__restrict__ want_pre,
                                 const uint32_t *__restrict__ total, int32_t n_units,
                                 const uint32_t *__restrict__ uinst, const uint32_t *__restrict__ uoff,
                                 const uint32_t *__restrict__ empty_list, const uint32_t *__restrict__ e_beg,
                                 const uint32_t *__restrict__ e_cnt, const uint32_t *__restrict__ e_cur,
                                 uint32_t empty_key, uint32_t *__restrict__ newmod, uint32_t *__restrict__ stamp,
-                                uint32_t *__restrict__ want, uint32_t cur) {
+                                uint32_t cur) {
   const uint32_t nm = counters[cMovers];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nm; i += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t u = movers[i];
@@ -1387,10 +1403,9 @@ __global__ void k_resolve_batch(const uint32_t *__restrict__ movers, const uint3
     if (t == empty_key) {
       const uint32_t I = uinst[u];
       const uint32_t first = uoff[I];
-      const uint32_t before = (int32_t)first < n_units ? rank[first] : *total;
-      const uint32_t r = e_cur[I] + (rank[u] - before);
+      const uint32_t before = (int32_t)first < n_units ? want_rank(want, want_pre, first) : *total;
+      const uint32_t r = e_cur[I] + (want_rank(want, want_pre, u) - before);
       t = r < e_cnt[I] ? empty_list[e_beg[I] + r] : kInvalid;
-      want[u] = 0u;
     }
     newmod[u] = t;
     if (t != kInvalid) stamp[u] = cur;
@@ -1398,14 +1413,14 @@ __global__ void k_resolve_batch(const uint32_t *__restrict__ movers, const uint3
 }
 
 // after k_resolve_batch: each instance's taken empty modules
-__global__ void k_bempty_advance(int32_t nb, const uint32_t *__restrict__ uoff, const uint32_t *__restrict__ rank,
-                                 const uint32_t *__restrict__ total, int32_t n_units,
-                                 const uint32_t *__restrict__ e_cnt, uint32_t *__restrict__ e_cur) {
+__global__ void k_bempty_advance(int32_t nb, const uint32_t *__restrict__ uoff, const uint32_t *__restrict__ want,
+                                 const uint32_t *__restrict__ want_pre, const uint32_t *__restrict__ total,
+                                 int32_t n_units, const uint32_t *__restrict__ e_cnt, uint32_t *__restrict__ e_cur) {
   const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= nb) return;
   const uint32_t f = uoff[I], l = uoff[I + 1];
-  const uint32_t a = (int32_t)f < n_units ? rank[f] : *total;
-  const uint32_t b = (int32_t)l < n_units ? rank[l] : *total;
+  const uint32_t a = (int32_t)f < n_units ? want_rank(want, want_pre, f) : *total;
+  const uint32_t b = (int32_t)l < n_units ? want_rank(want, want_pre, l) : *total;
   const uint32_t req = b - a, left = e_cnt[I] - e_cur[I];
   e_cur[I] += req < left ? req : left;
 }
@@ -1997,6 +2012,17 @@ __global__ void k_class_key(const uint32_t *__restrict__ optr, const uint32_t *_
   vals[i] = (uint32_t)i;
 }
 
+int64_t want_words(int64_t n) { return n / 32 + 1; }
+
+// want_pre (in c.rank_tmp) = exclusive scan of the request words' bit counts;
+// *total (device, may be null) = the number of requests
+void want_prefix(Ctx &c, int64_t n, uint32_t *total, hipStream_t s) {
+  const int64_t nw = want_words(n);
+  k_popc_words<<<grid_for(nw, 256), 256, 0, s>>>(c.want_empty.p, nw, c.rank_tmp.p);
+  RMX_LAUNCH_CHECK();
+  exclusive_scan_u32(c.rank_tmp.p, c.rank_tmp.p, nw, total, c.scratch, s);
+}
+
 unsigned stride_grid(int64_t items, int per_block) {
   int64_t g = (items + per_block - 1) / per_block;
   if (g > 8192) g = 8192;
@@ -2111,7 +2137,7 @@ void ensure_sweep_buffers(Ctx &c) {
   c.mv_cap = (uint32_t)((n + 63) / 64 + 1024);
   c.movers_st.alloc((int64_t)kStripes * c.mv_cap);
   c.empty_list.alloc(n);
-  c.want_empty.alloc(n);
+  c.want_empty.alloc(want_words(n));
   c.rank_tmp.alloc(n);
   c.counters.alloc(kCounterWords);
   c.agg_buf.alloc(4 * (int64_t)n);
@@ -2121,7 +2147,7 @@ void ensure_sweep_buffers(Ctx &c) {
   fill_u8(c.act_next.p, 0, n, c.stream);
   fill_u8(c.wmark.p, 0, n, c.stream);
   fill_u32(c.stamp.p, 0u, n, c.stream);
-  fill_u32(c.want_empty.p, 0u, n, c.stream);
+  fill_u32(c.want_empty.p, 0u, want_words(n), c.stream);
   fill_u32(c.counters.p, 0u, kCounterWords, c.stream);
 }
 
@@ -2454,24 +2480,26 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
     }
     ++c.stamp_now;
     if (batch) {
-      if (n_want > 0) exclusive_scan_u32(c.want_empty.p, c.rank_tmp.p, n, c.counters.p + 9, c.scratch, s);
+      if (n_want > 0) want_prefix(c, n, c.counters.p + 9, s);
       k_resolve_batch<<<stride_grid(n_movers, 256), 256, 0, s>>>(
-          c.movers.p, c.counters.p, c.target.p, c.rank_tmp.p, c.counters.p + 9, n, L.uinst.p, L.uoff.p,
-          c.empty_list.p, c.b_ebeg.p, c.b_ecnt.p, c.b_ecur.p, a.empty_key, c.newmod.p, c.stamp.p, c.want_empty.p,
+          c.movers.p, c.counters.p, c.target.p, c.want_empty.p, c.rank_tmp.p, c.counters.p + 9, n, L.uinst.p,
+          L.uoff.p, c.empty_list.p, c.b_ebeg.p, c.b_ecnt.p, c.b_ecur.p, a.empty_key, c.newmod.p, c.stamp.p,
           c.stamp_now);
       RMX_LAUNCH_CHECK();
       if (n_want > 0) {
-        k_bempty_advance<<<grid_for(c.nb, 256), 256, 0, s>>>(c.nb, L.uoff.p, c.rank_tmp.p, c.counters.p + 9, n,
-                                                              c.b_ecnt.p, c.b_ecur.p);
+        k_bempty_advance<<<grid_for(c.nb, 256), 256, 0, s>>>(c.nb, L.uoff.p, c.want_empty.p, c.rank_tmp.p,
+                                                              c.counters.p + 9, n, c.b_ecnt.p, c.b_ecur.p);
         RMX_LAUNCH_CHECK();
+        fill_u32(c.want_empty.p, 0u, want_words(n), s);   // the requests are served
       }
     } else {
-      if (n_want > 0 && n_empty > 0) exclusive_scan_u32(c.want_empty.p, c.rank_tmp.p, n, nullptr, c.scratch, s);
-      k_resolve<<<stride_grid(n_movers, 256), 256, 0, s>>>(c.movers.p, c.counters.p, c.target.p, c.rank_tmp.p,
-                                                            c.empty_list.p, empty_cursor, n_empty, a.empty_key,
-                                                            c.newmod.p, c.stamp.p, c.want_empty.p, c.stamp_now);
+      if (n_want > 0 && n_empty > 0) want_prefix(c, n, nullptr, s);
+      k_resolve<<<stride_grid(n_movers, 256), 256, 0, s>>>(c.movers.p, c.counters.p, c.target.p, c.want_empty.p,
+                                                            c.rank_tmp.p, c.empty_list.p, empty_cursor, n_empty,
+                                                            a.empty_key, c.newmod.p, c.stamp.p, c.stamp_now);
       RMX_LAUNCH_CHECK();
       if (n_want > 0) {
+        fill_u32(c.want_empty.p, 0u, want_words(n), s);   // the requests are served
         uint32_t left = n_empty - empty_cursor;
         empty_cursor += n_want < left ? n_want : left;
       }
