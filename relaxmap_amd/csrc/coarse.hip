@@ -88,7 +88,14 @@ __global__ void k_sub_units(const uint32_t *__restrict__ sptr, int32_t n, ArcW a
   const int lane = (int)(t % G);
   if (a >= n) return;
   long long x = 0;
-  for (uint32_t k = sptr[a] + lane; k < sptr[a + 1]; k += G) x += aw.fxw(k);
+  const uint32_t e = sptr[a + 1];
+  for (uint32_t k0 = sptr[a] + lane; k0 < e; k0 += kRowAhead * G) {
+    long long v[kRowAhead];
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) v[j] = k0 + j * G < e ? aw.fxw(k0 + j * G) : 0ll;
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) x += v[j];
+  }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, G);
   if (lane == 0) {
@@ -378,7 +385,14 @@ __global__ void k_super_fix(const uint32_t *__restrict__ optr, const long long *
   const int lane = (int)(t % G);
   if (a >= ns) return;
   long long x = 0;
-  for (uint32_t k = optr[a] + lane; k < optr[a + 1]; k += G) x += owx[k];
+  const uint32_t e = optr[a + 1];
+  for (uint32_t k0 = optr[a] + lane; k0 < e; k0 += kRowAhead * G) {
+    long long v[kRowAhead];
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) v[j] = k0 + j * G < e ? owx[k0 + j * G] : 0ll;
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) x += v[j];
+  }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, G);
   if (lane == 0) {

@@ -253,8 +253,17 @@ __global__ void k_exit_flows(int32_t n, const uint32_t *__restrict__ optr, const
   if (u >= n) return;
   const uint32_t m = mod[u];
   long long acc = 0;
-  for (uint32_t k = optr[u] + lane; k < optr[u + 1]; k += G)
-    if (mod[oidx[k]] != m) acc += aw.fxw(k);
+  const uint32_t e = optr[u + 1];
+  for (uint32_t k0 = optr[u] + lane; k0 < e; k0 += kRowAhead * G) {
+    uint32_t v[kRowAhead];
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) v[j] = k0 + j * G < e ? oidx[k0 + j * G] : 0u;
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) v[j] = k0 + j * G < e ? mod[v[j]] : m;
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j)
+      if (v[j] != m) acc += aw.fxw(k0 + j * G);
+  }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
   if (lane == 0 && acc != 0) atomicAdd((unsigned long long *)&ms[m].x, (unsigned long long)acc);
@@ -294,8 +303,21 @@ __global__ void k_p5_ext(int32_t n, const uint32_t *__restrict__ optr, const uin
   if (a >= n) return;
   const uint32_t m = mod[a];
   double acc = 0.0;
-  for (uint32_t k = optr[a] + lane; k < optr[a + 1]; k += G)
-    if (mod[oidx[k]] != m) acc += ow[k];
+  const uint32_t e = optr[a + 1];
+  for (uint32_t k0 = optr[a] + lane; k0 < e; k0 += kRowAhead * G) {
+    uint32_t v[kRowAhead];
+    double w[kRowAhead];
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) v[j] = k0 + j * G < e ? oidx[k0 + j * G] : 0u;
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) {
+      w[j] = k0 + j * G < e ? ow[k0 + j * G] : 0.0;
+      v[j] = k0 + j * G < e ? mod[v[j]] : m;
+    }
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j)   // in the plain loop's order
+      if (v[j] != m) acc += w[j];
+  }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
   if (lane == 0) ext[a] = acc;

@@ -31,6 +31,11 @@ constexpr int kMaxBatches = 256 / kNumBuckets - 1;
 // the neighbour's module), each aggregated in an LDS table.
 constexpr int kHubPart = 1024;
 constexpr int kTotChunk = 4096;   // module ids per workgroup of the per-instance totals
+// Loads that a lane of a G-lane row kernel keeps in flight: a hub row of 60k
+// arcs then costs its 16 lanes about 470 round trips to memory instead of
+// 3750.  Each lane still adds its arcs in the order of the plain loop (k, k +
+// G, k + 2G, ...), so every sum is bit-identical to it.
+constexpr int kRowAhead = 8;
 
 // Device blocks kept for reuse within one context.  All work of a context
 // runs on one stream, so a block handed back while kernels that use it are

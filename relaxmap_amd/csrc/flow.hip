@@ -73,16 +73,32 @@ __global__ void k_normalize(const uint32_t *__restrict__ optr, int32_t n, const 
   if (r >= n) return;
   const uint32_t b = optr[r], e = optr[r + 1];
   double s = 0.0;
-  for (uint32_t k = b + lane; k < e; k += G) s += raw[k];
+  for (uint32_t k0 = b + lane; k0 < e; k0 += kRowAhead * G) {
+    double v[kRowAhead];
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) v[j] = k0 + j * G < e ? raw[k0 + j * G] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j)
+      if (k0 + j * G < e) s += v[j];
+  }
   s = group_sum<G>(s);
   if (lane == 0) {
     dangling[r] = (b == e) ? 1 : 0;
     rowsum[r] = s;
   }
-  for (uint32_t k = b + lane; k < e; k += G) {
-    double v = raw[k] / s;
-    out_w[k] = v;
-    if (MIXED) out_wf[k] = (float)v;
+  for (uint32_t k0 = b + lane; k0 < e; k0 += kRowAhead * G) {
+    double v[kRowAhead];
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) v[j] = k0 + j * G < e ? raw[k0 + j * G] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) {
+      const uint32_t k = k0 + j * G;
+      if (k < e) {
+        const double q = v[j] / s;
+        out_w[k] = q;
+        if (MIXED) out_wf[k] = (float)q;
+      }
+    }
   }
 }
 
@@ -759,15 +775,23 @@ __global__ void k_flows(const uint32_t *__restrict__ optr, int32_t n, double *__
   const double pr = x[r];
   double sf = 0.0;
   long long sx = 0;
-  for (uint32_t k = b + lane; k < e; k += G) {
-    double f = pr * out_w[k];
-    float ff = (float)f;
-    out_w[k] = f;
-    if (mixed) out_wf[k] = ff;   // the contraction's weights; the sweeps read out_wx
-    sf += f;
-    const long long fx = fx_of(mixed ? (double)ff : f);
-    out_wx[k] = fx;
-    sx += fx;
+  for (uint32_t k0 = b + lane; k0 < e; k0 += kRowAhead * G) {
+    double w[kRowAhead];
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) w[j] = k0 + j * G < e ? out_w[k0 + j * G] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) {
+      const uint32_t k = k0 + j * G;
+      if (k >= e) break;
+      const double f = pr * w[j];
+      const float ff = (float)f;
+      out_w[k] = f;
+      if (mixed) out_wf[k] = ff;   // the contraction's weights; the sweeps read out_wx
+      sf += f;
+      const long long fx = fx_of(mixed ? (double)ff : f);
+      out_wx[k] = fx;
+      sx += fx;
+    }
   }
   sf = group_sum<G>(sf);
   sx = group_sum_ll<G>(sx);
