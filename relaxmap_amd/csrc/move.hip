@@ -1447,6 +1447,21 @@ __global__ void k_bempty_seg(int32_t nb, const uint32_t *__restrict__ off, const
   e_cur[I] = 0;
 }
 
+// the heads idx[b..e) of one unit's arcs become active (G lanes, kRowAhead
+// heads per lane in flight)
+template <int G>
+__device__ inline void mark_heads(const uint32_t *__restrict__ idx, uint32_t b, uint32_t e, int lane,
+                                  uint8_t *act_next) {
+  for (uint32_t k0 = b + lane; k0 < e; k0 += kRowAhead * G) {
+    uint32_t v[kRowAhead];
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) v[j] = k0 + j * G < e ? idx[k0 + j * G] : 0u;
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j)
+      if (k0 + j * G < e) act_next[v[j]] = 1;
+  }
+}
+
 // G lanes per mover: module statistics, exit flows and activity marks.
 // X changes of a move of u from o to n with every neighbour fixed:
 //   X_o += -(W_u - A_o) + B_o,  X_n += (W_u - A_n) - B_n;
@@ -1497,20 +1512,30 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
       if (big) {
         if (lane == 0) big_list[atomicAdd(big_cnt, 1u)] = u;
       } else if (mark && !pull) {
-        for (uint32_t k = ob + lane; k < oe; k += G) act_next[oidx[k]] = 1;
-        for (uint32_t k = iptr[u] + lane; k < iptr[u + 1]; k += G) act_next[iidx[k]] = 1;
+        mark_heads<G>(oidx, ob, oe, lane, act_next);
+        mark_heads<G>(iidx, iptr[u], iptr[u + 1], lane, act_next);
       }
       live = stamp[u] == cur;
       if (live) {
         const uint32_t o = mod[u], n = newmod[u];
         long long dxo = 0, dxn = 0;
-        for (uint32_t k = ob + lane; k < (big ? ob : oe); k += G) {
-          const uint32_t v = oidx[k];
-          if (stamp[v] != cur) continue;
-          const long long w = aout.fxw(k);
-          const uint32_t ov = mod[v], nv = newmod[v];
-          dxo += w * ((long long)(o != ov) - (long long)(o != nv));
-          dxn += w * ((long long)(n != nv) - (long long)(n != ov));
+        // kRowAhead arcs per lane in flight: heads, then their stamps
+        const uint32_t oend = big ? ob : oe;
+        for (uint32_t k0 = ob + lane; k0 < oend; k0 += kRowAhead * G) {
+          uint32_t v[kRowAhead];
+          bool co[kRowAhead];
+#pragma unroll
+          for (int j = 0; j < kRowAhead; ++j) v[j] = k0 + j * G < oend ? oidx[k0 + j * G] : 0u;
+#pragma unroll
+          for (int j = 0; j < kRowAhead; ++j) co[j] = k0 + j * G < oend && stamp[v[j]] == cur;
+#pragma unroll
+          for (int j = 0; j < kRowAhead; ++j) {
+            if (!co[j]) continue;
+            const long long w = aout.fxw(k0 + j * G);
+            const uint32_t ov = mod[v[j]], nv = newmod[v[j]];
+            dxo += w * ((long long)(o != ov) - (long long)(o != nv));
+            dxn += w * ((long long)(n != nv) - (long long)(n != ov));
+          }
         }
         dxo = group_sum_ll<G>(dxo);
         dxn = group_sum_ll<G>(dxn);
