@@ -223,14 +223,28 @@ __global__ void k_reduce_segments(const uint32_t *__restrict__ start, int64_t n_
   const uint32_t e = (s + 1 < n_seg) ? start[s + 1] : (uint32_t)m;
   double sum = 0.0;
   long long sx = 0;
-  for (uint32_t i = b; i < e; ++i) {
-    const uint32_t k = arc[i];
-    // mixed precision reads only the FP32 flow of a node-level arc: its
-    // FP64 sum and its fixed-point value (the one the sweeps use) both
-    // come from it
-    const double f = (mixed && !wx) ? (double)__uint_as_float(k) : w[k];
-    sum += f;
-    sx += wx ? wx[k] : fx_of(f);
+  // kRowAhead elements in flight, added in order: a long segment (two large
+  // super nodes joined by many arcs) does not cost one round trip per arc
+  for (uint32_t i0 = b; i0 < e; i0 += kRowAhead) {
+    uint32_t k[kRowAhead];
+    double f[kRowAhead];
+    long long fx[kRowAhead];
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) k[j] = i0 + j < e ? arc[i0 + j] : 0u;
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) {
+      // mixed precision reads only the FP32 flow of a node-level arc: its
+      // FP64 sum and its fixed-point value (the one the sweeps use) both
+      // come from it
+      f[j] = i0 + j < e ? ((mixed && !wx) ? (double)__uint_as_float(k[j]) : w[k[j]]) : 0.0;
+      fx[j] = (wx && i0 + j < e) ? wx[k[j]] : 0ll;
+    }
+#pragma unroll
+    for (int j = 0; j < kRowAhead; ++j) {
+      if (i0 + j >= e) break;
+      sum += f[j];
+      sx += wx ? fx[j] : fx_of(f[j]);
+    }
   }
   const uint64_t key = keys[b];
   const uint64_t mask = (1ull << bits) - 1ull;
