@@ -226,6 +226,13 @@ struct PushBins {
   bool fp64 = false;   // FP64 products (val64) instead of FP32 (val)
   DevBuf<uint32_t> bptr, slot, trow;
   DevBuf<uint16_t> bdst;
+  // FP32 products staged by tile (flow.hip, k_pb_push_staged): tiles of
+  // kStageTile arcs, each arc's position in its tile's bin-major order, the
+  // slot of every such position, and the row of every 8th arc
+  bool staged = false;
+  int64_t stiles = 0;
+  DevBuf<uint16_t> lpos;
+  DevBuf<uint32_t> gslot, grow;
   DevBuf<float> val;
   DevBuf<double> val64;
   void alloc_values(int64_t m) {

@@ -433,6 +433,149 @@ __global__ __launch_bounds__(256) void k_pb_push(const uint32_t *__restrict__ op
   }
 }
 
+// The staged push's layout: tiles of kStageTile arcs.  Inside a tile the
+// arcs are given positions in bin-major order (the tile's arcs of bin b take
+// positions lbase(b) .. lbase(b) + count(t, b) - 1, lbase the tile's
+// exclusive scan of its counts), and position p of tile t is tied to slot
+// gslot[t kStageTile + p] = base(t, b) + (p - lbase(b)): the tile's run of
+// slots in bin b.  Which arc of a run takes which position follows LDS
+// atomic order; lpos[k] and gslot agree on it, and the pull's fixed-point
+// sums do not depend on it.
+constexpr int kStageTile = 16384;   // arcs per staged tile (64 KB of FP32 products in LDS)
+constexpr int kStageBlock = 1024;
+
+__global__ __launch_bounds__(kLayoutBlock) void k_pb_place_staged(const uint32_t *__restrict__ oidx, int64_t m,
+                                                                  int32_t nbins, int rshift,
+                                                                  const uint32_t *__restrict__ C,
+                                                                  const uint32_t *__restrict__ Ccnt,
+                                                                  uint16_t *__restrict__ lpos,
+                                                                  uint32_t *__restrict__ gslot,
+                                                                  uint16_t *__restrict__ bdst) {
+  __shared__ uint32_t lcur[kLayoutMaxBins];   // next position of each bin in the tile
+  __shared__ uint32_t goff[kLayoutMaxBins];   // slot of position p in bin b: goff[b] + p
+  __shared__ uint32_t s_scan[kLayoutBlock / kWave];
+  const int64_t tile = blockIdx.x;
+  const uint32_t *cnt = Ccnt + tile * nbins, *base = C + tile * nbins;
+  // exclusive scan of the tile's counts, kLayoutMaxBins / kLayoutBlock bins per lane
+  constexpr int PER = kLayoutMaxBins / kLayoutBlock;
+  uint32_t loc = 0;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = threadIdx.x * PER + q;
+    loc += b < nbins ? cnt[b] : 0u;
+  }
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  uint32_t x = loc;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, kWave);
+    if (lane >= off) x += y;
+  }
+  if (lane == kWave - 1) s_scan[w] = x;
+  __syncthreads();
+  uint32_t run = x - loc;
+  for (int i = 0; i < w; ++i) run += s_scan[i];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = threadIdx.x * PER + q;
+    if (b < nbins) {
+      lcur[b] = run;
+      goff[b] = base[b] - run;   // modulo 2^32, as the slots
+      run += cnt[b];
+    }
+  }
+  __syncthreads();
+  const uint32_t mask = (1u << rshift) - 1u;
+  const int64_t a0 = tile * kStageTile, a1 = std::min<int64_t>(m, a0 + kStageTile);
+  for (int64_t k0 = a0 + threadIdx.x; k0 < a1; k0 += 4 * kLayoutBlock) {
+    uint32_t d[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t k = k0 + j * kLayoutBlock;
+      d[j] = k < a1 ? oidx[k] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t k = k0 + j * kLayoutBlock;
+      if (k < a1) {
+        const uint32_t b = d[j] >> rshift;
+        const uint32_t p = atomicAdd(&lcur[b], 1u);
+        const uint32_t sl = goff[b] + p;
+        lpos[k] = (uint16_t)p;
+        gslot[a0 + p] = sl;
+        bdst[sl] = (uint16_t)(d[j] & mask);
+      }
+    }
+  }
+}
+
+// grow[g] = the row of arc 8 g (one wavefront per row)
+__global__ void k_pb_group_rows(const uint32_t *__restrict__ optr, int32_t n, uint32_t *__restrict__ grow) {
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (r >= n) return;
+  const uint32_t g0 = (optr[r] + 7) / 8, g1 = (optr[r + 1] + 7) / 8;
+  for (uint32_t g = g0 + lane; g < g1; g += kWave) grow[g] = (uint32_t)r;
+}
+
+// Staged push (FP32 products): one workgroup per tile of kStageTile arcs
+// computes every product x_tail w into the tile's bin-major order in LDS,
+// then writes the tile's runs of slots with consecutive lanes, so that the
+// stores of a run leave as whole segments instead of one request per arc.
+// The products are those of the plain push, so the pull's sums are the same
+// bit for bit.  Tiles go to the XCDs in contiguous eighths.
+__global__ __launch_bounds__(kStageBlock) void k_pb_push_staged(const uint32_t *__restrict__ optr,
+                                                                const uint32_t *__restrict__ grow, int64_t ntl,
+                                                                int64_t per_xcd, int64_t m,
+                                                                const uint16_t *__restrict__ lpos,
+                                                                const uint32_t *__restrict__ gslot,
+                                                                const float *__restrict__ wf,
+                                                                const float *__restrict__ xf,
+                                                                float *__restrict__ val) {
+  __shared__ float stage[kStageTile];
+  const int64_t tile = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (tile >= ntl) return;
+  const int64_t a0 = tile * kStageTile;
+  const int64_t a1 = std::min<int64_t>(m, a0 + kStageTile);
+  for (int64_t k0 = a0 + 8 * (int64_t)threadIdx.x; k0 < a1; k0 += 8 * kStageBlock) {
+    uint32_t row = grow[k0 / 8];   // k0 is a multiple of 8
+    uint32_t nxt = optr[row + 1];
+    float xr = xf[row];
+    if (k0 + 8 <= a1) {
+      const u32x4 lp = stream_u4((const uint32_t *)(lpos + k0));   // 8 positions of 16 bits
+      const f32x4 w0 = stream_f4(wf + k0), w1 = stream_f4(wf + k0 + 4);
+      const uint32_t pw[4] = {lp.x, lp.y, lp.z, lp.w};
+      const float ww[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t k = (uint32_t)(k0 + j);
+        if (k >= nxt) {
+          do {
+            ++row;
+            nxt = optr[row + 1];
+          } while (k >= nxt);
+          xr = xf[row];
+        }
+        stage[(pw[j >> 1] >> (16 * (j & 1))) & 0xffffu] = xr * ww[j];
+      }
+    } else {
+      for (int64_t k = k0; k < a1; ++k) {
+        if ((uint32_t)k >= nxt) {
+          do {
+            ++row;
+            nxt = optr[row + 1];
+          } while ((uint32_t)k >= nxt);
+          xr = xf[row];
+        }
+        stage[lpos[k]] = xr * wf[k];
+      }
+    }
+  }
+  __syncthreads();
+  const int n_t = (int)(a1 - a0);
+  for (int p = threadIdx.x; p < n_t; p += kStageBlock) val[gslot[a0 + p]] = stage[p];
+}
+
 // Pull: one workgroup per bin of R rows
 __global__ __launch_bounds__(kPullBlock) void k_pb_pull(const uint32_t *__restrict__ bptr,
                                                         const uint16_t *__restrict__ bdst,
@@ -970,12 +1113,16 @@ void spmv_all(Ctx &c, const PullCSR &L, const RowBins &rb, double *y) {
 // Rows per bin: a power of two from 256 to kPullRowsMax (kPullRowsMax64 for
 // FP64 products), about n / 2048, so that a large network gives thousands of
 // bins (workgroups of the pull).  The slots come from the per-tile counts
-// above when the bins fit the LDS counters and the level has its in-CSR,
-// else from a stable sort of the out-arcs by bin.
+// above when the bins fit the LDS counters and the level has its in-CSR
+// (FP32 products: in the staged push's tiles), else from a stable sort of
+// the out-arcs by bin.  A network without arcs has no push tiles.
 void build_push_bins(Ctx &c, const Level &L, PushBins &pb) {
   hipStream_t s = c.stream;
   const int32_t n = L.n;
   const int64_t m = L.m;
+  pb.staged = false;
+  pb.stiles = 0;
+  pb.ntiles = 0;
   const int rows_max = pb.fp64 ? kPullRowsMax64 : kPullRowsMax;
   int rs = 8;
   while ((1 << rs) < rows_max && ((int64_t)1 << rs) * 2048 < n) ++rs;
@@ -986,6 +1133,39 @@ void build_push_bins(Ctx &c, const Level &L, PushBins &pb) {
   if (m == 0) {
     fill_u32(pb.bptr.p, 0u, (int64_t)pb.nbins + 1, s);
     return;
+  }
+  if (!pb.fp64 && pb.nbins <= kLayoutMaxBins && L.in_ptr.n > n) {
+    // FP32 products: the staged push's layout (tiles of kStageTile arcs)
+    const int64_t ntl = (m + kStageTile - 1) / kStageTile;
+    const int64_t nq = (ntl + kLayoutChunk - 1) / kLayoutChunk;
+    k_pb_bptr<<<grid_for((int64_t)pb.nbins + 1, 256), 256, 0, s>>>(L.in_ptr.p, n, pb.nbins, rs, pb.bptr.p);
+    RMX_LAUNCH_CHECK();
+    DevBuf<uint32_t> C(c.cache), Ccnt(c.cache), P(c.cache);
+    C.alloc(ntl * pb.nbins);
+    Ccnt.alloc(ntl * pb.nbins);
+    P.alloc(nq * pb.nbins);
+    k_pb_count<<<(unsigned)ntl, kLayoutBlock, 0, s>>>(L.out_idx.p, m, kStageTile, pb.nbins, rs, C.p);
+    RMX_LAUNCH_CHECK();
+    RMX_HIP(hipMemcpyAsync(Ccnt.p, C.p, sizeof(uint32_t) * ntl * pb.nbins, hipMemcpyDeviceToDevice, s));
+    k_pb_colsum<<<grid_for(nq * pb.nbins, 256), 256, 0, s>>>(C.p, ntl, pb.nbins, nq, P.p);
+    RMX_LAUNCH_CHECK();
+    k_pb_colscan<<<grid_for(pb.nbins, 256), 256, 0, s>>>(pb.bptr.p, pb.nbins, nq, P.p);
+    RMX_LAUNCH_CHECK();
+    k_pb_colbase<<<grid_for(nq * pb.nbins, 256), 256, 0, s>>>(P.p, ntl, pb.nbins, nq, C.p);
+    RMX_LAUNCH_CHECK();
+    pb.lpos.alloc(m);
+    pb.gslot.alloc(m);
+    pb.bdst.alloc(m);
+    pb.grow.alloc((m + 7) / 8);
+    pb.alloc_values(m);
+    k_pb_place_staged<<<(unsigned)ntl, kLayoutBlock, 0, s>>>(L.out_idx.p, m, pb.nbins, rs, C.p, Ccnt.p, pb.lpos.p,
+                                                             pb.gslot.p, pb.bdst.p);
+    RMX_LAUNCH_CHECK();
+    k_pb_group_rows<<<grid_for((int64_t)n * kWave, 256), 256, 0, s>>>(L.out_ptr.p, n, pb.grow.p);
+    RMX_LAUNCH_CHECK();
+    pb.staged = true;
+    pb.stiles = ntl;
+    return;   // C, Ccnt and P go back to the cache: later work on this stream runs after the layout
   }
   pb.ntiles = (m + kPushTile - 1) / kPushTile;
   pb.trow.alloc(pb.ntiles + 1);
@@ -1061,7 +1241,17 @@ void push_pull(Ctx &c, const PushBins &pb, const Level &L, const PullCSR &F, dou
     if (c.prof.on) c.prof.end(s, "k_pb_pull64", (double)L.m * 10.0 + (double)L.n * (F.rinst ? 20.0 : 16.0));
     return;
   }
-  if (pb.ntiles > 0) {
+  if (pb.staged && pb.stiles > 0) {
+    // per arc its position, FP32 weight, the position's slot and the product (14 B),
+    // per 8 arcs a row (0.5 B), per row its bound and x (8)
+    const int64_t per = (pb.stiles + 7) / 8;
+    if (c.prof.on) c.prof.begin(s);
+    k_pb_push_staged<<<(unsigned)(8 * per), kStageBlock, 0, s>>>(L.out_ptr.p, pb.grow.p, pb.stiles, per, L.m,
+                                                                  pb.lpos.p, pb.gslot.p, L.out_wf.p, c.xf.p,
+                                                                  pb.val.p);
+    RMX_LAUNCH_CHECK();
+    if (c.prof.on) c.prof.end(s, "k_pb_push_staged", (double)L.m * 14.5 + (double)L.n * 8.0);
+  } else if (pb.ntiles > 0) {
     const int64_t per = (pb.ntiles + 7) / 8;
     if (c.prof.on) c.prof.begin(s);
     k_pb_push<<<(unsigned)(8 * per), 256, 0, s>>>(L.out_ptr.p, pb.trow.p, pb.ntiles, per, L.m, pb.slot.p,
