@@ -8,7 +8,9 @@ blocked path against the FP64 in-CSR pull.
 - fewer than 8 arcs in a tile (the scalar tail);
 - arc counts that are whole multiples of the tile;
 - many bins, and a hub that every node points to (one LDS accumulator
-  takes a whole bin's arcs).
+  takes a whole bin's arcs);
+- a context that uploads another network, one without arcs, and the first
+  again: each gets its own layout.
 """
 import numpy as np
 import pytest
@@ -88,6 +90,35 @@ def test_blocked_flows_deterministic(prec):
             net.initiate()
             runs.append(net.flows())
     assert np.array_equal(runs[0], runs[1])
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_layout_follows_each_upload(prec):
+    """A context keeps its push / pull layout from one steady state to the
+    next, until the next upload.  A network uploaded after another, in the
+    same context, gets its own layout: one without arcs (every node dangling,
+    so p is the uniform teleport distribution) and then the first one again
+    (the same flows as in a fresh context)."""
+    from relaxmap_amd import Network
+    n, s, d, w = _graphs()["hub_bins"]
+    ptr, idx, ww, _ = _csr(n, s, d, w)
+    with Network(precision=prec) as fresh:
+        fresh.upload(n, ptr, idx, ww)
+        fresh.initiate()
+        p_ref = fresh.flows()
+    with Network(precision=prec) as net:
+        net.upload(n, ptr, idx, ww)
+        net.initiate()
+        n2 = 5000
+        net.upload(n2, np.zeros(n2 + 1, np.int64), np.zeros(0, np.int32), np.zeros(0))
+        net.initiate()
+        p0 = net.flows()
+        net.upload(n, ptr, idx, ww)
+        net.initiate()
+        p1 = net.flows()
+    print(f"[gpu] re-upload {prec}: arc-less L1 {np.abs(p0 - 1.0 / n2).sum():.2e}", flush=True)
+    assert np.abs(p0 - 1.0 / n2).sum() <= 1e-12
+    assert np.array_equal(p1, p_ref)
 
 
 @pytest.mark.parametrize("key", ["lfr1k_mu03", "lfr5k_mu05", "sbm1k", "flow_net"])
