@@ -72,6 +72,15 @@ int rmx_set_hub_path(rmx_ctx *ctx, int global_tables);
  * identical results.  No reference counterpart. */
 int rmx_set_graph_capture(rmx_ctx *ctx, int on);
 
+/* The move evaluation's candidate bound for degree bucket `bucket` (0-6 the
+ * buckets of rmx_set_bucket_cutoffs, 7 the hub partitions, -1 all): a lower
+ * bound of a candidate's dL from the unit's flow to it alone drops the
+ * candidates that cannot be a move before their module statistics are read.
+ * 1 on, 0 off; the default is on for units of more than 32 arcs (buckets
+ * 2-7; those of buckets 0-1 are evaluated unbounded in any case).  Both give
+ * identical moves.  No reference counterpart. */
+int rmx_set_candidate_bound(rmx_ctx *ctx, int bucket, int mode);
+
 /* How rmx_partition runs attempts > 1 on one network: 0 (default) as one
  * batch of instances (see rmx_batch_upload), a copy of the network per
  * attempt with its own seed, when attempts x arcs <= 2^28 and attempts x nodes

@@ -75,6 +75,7 @@ def load_library():
         "rmx_set_hub_path": (C.c_int, [vp, C.c_int]),
         "rmx_set_fp64_flow": (C.c_int, [vp, C.c_int]),
         "rmx_set_graph_capture": (C.c_int, [vp, C.c_int]),
+        "rmx_set_candidate_bound": (C.c_int, [vp, C.c_int, C.c_int]),
         "rmx_set_attempt_mode": (C.c_int, [vp, C.c_int]),
         "rmx_net_read": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(vp), C.c_char_p, C.c_int]),
         "rmx_net_free": (None, [vp]),
@@ -237,6 +238,12 @@ class Network:
         captured HIP graph each (True) or launch by launch with a host wait
         per batch (False, the default and the faster); same results."""
         self._check(self._L.rmx_set_graph_capture(self._h, 1 if on else 0))
+
+    def set_candidate_bound(self, mode, bucket=-1):
+        """Candidate bound of the move evaluation on (1, the default for
+        units of more than 32 arcs) or off (0) for one degree bucket (0-6, 7
+        the hub partitions) or all (-1); same moves either way."""
+        self._check(self._L.rmx_set_candidate_bound(self._h, int(bucket), int(mode)))
 
     def close(self):
         if getattr(self, "_h", None):

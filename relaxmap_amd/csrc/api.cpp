@@ -180,6 +180,16 @@ int rmx_set_graph_capture(rmx_ctx *ctx, int on) {
   });
 }
 
+int rmx_set_candidate_bound(rmx_ctx *ctx, int bucket, int mode) {
+  return guarded(ctx, [&] {
+    if (bucket < -1 || bucket > 7) return fail(ctx, RMX_EINVAL, "candidate bound: bucket must be -1..7");
+    if (mode != 0 && mode != 1) return fail(ctx, RMX_EINVAL, "candidate bound: mode must be 0 or 1");
+    for (int b = 0; b < 8; ++b)
+      if (bucket < 0 || bucket == b) ctx->c.bound_mode[b] = mode;
+    return RMX_OK;
+  });
+}
+
 }  // extern "C"
 
 namespace {
@@ -977,6 +987,7 @@ int batched_attempts(rmx_ctx *ctx, const rmx_partition_opts &o, rmx_partition_re
   t->c.cuts = c.cuts;
   t->c.hub_global = c.hub_global;
   t->c.use_graph = c.use_graph;
+  for (int b = 0; b < 8; ++b) t->c.bound_mode[b] = c.bound_mode[b];
   auto check = [&](int rc, const char *what) {
     if (rc != RMX_OK) throw HipError(std::string("attempts: ") + what + ": " + rmx_last_error(t));
   };

@@ -254,6 +254,10 @@ struct Ctx {
   int fp64_blocked = 1;   // FP64 steady state: 1 blocked push / pull, 0 the in-CSR pull
   int use_graph = 0;   // sweeps of a batch or a sub-module partition as one captured graph
   int attempts_sequential = 0;   // 1: attempts one after another even where a batch of copies fits
+  // candidate bound of the evaluation kernels per degree bucket (7: the hub
+  // partitions): 1 on, 0 off (units of up to 32 arcs have no bound code:
+  // one round of candidate reads, nothing to gain)
+  int bound_mode[8] = {0, 0, 1, 1, 1, 1, 1, 1};
   BucketCuts cuts = {{kBucketMaxDeg[0], kBucketMaxDeg[1], kBucketMaxDeg[2], kBucketMaxDeg[3], kBucketMaxDeg[4],
                       kBucketMaxDeg[5], kBucketMaxDeg[6], kBucketMaxDeg[7]}};
   bool initiated = false;
@@ -268,6 +272,7 @@ struct Ctx {
 
   // module SoA, N slots (module ids are node ids, as in the reference)
   DevBuf<Stat4> ms;                     // fixed-point sums: flow, teleport, dangling, exit arcs
+  DevBuf<unsigned long long> smax;      // during a sweep: upper bounds of every module's T and P (fixed point)
   DevBuf<double> qv;                    // exit probability of every module (0 when empty), during a sweep
   DevBuf<int32_t> mNodes, mUnits, mStale;  // original nodes, units at the current level, stale count
   // one bit per module: it holds exactly one unit (mUnits == 1), rebuilt at

@@ -106,6 +106,11 @@ struct EvalArgs {
   // candidate and its dL, written before the mover test (else null)
   uint32_t *best_m;
   double *best_d;
+  // candidate bound: smax[0] / smax[1] are upper bounds of every module's
+  // teleport weight T and flow P (fixed point) at the batch start; bound is
+  // on (1) or off (0) for this launch
+  const unsigned long long *smax;
+  int bound;
 };
 
 // Every unit draws its batch independently.  (Drawing by blocks of
@@ -460,6 +465,112 @@ __device__ inline void eval_cands(const EvalArgs &a, const UnitCtx &U, const uin
   }
 }
 
+// ------------------------------------------------------- candidate bound
+// A lower bound of a candidate's dL from the unit's flow f to it alone, so
+// that a candidate that cannot beat a threshold is dropped before its module
+// row is read.  With X = q_m, Y = q_m + P_m and d = q_u - f - tele_m,
+//   dL(m) = A(d) + F(X, Y, d) + Kr,   A(d) = phi(S + d1 + d) - phi(S),
+//   F = -2 [phi(X + d) - phi(X)] + [phi(Y + d + p) - phi(Y)].
+// Every module's exit is at most its flow (q_m <= P_m, so Y >= 2X) and a
+// unit's exit at most its flow (q_u <= p, so d <= p).  F grows with Y when
+// d + p >= 0 and F(X, 2X, d) grows with X when p >= d, so over all modules
+//   F >= phi(d + p) - 2 phi(d)                      (d >= 0: X = Y = 0)
+//   F >= 2 phi(-d) + phi(p - d) - phi(-2d)          (d < 0: X = -d, Y = -2d).
+// A + F_min grows with d: for d >= 0 its slope is ln((S + d1 + d)(d + p) / d^2)
+// with S + d1 >= 0, and for d < 0 it is ln(4 (S + d1 + d) / (p - d)), which
+// is checked at the lowest d.  The lowest d comes from the largest
+// teleportation flow between the unit and any module,
+//   tele_m <= tele_out T_max + P_max tau_u   (alpha P_m + beta D_m <= P_m).
+// A candidate is dropped when bound - margin > threshold, the margin covering
+// the rounding of the evaluated dL (FP32 terms in mixed precision): it then
+// evaluates above the threshold, so it can be neither the best nor a move.
+struct BoundCtx {
+  double p, tm;
+  bool on;
+};
+
+__device__ inline BoundCtx bound_ctx(const EvalArgs &a, const UnitCtx &U) {
+  BoundCtx B;
+  B.p = fx_to(U.p);
+  const double tmax = fx_to((long long)a.smax[0]), pmax = fx_to((long long)a.smax[1]);
+  B.tm = U.tele_out * tmax + pmax * U.td;
+  B.on = U.qu <= B.p * (1.0 + 1e-12) && U.S > 0.0;
+  return B;
+}
+
+// true: the candidate with flows fo + fi (fixed point) evaluates above thr.
+// The bound's terms are evaluated in FP32 in difference form,
+//   A = e ln(S + e) + S log1p(e / S) with e = d1 + d,
+//   F_min = (d + p) ln(d + p) - 2 d ln d          (d >= 0)
+//         = (p - d) ln(p - d) + 2 d ln 2          (d < 0),
+// with a margin of 1e-3 of the terms' scale: far above the FP32 rounding of
+// both the bound and the evaluated dL (about 1e-6 of that scale).
+__device__ inline bool bounded_out(const UnitCtx &U, const BoundCtx &B, long long fsum, double thr) {
+  const double dhi = U.qu - kBeta * fx_to(fsum);
+  const double dlo = dhi - B.tm;
+  const double p = B.p;
+  if (!(dlo + p >= 0.0)) return false;
+  const double e = U.d1 + dlo;
+  if (!(dlo >= 0.0) && !(4.0 * (U.S + e) >= p - dlo)) return false;
+  const float S = (float)U.S, ef = (float)e, y = (float)(U.S + e);
+  const float A = y > 0.f ? ef * __logf(y) + S * log1pf(ef / S) : -S * __logf(S);
+  const float d = (float)dlo, pf = (float)p;
+  float F;
+  if (dlo >= 0.0) F = (d + pf) * __logf(d + pf) - (d > 0.f ? 2.f * d * __logf(d) : 0.f);
+  else F = (pf - d) * __logf(pf - d) + 1.3862944f * d;
+  const double scale = fabs(U.d1) + 4.0 * fmax(fabs(dlo), fabs(dhi)) + p;
+  return (double)(A + F) + U.Kr - (1e-3 * scale + 1e-11) > thr;
+}
+
+// eval_cands over a group of G lanes (G <= 64, inside one wavefront; gmask
+// its lanes).  With BOUND and a.bound set, the candidates whose bound lies
+// above dL = 0 (so that they cannot be a move), those of the unit's own
+// module and those the singleton rule excludes are first dropped, the rest
+// compacted in place to the front, in order (a slot only moves down and
+// every lane of the group reads before any lane writes), and evaluated.
+// *rows: the module rows read.  (A tighter threshold, the dL of the unit's
+// largest-flow candidate evaluated first or of the first round of
+// candidates, drops more rows but made the kernels slower: they wait on
+// chains of dependent reads, and the first costs one more per unit.)
+template <bool BOUND, int UC, int G, bool F32, typename Slot>
+__device__ inline void eval_cands_bounded(const EvalArgs &a, const UnitCtx &U, uint32_t *keys, long long *fo,
+                                          long long *fi, int lane, int n, uint64_t gmask, Slot slot, Best &b,
+                                          int *rows) {
+  int cnt = n;
+  if (BOUND && a.bound != 0 && n > 1) {
+    const BoundCtx B = bound_ctx(a, U);
+    cnt = 0;
+    for (int j0 = 0; j0 < n; j0 += G) {
+      const int j = j0 + lane;
+      uint32_t key = kEmptyKey;
+      long long vo = 0, vi = 0;
+      bool keep = false;
+      if (j < n) {
+        const int s = slot(j);
+        key = keys[s];
+        vo = fo[s];
+        vi = fi[s];
+        keep = !(key == U.o || (U.single && key > U.o && one_unit(a, key)));
+        if (keep && B.on) keep = !bounded_out(U, B, vo + vi, 0.0);
+      }
+      const uint64_t bal = __ballot(keep) & gmask;
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (keep) {
+        const int d = slot(cnt + (int)__popcll(bal & lanes_below()));
+        keys[d] = key;
+        fo[d] = vo;
+        fi[d] = vi;
+      }
+      cnt += (int)__popcll(bal);
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  eval_cands<UC, G, F32>(a, U, keys, fo, fi, lane, cnt, slot, b);
+  *rows = cnt;
+}
+
 // G lanes per unit (G <= 64: the group lives inside one wavefront), H keys.
 // After the flows are aggregated, the non-empty slots are compacted in place
 // to the front of the table (a slot only ever moves down, and every lane
@@ -529,10 +640,12 @@ __global__ __launch_bounds__(256) void k_eval_group(EvalArgs a) {
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
-    if (lane == 0) ncand = (unsigned long long)nc;
     UnitCtx U;
     unit_ctx(a, ul, fo_o, fi_o, U);
-    eval_cands<(G >= 32 ? 2 : 1), G, F32>(a, U, keys, fo, fi, lane, nc, [](int j) { return j; }, b);
+    int rows = 0;
+    eval_cands_bounded<(G >= 32), (G >= 32 ? 2 : 1), G, F32>(a, U, keys, fo, fi, lane, nc, gmask,
+                                                             [](int j) { return j; }, b, &rows);
+    if (lane == 0) ncand = (unsigned long long)rows;
     if (lane == 0 && U.offer_empty) {
       const double dl = delta_empty(U);
       if (better(dl, a.empty_key, b) && dl < 0.0) { b.d = dl; b.m = a.empty_key; b.i = -1; }
@@ -646,13 +759,15 @@ __global__ __launch_bounds__(256) void k_eval_block(EvalArgs a) {
     UnitCtx U;
     unit_ctx(a, ul, fo_o, fi_o, U);
     Best b{0.0, kNone, -1};
-    eval_cands<2, kWave, F32>(a, U, keys, fo, fi, wl, nw, [w](int j) { return wave_slot(j, w); }, b);
+    int rows = 0;
+    eval_cands_bounded<true, 2, kWave, F32>(a, U, keys, fo, fi, wl, nw, ~0ull,
+                                            [w](int j) { return wave_slot(j, w); }, b, &rows);
     if (threadIdx.x == 0 && U.offer_empty) {
       const double dl = delta_empty(U);
       if (better(dl, a.empty_key, b) && dl < 0.0) { b.d = dl; b.m = a.empty_key; b.i = -1; }
     }
     b = group_best<kWave>(b);
-    const unsigned long long nc_w = (unsigned long long)nw;
+    const unsigned long long nc_w = (unsigned long long)rows;
     __syncthreads();   // keys are not read again: the per-wave bests reuse them
     double *r_bd = (double *)s_keys;                      // NW doubles
     uint32_t *r_bm = s_keys + 2 * NW;                     // NW ids
@@ -1276,28 +1391,33 @@ __global__ __launch_bounds__(256) void k_hubp_eval(EvalArgs a, HubParts H) {
     UnitCtx U;
     unit_ctx(a, ul, H.own[2 * i], H.own[2 * i + 1], U);
     Best b{0.0, kNone, -1};
-    eval_cands<4, kWave, F32>(a, U, s_keys, s_out, s_in, wl, nw, [w](int j) { return wave_slot(j, w); }, b);
+    int rows = 0;
+    eval_cands_bounded<true, 4, kWave, F32>(a, U, s_keys, s_out, s_in, wl, nw, ~0ull,
+                                            [w](int j) { return wave_slot(j, w); }, b, &rows);
     b = group_best<kWave>(b);
     __syncthreads();   // keys are not read again: the per-wave bests reuse them
     double *r_bd = (double *)s_keys;
     uint32_t *r_bm = s_keys + 2 * NW;
     int *r_bi = (int *)(s_keys + 3 * NW);
-    unsigned long long *r_nc = (unsigned long long *)(s_keys + 4 * NW);
+    unsigned long long *r_nc = (unsigned long long *)(s_keys + 4 * NW);   // table slots in use
+    unsigned long long *r_rows = (unsigned long long *)(s_keys + 6 * NW);   // module rows read
     if (wl == 0) {
       r_bd[w] = b.d;
       r_bm[w] = b.m;
       r_bi[w] = b.i;
       r_nc[w] = (unsigned long long)nw;
+      r_rows[w] = (unsigned long long)rows;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
       Best bb{r_bd[0], r_bm[0], r_bi[0]};
       unsigned long long nc = r_nc[0];
+      cand_acc += r_rows[0];
       for (int q = 1; q < NW; ++q) {
         nc += r_nc[q];
+        cand_acc += r_rows[q];
         if (better(r_bd[q], r_bm[q], bb)) { bb.d = r_bd[q]; bb.m = r_bm[q]; bb.i = r_bi[q]; }
       }
-      cand_acc += nc;
       if (nc >= (unsigned long long)kHubTable || H.force_ovf) {
         // the table filled up (an insert may have failed): the hub goes to the fallback
         H.ovf[i] = 1u;
@@ -1487,7 +1607,8 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
                                                 const int32_t *__restrict__ nmem, Stat4 *ms, int32_t *mNodes,
                                                 int32_t *mUnits, uint8_t *act_next, int mark, uint32_t *stripes,
                                                 uint32_t big_deg, uint32_t *big_list, uint32_t *big_cnt,
-                                                uint32_t pull_min, uint8_t *wmark, uint32_t *pull_flag) {
+                                                uint32_t pull_min, uint8_t *wmark, uint32_t *pull_flag,
+                                                unsigned long long *smax) {
   constexpr int GPB = 256 / G;   // movers per workgroup and round
   // Each round takes GPB movers; their module-row updates (old and new module
   // of each) are merged by module in LDS before the atomics, so that movers
@@ -1501,6 +1622,7 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
   const int lane = threadIdx.x % G;
   const int gl = threadIdx.x / G;
   uint32_t moved_here = 0;
+  unsigned long long grown[2] = {0, 0};   // largest flow and teleport weight a module row reached here
   for (int64_t r0 = (int64_t)blockIdx.x * GPB; r0 < nm; r0 += (int64_t)gridDim.x * GPB) {
     const int64_t i = r0 + gl;
     bool live = i < nm;   // this group's mover commits a row update this round
@@ -1585,8 +1707,11 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
         }
         unsigned long long *row = (unsigned long long *)&ms[m];
 #pragma unroll
-        for (int f = 0; f < 4; ++f)
-          if (dv[f]) atomicAdd(row + f, (unsigned long long)dv[f]);
+        for (int f = 0; f < 4; ++f) {
+          if (!dv[f]) continue;
+          const unsigned long long old = atomicAdd(row + f, (unsigned long long)dv[f]);
+          if (f < 2 && dv[f] > 0) grown[f] = max(grown[f], old + (unsigned long long)dv[f]);
+        }
         if (dn) atomicAdd(&mNodes[m], dn);
         if (du) atomicAdd(&mUnits[m], du);
       }
@@ -1595,6 +1720,17 @@ __global__ __launch_bounds__(256) void k_commit(const uint32_t *__restrict__ mov
   }
   moved_here = wave_sum(moved_here);
   if ((threadIdx.x & (kWave - 1)) == 0 && moved_here) atomicAdd(my_stripe(stripes) + cMoved, moved_here);
+  // the candidate bound's module maxima (one atomic per wavefront at most)
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    grown[0] = max(grown[0], (unsigned long long)__shfl_xor(grown[0], off, kWave));
+    grown[1] = max(grown[1], (unsigned long long)__shfl_xor(grown[1], off, kWave));
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    // Stat4 holds p then t; smax holds T then P
+    if (grown[1] > smax[0]) atomicMax(&smax[0], grown[1]);
+    if (grown[0] > smax[1]) atomicMax(&smax[1], grown[0]);
+  }
 }
 
 // Profiling only: arcs of a batch's movers, split at big_deg: out-arcs and
@@ -1970,6 +2106,47 @@ void launch_bucket(int bk, const EvalArgs &a, int32_t cnt, bool f32, hipStream_t
 }
 
 // Arguments of the evaluation kernels for one sweep of level L.
+// Largest teleport weight and flow of any module row (fixed point), into
+// smax[0] and smax[1] (zeroed by the caller).
+__global__ __launch_bounds__(256) void k_stat_max(const Stat4 *__restrict__ ms, int64_t n,
+                                                  unsigned long long *smax) {
+  __shared__ unsigned long long red[2][256 / kWave];
+  unsigned long long tm = 0, pm = 0;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i0 = (int64_t)blockIdx.x * 256 + threadIdx.x; i0 < n; i0 += 4 * stride) {
+    long long t[4], p[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t i = i0 + q * stride;
+      t[q] = i < n ? ms[i].t : 0;
+      p[q] = i < n ? ms[i].p : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      tm = max(tm, (unsigned long long)t[q]);
+      pm = max(pm, (unsigned long long)p[q]);
+    }
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    tm = max(tm, (unsigned long long)__shfl_xor(tm, off, kWave));
+    pm = max(pm, (unsigned long long)__shfl_xor(pm, off, kWave));
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    red[0][threadIdx.x / kWave] = tm;
+    red[1][threadIdx.x / kWave] = pm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 256 / kWave; ++w) {
+      tm = max(tm, red[0][w]);
+      pm = max(pm, red[1][w]);
+    }
+    if (tm) atomicMax(&smax[0], tm);
+    if (pm) atomicMax(&smax[1], pm);
+  }
+}
+
 EvalArgs sweep_args(Ctx &c, Level &L, bool prioritized, double vthresh, uint64_t sweep_seed, bool batch) {
   EvalArgs a{};
   a.bucket = L.bucket.p;
@@ -1995,6 +2172,17 @@ EvalArgs sweep_args(Ctx &c, Level &L, bool prioritized, double vthresh, uint64_t
     RMX_LAUNCH_CHECK();
     a.mOne = c.mOne.p;
     a.mMulti = node ? c.mMulti.p : nullptr;
+  }
+  {   // upper bounds of the module rows' T and P for the candidate bound;
+      // the commits raise them as modules grow
+    c.smax.alloc(2);
+    fill_i64((long long *)c.smax.p, 0, 2, c.stream);
+    if (c.ms.n > 0) {
+      k_stat_max<<<(unsigned)std::min<int64_t>(512, grid_for(c.ms.n, 256)), 256, 0, c.stream>>>(c.ms.p, c.ms.n,
+                                                                                              c.smax.p);
+      RMX_LAUNCH_CHECK();
+    }
+    a.smax = c.smax.p;
   }
   a.act = c.act.p;
   a.all_active = prioritized ? 0 : 1;
@@ -2335,6 +2523,7 @@ static uint32_t eval_batch(Ctx &c, Level &L, EvalArgs &a, const std::vector<uint
     a.list = c.batch_list.p + seg[key];
     a.list_n = cnt;
     bool hub_profiled = false;   // the partitioned hub kernels account themselves
+    a.bound = a.smax ? c.bound_mode[bk] : 0;
     if (c.prof.on) c.prof.begin(s);
     switch (bk) {
       case 0: case 1: case 2: case 3: case 4: case 5: case 6:
@@ -2539,7 +2728,7 @@ int64_t move_sweep(Ctx &c, bool prioritized, double vthresh, uint64_t seed, int6
         c.movers.p, c.counters.p, L.out_ptr.p, L.out_idx.p, L.in_ptr.p, L.in_idx.p, aout, L.mod.p, c.newmod.p,
         c.stamp.p, c.stamp_now, c.agg_buf.p, L.us.p, L.nmem.p, c.ms.p, c.mNodes.p, c.mUnits.p, c.act_next.p,
         prioritized ? 1 : 0, c.counters.p + kStripeBase, big_deg, c.big_list.p, c.counters.p + cBig, pull_min,
-        c.wmark.p, c.counters.p + cPull);
+        c.wmark.p, c.counters.p + cPull, c.smax.p);
     RMX_LAUNCH_CHECK();
     const bool pushed_marks = prioritized && n_movers < pull_min;
     if (c.prof.on) {
