@@ -319,7 +319,7 @@ struct Ctx {
   // partitioned hub evaluation: per batch hub arc, per partition, per hub
   DevBuf<uint32_t> hp_mtmp, hp_ekey;
   DevBuf<long long> hp_ew;
-  DevBuf<uint32_t> hp_bcnt, hp_boff, hp_bm;
+  DevBuf<uint32_t> hp_bcnt, hp_bcur, hp_boff, hp_bm;
   DevBuf<double> hp_bd;
   DevBuf<long long> hp_bfo, hp_bfi;
   DevBuf<uint32_t> hp_plan, hp_ovf, hp_hubu, hp_chub, hp_phub;

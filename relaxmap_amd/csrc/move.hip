@@ -1079,7 +1079,8 @@ struct HubParts {
   uint32_t *PB;     // partition prefix per hub (n_h + 1)
   uint32_t *AB;     // arc prefix per hub (n_h + 1)
   uint32_t *tot;    // [0] arc chunks, [1] partitions, [2] arcs
-  uint32_t *bcnt;   // entries per partition; then the scatter cursor
+  uint32_t *bcnt;   // entries per partition
+  uint32_t *bcur;   // the scatter's cursor per partition
   uint32_t *boff;   // first entry of each partition (scan of the counts)
   uint32_t *mtmp;   // module of each batch hub arc (kInBit: in-arc)
   uint32_t *ekey;   // entries grouped by partition (module | kInBit)
@@ -1109,44 +1110,54 @@ __device__ inline int lds_insert_bounded(uint32_t *keys, int H, uint32_t m) {
   return -1;   // table full
 }
 
-// one workgroup of 1024 lanes: the hub list is scanned in steps of 1024
+// One workgroup of 1024 lanes.  Lane t takes the contiguous hubs
+// [t k, t k + k), k = ceil(list_n / 1024): it reads their degrees (several
+// in flight) and parks them in AB, and one workgroup scan of the lanes'
+// totals gives every lane the prefix at its first hub.
 constexpr int kPlanBlock = 1024;
 __global__ __launch_bounds__(kPlanBlock) void k_hubp_plan(EvalArgs a, HubParts H) {
   __shared__ uint32_t lds[kPlanBlock / kWave];
-  uint32_t ca = 0, cp = 0, cb = 0;
+  const int n = a.list_n;
+  const int k = (n + kPlanBlock - 1) / kPlanBlock;
+  const int i0 = min(n, (int)threadIdx.x * k), i1 = min(n, i0 + k);
+  uint32_t ac = 0, pc = 0, dc = 0;
+  for (int b = i0; b < i1; b += 4) {
+    uint32_t u[4], deg[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) u[q] = b + q < i1 ? a.bucket[a.list[b + q]] : 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      deg[q] = b + q < i1 ? (a.optr[u[q] + 1] - a.optr[u[q]]) + (a.iptr[u[q] + 1] - a.iptr[u[q]]) : 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (b + q >= i1) break;
+      H.hubu[b + q] = u[q];
+      H.AB[b + q + 1] = deg[q];   // this lane reads it back below
+      ac += (deg[q] + kHubArcChunk - 1) / kHubArcChunk;
+      pc += (deg[q] + kHubPart - 1) / kHubPart;
+      dc += deg[q];
+    }
+  }
+  uint32_t ta, tp, tb;
+  uint32_t ca = block_incl_scan<kPlanBlock>(ac, lds, &ta) - ac;
+  uint32_t cp = block_incl_scan<kPlanBlock>(pc, lds, &tp) - pc;
+  uint32_t cb = block_incl_scan<kPlanBlock>(dc, lds, &tb) - dc;
   if (threadIdx.x == 0) {
     H.A[0] = 0;
     H.PB[0] = 0;
     H.AB[0] = 0;
+    H.tot[0] = ta;
+    H.tot[1] = tp;
+    H.tot[2] = tb;
   }
-  for (int base = 0; base < a.list_n; base += kPlanBlock) {
-    const int i = base + threadIdx.x;
-    uint32_t ac = 0, pc = 0, dc = 0;
-    if (i < a.list_n) {
-      const uint32_t u = a.bucket[a.list[i]];
-      H.hubu[i] = u;
-      const uint32_t deg = (a.optr[u + 1] - a.optr[u]) + (a.iptr[u + 1] - a.iptr[u]);
-      ac = (deg + kHubArcChunk - 1) / kHubArcChunk;
-      pc = (deg + kHubPart - 1) / kHubPart;
-      dc = deg;
-    }
-    uint32_t ta, tp, tb;
-    const uint32_t ia = block_incl_scan<kPlanBlock>(ac, lds, &ta);
-    const uint32_t ip = block_incl_scan<kPlanBlock>(pc, lds, &tp);
-    const uint32_t ib = block_incl_scan<kPlanBlock>(dc, lds, &tb);
-    if (i < a.list_n) {
-      H.A[i + 1] = ca + ia;
-      H.PB[i + 1] = cp + ip;
-      H.AB[i + 1] = cb + ib;
-    }
-    ca += ta;
-    cp += tp;
-    cb += tb;
-  }
-  if (threadIdx.x == 0) {
-    H.tot[0] = ca;
-    H.tot[1] = cp;
-    H.tot[2] = cb;
+  for (int i = i0; i < i1; ++i) {
+    const uint32_t deg = H.AB[i + 1];
+    ca += (deg + kHubArcChunk - 1) / kHubArcChunk;
+    cp += (deg + kHubPart - 1) / kHubPart;
+    cb += deg;
+    H.A[i + 1] = ca;
+    H.PB[i + 1] = cp;
+    H.AB[i + 1] = cb;
   }
 }
 
@@ -1157,7 +1168,16 @@ __global__ __launch_bounds__(256) void k_hubp_owners(EvalArgs a, HubParts H) {
   const int lane = threadIdx.x & (kWave - 1);
   if (i >= a.list_n) return;
   for (uint32_t c = H.A[i] + lane; c < H.A[i + 1]; c += kWave) H.chub[c] = (uint32_t)i;
-  for (uint32_t p = H.PB[i] + lane; p < H.PB[i + 1]; p += kWave) H.phub[p] = (uint32_t)i;
+  for (uint32_t p = H.PB[i] + lane; p < H.PB[i + 1]; p += kWave) {
+    H.phub[p] = (uint32_t)i;
+    H.bcnt[p] = 0;   // the hub's partitions start empty
+    H.bcur[p] = 0;
+  }
+  if (lane == 0) {
+    H.own[2 * i] = 0;
+    H.own[2 * i + 1] = 0;
+    H.ovf[i] = 0;
+  }
 }
 
 struct HubChunk {
@@ -1282,14 +1302,14 @@ __global__ __launch_bounds__(256) void k_hubp_scatter(EvalArgs a, HubParts H) {
         key[q] = H.mtmp[H.AB[h.i] + j];
         if (key[q] != kOwnArc) {
           part[q] = part_of(key[q] & ~kInBit, h.np);
-          rank[q] = local ? atomicAdd(&hcnt[part[q]], 1u) : atomicAdd(&H.bcnt[H.PB[h.i] + part[q]], 1u);
+          rank[q] = local ? atomicAdd(&hcnt[part[q]], 1u) : atomicAdd(&H.bcur[H.PB[h.i] + part[q]], 1u);
         }
       }
     }
     __syncthreads();
     if (local)
       for (uint32_t b = threadIdx.x; b < h.np; b += 256)
-        if (hcnt[b]) hbase[b] = atomicAdd(&H.bcnt[H.PB[h.i] + b], hcnt[b]);
+        if (hcnt[b]) hbase[b] = atomicAdd(&H.bcur[H.PB[h.i] + b], hcnt[b]);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -2431,6 +2451,7 @@ static void ensure_hub_buffers(Ctx &c, Level &L, int32_t n_hubs) {
     }
     if (c.hp_bcnt.n < L.hub_parts + 1) {
       c.hp_bcnt.alloc(L.hub_parts + 1);
+      c.hp_bcur.alloc(L.hub_parts + 1);
       c.hp_boff.alloc(L.hub_parts + 1);
       c.hp_bm.alloc(L.hub_parts);
       c.hp_bd.alloc(L.hub_parts);
@@ -2548,6 +2569,7 @@ static uint32_t eval_batch(Ctx &c, Level &L, EvalArgs &a, const std::vector<uint
         H.AB = H.PB + (cnt + 1);
         H.tot = H.AB + (cnt + 1);
         H.bcnt = c.hp_bcnt.p;
+        H.bcur = c.hp_bcur.p;
         H.boff = c.hp_boff.p;
         H.mtmp = c.hp_mtmp.p;
         H.ekey = c.hp_ekey.p;
@@ -2572,14 +2594,11 @@ static uint32_t eval_batch(Ctx &c, Level &L, EvalArgs &a, const std::vector<uint
         t0();
         k_hubp_owners<<<grid_for((int64_t)cnt * kWave, 256), 256, 0, s>>>(a, H);
         t1(1);
-        fill_u32(H.bcnt, 0u, NB + 1, s);
-        fill_i64(H.own, 0, 2 * (int64_t)cnt, s);
-        fill_u32(H.ovf, 0u, cnt, s);
         t0();
         k_hubp_count<<<4096, 256, 0, s>>>(a, H);
         t1(2);
+        // partitions beyond this batch's hold stale counts: only boff[0..tot[1]] is read
         exclusive_scan_u32(H.bcnt, H.boff, NB + 1, nullptr, c.scratch, s);
-        fill_u32(H.bcnt, 0u, NB + 1, s);
         t0();
         k_hubp_scatter<<<4096, 256, 0, s>>>(a, H);
         t1(3);
