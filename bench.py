@@ -50,10 +50,10 @@ def nmi(a, b):
     return float(2 * mi / (h(ca) + h(cb)))
 
 
-def run_path(graph, precision, steps, warmup, args):
+def run_path(graph, precision, steps, warmup, args, pull=None):
     from relaxmap_amd import Network
     net = Network(precision=precision, batches=args.batches)
-    if args.fp64_pull:
+    if args.fp64_pull if pull is None else pull:
         net.set_fp64_flow(False)
     t = time.perf_counter()
     graph.upload_to(net)
@@ -433,6 +433,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="only the FP64 path (for its kernel trace); value is then the FP64 path's")
+    ap.add_argument("--no-fp64-pull", action="store_true", help="skip the short FP64 run with the in-CSR pull steady state")
     ap.add_argument("--fp64-pull", action="store_true",
                     help="the FP64 steady state by the in-CSR pull instead of the blocked push / pull")
     ap.add_argument("--no-profile", action="store_true")
@@ -471,6 +472,11 @@ def main():
         p_fp64 = net_d.flows()
         part_d = net_d.partition()
         net_d.close()
+    rec_p = None
+    if not args.no_fp64 and not args.fp64_pull and not args.no_fp64_pull:
+        # the FP64 path with the in-CSR pull steady state (the plain CSR pull SpMV): a short run beside it
+        net_p, rec_p = run_path(g, "fp64", 2, 1, args, pull=True)
+        net_p.close()
 
     sweep_s = sum(r["sweep_s"] for r in rec_m)
     visits = sum(r["edge_visits"] for r in rec_m)
@@ -500,6 +506,12 @@ def main():
                 "flow_s": rec_d[-1]["initiate_s"], "partition_s": rec_d[-1]["partition_s"]}
         out["fp64"] = fp64
         out["speedup_mixed_vs_fp64"] = fp64["time_to_converged_s"] / mixed["time_to_converged_s"]
+        if rec_p:
+            tp = sum(r["step_s"] for r in rec_p) / len(rec_p)
+            out["fp64_pull"] = {"time_to_converged_s": tp, "flow_s": rec_p[-1]["initiate_s"],
+                                "flow_iterations": rec_p[-1]["iterations"], "steps": len(rec_p),
+                                "codelength_bits": rec_p[-1]["codelength"] / math.log(2)}
+            out["speedup_mixed_vs_fp64_pull"] = tp / mixed["time_to_converged_s"]
         out["flow_l1_mixed_vs_fp64"] = float(np.abs(p_mixed - p_fp64).sum())
         out["codelength_rel_mixed_vs_fp64"] = (mixed["codelength_bits"] - fp64["codelength_bits"]) / fp64["codelength_bits"]
         if g.community[0] >= 0:
